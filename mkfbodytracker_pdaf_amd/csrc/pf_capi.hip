@@ -1,0 +1,1001 @@
+// pf_capi.hip -- C ABI of the gfx950 particle-filter hot path (include/pf_rbpf.h).
+//
+// The handle is one reference ParticleFilter (src/pf2DRao.h:13-31) with its
+// my_gmm model tables resident in HBM.  Host code here only validates,
+// converts layouts (row-major double <-> FP32 SoA packed) and launches; every
+// arithmetic step of the filter runs in the kernels of pf_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pf_rbpf.h"
+#include "pf_internal.hpp"
+#include "pf_kernels.hpp"
+
+using namespace mkfbt;
+
+namespace mkfbt {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+}  // namespace mkfbt
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_error(PF_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+    } while (0)
+
+struct pf_handle {
+    pf_config cfg{};
+    int N = 0, ld = 0;
+    int D = 0, K = 0, m = 0, J = 0, npose = 0, cap = 16;
+    bool model = false, tracks = false;
+    unsigned int clutter_mask = 0, order_packed = 0;
+    hipStream_t stream = nullptr;
+    // state (FP32 SoA, packed covariance), double buffered
+    float *xA = nullptr, *PA = nullptr, *xB = nullptr, *PB = nullptr;
+    int *anc = nullptr, *comp = nullptr, *bins = nullptr;
+    float *lw = nullptr;
+    // model
+    float *table = nullptr;
+    int table_floats = 0;
+    unsigned long long *thr = nullptr;
+    unsigned long long pi_total = 0;
+    float *mu = nullptr, *sig = nullptr;
+    std::vector<double> h_mu, h_sig, h_pi, h_gamma, h_H, h_BH, h_R;
+    // frame bookkeeping
+    FrameScratch *fs = nullptr;
+    int *frame = nullptr;
+    uint32_t *uni = nullptr;   // 2 words for the host-driven step
+    double *lml1 = nullptr;
+    // resampler scratch
+    unsigned long long *cloc = nullptr, *bsum = nullptr;
+    double *bexp = nullptr;
+    int Lcap = 0;
+    // staging for host-driven calls
+    float *zbuf = nullptr;
+    int *cntbuf = nullptr;
+    float *candbuf = nullptr;
+    double *est_partial = nullptr, *est_out = nullptr;
+    double *gbuf = nullptr;
+    size_t gbuf_bytes = 0;
+    int *ibuf = nullptr;
+    size_t ibuf_bytes = 0;
+    // timing of the last device-resident run
+    std::vector<hipEvent_t> ev;
+    int timed_launches = 0;
+    double last_step_ms = 0.0, last_frame_ms = 0.0;
+};
+
+namespace {
+
+int check_handle(pf_handle *h)
+{
+    if (!h) return set_error(PF_ERR_ARG, "null handle");
+    if (hipSetDevice(h->cfg.device) != hipSuccess) return set_error(PF_ERR_HIP, "hipSetDevice failed");
+    return PF_OK;
+}
+
+template <typename T>
+hipError_t dalloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1));
+}
+
+int ensure_resample_scratch(pf_handle *h, int L)
+{
+    if (L <= h->Lcap) return PF_OK;
+    (void)hipFree(h->cloc);
+    (void)hipFree(h->bsum);
+    (void)hipFree(h->bexp);
+    h->cloc = nullptr; h->bsum = nullptr; h->bexp = nullptr;
+    const int nb = (L + kWeightsBlock - 1) / kWeightsBlock;
+    HIP_TRY(dalloc(&h->cloc, (size_t)L));
+    HIP_TRY(dalloc(&h->bsum, (size_t)nb));
+    HIP_TRY(dalloc(&h->bexp, (size_t)nb));
+    h->Lcap = L;
+    return PF_OK;
+}
+
+int ensure_gbuf(pf_handle *h, size_t bytes)
+{
+    if (bytes <= h->gbuf_bytes) return PF_OK;
+    (void)hipFree(h->gbuf);
+    h->gbuf = nullptr;
+    HIP_TRY(hipMalloc((void **)&h->gbuf, bytes));
+    h->gbuf_bytes = bytes;
+    return PF_OK;
+}
+
+int ensure_ibuf(pf_handle *h, size_t bytes)
+{
+    if (bytes <= h->ibuf_bytes) return PF_OK;
+    (void)hipFree(h->ibuf);
+    h->ibuf = nullptr;
+    HIP_TRY(hipMalloc((void **)&h->ibuf, bytes));
+    h->ibuf_bytes = bytes;
+    return PF_OK;
+}
+
+int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(PF_ERR_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
+    return PF_OK;
+}
+
+// ---------------------------------------------------------- kernel dispatch
+template <int D>
+int launch_step_d(pf_handle *h, const StepArgs &a, int mode, int block)
+{
+    const size_t lds = sizeof(float) * (size_t)(4 * h->K + h->table_floats);
+    const int grid = (h->N + block - 1) / block;
+    if (mode == kModeRef)
+        hipLaunchKernelGGL((k_step<D, kModeRef>), dim3(grid), dim3(block), lds, h->stream, a);
+    else
+        hipLaunchKernelGGL((k_step<D, kModeCand>), dim3(grid), dim3(block), lds, h->stream, a);
+    return check_launch("k_step");
+}
+
+int launch_step(pf_handle *h, const StepArgs &a, int mode)
+{
+    int block = h->cfg.block_size > 0 ? h->cfg.block_size : 256;
+    switch (h->D) {
+    case 8: return launch_step_d<8>(h, a, mode, block);
+    case 12: return launch_step_d<12>(h, a, mode, block);
+    case 16: return launch_step_d<16>(h, a, mode, block);
+    default: return set_error(PF_ERR_UNSUPPORTED, "state dimension %d not compiled", h->D);
+    }
+}
+
+template <template <int> class F, typename... Args>
+int dispatch_d(int D, Args... args)
+{
+    switch (D) {
+    case 8: F<8>::run(args...); return PF_OK;
+    case 12: F<12>::run(args...); return PF_OK;
+    case 16: F<16>::run(args...); return PF_OK;
+    default: return set_error(PF_ERR_UNSUPPORTED, "state dimension %d not compiled", D);
+    }
+}
+
+template <int D>
+struct ResetL {
+    static void run(pf_handle *h)
+    {
+        hipLaunchKernelGGL((k_reset<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->bins,
+                           h->mu, h->sig, h->xA, h->PA, h->N, h->ld);
+    }
+};
+
+template <int D>
+struct GatherL {
+    static void run(pf_handle *h)
+    {
+        hipLaunchKernelGGL((k_gather<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->xA,
+                           h->PA, h->anc, h->xB, h->PB, h->N, h->ld);
+    }
+};
+
+template <int D>
+struct EstimateL {
+    static void run(pf_handle *h)
+    {
+        hipLaunchKernelGGL((k_estimate<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->xA,
+                           h->anc, h->N, h->ld, h->est_partial, h->fs, h->est_out);
+    }
+};
+
+// Resampling of the step's FP32 log-weights: weights + ancestors.
+int launch_resample_step(pf_handle *h, const uint32_t *uni, double *lml)
+{
+    WeightArgs w{};
+    w.lw32 = h->lw;
+    w.cloc = h->cloc;
+    w.bsum = h->bsum;
+    w.bexp = h->bexp;
+    w.fs = h->fs;
+    w.frame = h->frame;
+    w.uni = uni;
+    w.lml = lml;
+    w.L = h->N;
+    w.N = h->N;
+    w.ustride = 2;
+    w.uoff = 1;
+    w.advance = 1;
+    const int nb = (h->N + kWeightsBlock - 1) / kWeightsBlock;
+    hipLaunchKernelGGL((k_weights<kLogF32>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, w);
+    int st = check_launch("k_weights");
+    if (st) return st;
+    AncArgs an{h->cloc, h->bsum, h->fs, h->anc, h->N, h->N, nb};
+    hipLaunchKernelGGL(k_ancestors, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, an);
+    return check_launch("k_ancestors");
+}
+
+StepArgs step_args(pf_handle *h, int mode, const uint32_t *uni, const float *Z, long long zstride,
+                   int zld, const int *count, const float *cand, int *comp_out)
+{
+    StepArgs a{};
+    a.xin = h->xA;
+    a.Pin = h->PA;
+    a.xout = h->xB;
+    a.Pout = h->PB;
+    a.anc = h->anc;
+    a.lw = h->lw;
+    a.comp_out = comp_out;
+    a.fs = h->fs;
+    a.table = h->table;
+    a.thr = h->thr;
+    a.frame = h->frame;
+    a.uni = uni;
+    a.Z = Z;
+    a.z_frame_stride = zstride;
+    a.zld = zld;
+    a.count = count;
+    a.cand = cand;
+    a.pi_total = h->pi_total;
+    a.table_floats = h->table_floats;
+    a.K = h->K;
+    a.N = h->N;
+    a.ld = h->ld;
+    a.m = h->m;
+    a.J = h->J;
+    a.cap = h->cap;
+    a.order_packed = h->order_packed;
+    a.clutter_mask = h->clutter_mask;
+    const double pg = 1.0 - std::exp(-0.5 * h->cfg.gate_chi2);
+    a.pdl = (float)(h->cfg.p_detect / h->cfg.clutter_density);
+    a.b0 = (float)(1.0 - h->cfg.p_detect * pg);
+    a.gate = (float)h->cfg.gate_chi2;
+    (void)mode;
+    return a;
+}
+
+void swap_state(pf_handle *h)
+{
+    std::swap(h->xA, h->xB);
+    std::swap(h->PA, h->PB);
+}
+
+int read_status(pf_handle *h, int *degenerate)
+{
+    FrameScratch fs;
+    HIP_TRY(hipMemcpyAsync(&fs, h->fs, sizeof(fs), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *degenerate = (fs.status & 1u) ? 1 : 0;
+    if (fs.status) {
+        const unsigned int zero = 0;
+        HIP_TRY(hipMemcpyAsync(&h->fs->status, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    return PF_OK;
+}
+
+// Packed-SoA FP32 <-> row-major double conversions on the host.
+void pack_tracks(const pf_handle *h, const double *x, const double *P, std::vector<float> &xs,
+                 std::vector<float> &Ps)
+{
+    const int D = h->D, NP = npacked(D), N = h->N, ld = h->ld;
+    xs.assign((size_t)D * ld, 0.f);
+    Ps.assign((size_t)NP * ld, 0.f);
+    for (int j = 0; j < N; ++j) {
+        for (int i = 0; i < D; ++i) xs[(size_t)i * ld + j] = (float)x[(size_t)j * D + i];
+        for (int i = 0; i < D; ++i)
+            for (int k = i; k < D; ++k)
+                Ps[(size_t)pk(D, i, k) * ld + j] =
+                    (float)(0.5 * (P[(size_t)j * D * D + i * D + k] + P[(size_t)j * D * D + k * D + i]));
+    }
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+void pf_config_default(pf_config *cfg)
+{
+    if (!cfg) return;
+    cfg->n_particles = 500;
+    cfg->device = 0;
+    cfg->cand_capacity = 16;
+    cfg->block_size = 0;
+    cfg->p_detect = 0.95;
+    cfg->gate_chi2 = 9.210340371976184;  // chi-square 2 dof, 99%
+    cfg->clutter_density = 3.0 / (640.0 * 480.0);
+}
+
+const char *pf_last_error(void) { return g_last_error.c_str(); }
+
+int pf_create(pf_handle **out, const pf_config *cfg)
+{
+    if (!out) return set_error(PF_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    pf_config c;
+    pf_config_default(&c);
+    if (cfg) c = *cfg;
+    if (c.n_particles <= 0 || c.n_particles > (1 << 26))
+        return set_error(PF_ERR_ARG, "n_particles %d out of range (1..2^26)", c.n_particles);
+    if (c.cand_capacity <= 0) c.cand_capacity = 16;
+    if (c.cand_capacity > 256) return set_error(PF_ERR_ARG, "cand_capacity %d > 256", c.cand_capacity);
+    if (c.block_size != 0 && (c.block_size % 64 != 0 || c.block_size > 256 || c.block_size < 64))
+        return set_error(PF_ERR_ARG, "block_size must be 64, 128, 192 or 256");
+    if (!(c.p_detect > 0.0 && c.p_detect <= 1.0)) return set_error(PF_ERR_ARG, "p_detect must be in (0,1]");
+    if (!(c.gate_chi2 > 0.0)) return set_error(PF_ERR_ARG, "gate_chi2 must be > 0");
+    if (!(c.clutter_density > 0.0)) return set_error(PF_ERR_ARG, "clutter_density must be > 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return set_error(PF_ERR_HIP, "no HIP device available");
+    if (c.device < 0 || c.device >= ndev) return set_error(PF_ERR_ARG, "device %d out of range", c.device);
+    HIP_TRY(hipSetDevice(c.device));
+    pf_handle *h = new pf_handle();
+    h->cfg = c;
+    h->N = c.n_particles;
+    h->ld = (c.n_particles + 63) & ~63;
+    h->cap = c.cand_capacity;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return set_error(PF_ERR_HIP, "hipStreamCreate failed");
+    }
+    hipError_t e = hipSuccess;
+    e = e ? e : dalloc(&h->fs, 1);
+    e = e ? e : dalloc(&h->frame, 1);
+    e = e ? e : dalloc(&h->uni, 2);
+    e = e ? e : dalloc(&h->lml1, 1);
+    e = e ? e : dalloc(&h->anc, (size_t)h->ld);
+    e = e ? e : dalloc(&h->comp, (size_t)h->ld);
+    e = e ? e : dalloc(&h->bins, (size_t)h->ld);
+    e = e ? e : dalloc(&h->lw, (size_t)h->ld);
+    e = e ? e : dalloc(&h->est_out, 64);
+    e = e ? e : dalloc(&h->est_partial, (size_t)((h->N + 255) / 256) * 64);
+    if (e == hipSuccess) e = hipMemsetAsync(h->fs, 0, sizeof(FrameScratch), h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        pf_destroy(h);
+        return set_error(PF_ERR_HIP, "device allocation failed: %s", hipGetErrorString(e));
+    }
+    if (ensure_resample_scratch(h, h->N) != PF_OK) {
+        pf_destroy(h);
+        return PF_ERR_HIP;
+    }
+    *out = h;
+    return PF_OK;
+}
+
+int pf_destroy(pf_handle *h)
+{
+    if (!h) return PF_OK;
+    (void)hipSetDevice(h->cfg.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void *ptrs[] = {h->xA, h->PA, h->xB, h->PB, h->anc, h->comp, h->bins, h->lw, h->table, h->thr,
+                    h->mu, h->sig, h->fs, h->frame, h->uni, h->lml1, h->cloc, h->bsum, h->bexp,
+                    h->zbuf, h->cntbuf, h->candbuf, h->est_partial, h->est_out, h->gbuf, h->ibuf};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PF_OK;
+}
+
+int pf_load_model(pf_handle *h, int32_t K, int32_t d, int32_t npose, int32_t m, const int32_t *sel,
+                  const double *means, const double *covs, const double *weights,
+                  const double *gamma, const double *pca_proj, const double *pca_mean,
+                  const double *R)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (K <= 0 || K > 256) return set_error(PF_ERR_ARG, "K=%d out of range (1..256)", K);
+    if (!(d == 8 || d == 12 || d == 16))
+        return set_error(PF_ERR_UNSUPPORTED, "state dimension %d not supported (8, 12, 16)", d);
+    static const int32_t ref_sel[6] = {9, 10, 0, 1, 12, 13};  // my_gmm.cpp:61-67
+    if (!sel) {
+        if (m != 6) return set_error(PF_ERR_ARG, "default selection needs m == 6");
+        sel = ref_sel;
+    }
+    if (m <= 0 || (m & 1) || m > 2 * kMaxJoints)
+        return set_error(PF_ERR_ARG, "m=%d must be even and <= %d", m, 2 * kMaxJoints);
+    if (!means || !covs || !weights || !gamma || !pca_proj || !pca_mean)
+        return set_error(PF_ERR_ARG, "null model array");
+    for (int a = 0; a < m; ++a)
+        if (sel[a] < 0 || sel[a] >= npose) return set_error(PF_ERR_ARG, "sel[%d]=%d out of range", a, sel[a]);
+    std::vector<double> Rm((size_t)m * m, 0.0);
+    if (R) std::copy(R, R + (size_t)m * m, Rm.begin());
+    else for (int i = 0; i < m; ++i) Rm[(size_t)i * m + i] = 100.0;  // my_gmm.cpp:54
+    for (int r = 0; r < m; ++r)
+        for (int c = 0; c < m; ++c)
+            if ((r / 2) != (c / 2) && Rm[(size_t)r * m + c] != 0.0)
+                return set_error(PF_ERR_UNSUPPORTED, "R must be block diagonal in 2x2 joints");
+    for (int b = 0; b < m / 2; ++b) {
+        const double r00 = Rm[(2 * b) * m + 2 * b], r01 = Rm[(2 * b) * m + 2 * b + 1],
+                     r10 = Rm[(2 * b + 1) * m + 2 * b], r11 = Rm[(2 * b + 1) * m + 2 * b + 1];
+        if (!(r00 > 0.0) || !(r00 * r11 - r01 * r10 > 0.0) || r01 != r10)
+            return set_error(PF_ERR_NOT_SPD, "R joint block %d not symmetric positive definite", b);
+    }
+    double wmax = 0.0;
+    for (int k = 0; k < K; ++k) {
+        if (!(weights[k] >= 0.0) || !std::isfinite(weights[k]))
+            return set_error(PF_ERR_ARG, "weight %d invalid", k);
+        wmax = std::max(wmax, weights[k]);
+        if (!(gamma[k] >= -1.0 && gamma[k] <= 1.0)) return set_error(PF_ERR_ARG, "gamma %d outside [-1,1]", k);
+    }
+    if (!(wmax > 0.0)) return set_error(PF_ERR_ARG, "all component weights are zero");
+    // Sigma_k must be SPD (the tracks start from them, my_gmm.cpp:36).
+    for (int k = 0; k < K; ++k) {
+        std::vector<double> L(covs + (size_t)k * d * d, covs + (size_t)(k + 1) * d * d);
+        for (int jj = 0; jj < d; ++jj) {
+            double s = L[jj * d + jj];
+            for (int q = 0; q < jj; ++q) s -= L[jj * d + q] * L[jj * d + q];
+            if (!(s > 0.0)) return set_error(PF_ERR_NOT_SPD, "covariance of component %d is not SPD", k);
+            const double l = std::sqrt(s);
+            L[jj * d + jj] = l;
+            for (int i = jj + 1; i < d; ++i) {
+                double t = L[i * d + jj];
+                for (int q = 0; q < jj; ++q) t -= L[i * d + q] * L[jj * d + q];
+                L[i * d + jj] = t / l;
+            }
+        }
+    }
+
+    const int NP = npacked(d), CS = 2 + d + NP, J = m / 2;
+    h->D = d; h->K = K; h->m = m; h->J = J; h->npose = npose;
+    h->h_mu.assign(means, means + (size_t)K * d);
+    h->h_sig.assign(covs, covs + (size_t)K * d * d);
+    h->h_pi.assign(weights, weights + K);
+    h->h_gamma.assign(gamma, gamma + K);
+    h->h_H.assign((size_t)m * d, 0.0);
+    h->h_BH.assign(m, 0.0);
+    for (int a = 0; a < m; ++a) {                       // H = H1 pca_proj^T, BH = H1 pca_mean^T
+        for (int j = 0; j < d; ++j) h->h_H[(size_t)a * d + j] = pca_proj[(size_t)j * npose + sel[a]];
+        h->h_BH[a] = pca_mean[sel[a]];
+    }
+    h->h_R = Rm;
+
+    std::vector<float> tab((size_t)K * CS + (size_t)m * d + m + 3 * J);
+    std::vector<float> mu((size_t)K * d), sg((size_t)K * NP);
+    for (int k = 0; k < K; ++k) {                       // my_gmm.cpp:51-58
+        const double g = gamma[k];
+        float *c = &tab[(size_t)k * CS];
+        c[0] = (float)g;
+        c[1] = (float)(g * g);
+        for (int i = 0; i < d; ++i) {
+            c[2 + i] = (float)((1.0 - g) * means[(size_t)k * d + i]);
+            mu[(size_t)k * d + i] = (float)means[(size_t)k * d + i];
+        }
+        for (int i = 0; i < d; ++i)
+            for (int q = i; q < d; ++q) {
+                const double s = 0.5 * (covs[(size_t)k * d * d + i * d + q] + covs[(size_t)k * d * d + q * d + i]);
+                c[2 + d + pk(d, i, q)] = (float)((1.0 - g * g) * s);
+                sg[(size_t)k * NP + pk(d, i, q)] = (float)s;
+            }
+    }
+    float *Hs = &tab[(size_t)K * CS];
+    for (int i = 0; i < m * d; ++i) Hs[i] = (float)h->h_H[i];
+    for (int a = 0; a < m; ++a) Hs[m * d + a] = (float)h->h_BH[a];
+    for (int b = 0; b < J; ++b) {
+        Hs[m * d + m + 3 * b] = (float)Rm[(2 * b) * m + 2 * b];
+        Hs[m * d + m + 3 * b + 1] = (float)Rm[(2 * b) * m + 2 * b + 1];
+        Hs[m * d + m + 3 * b + 2] = (float)Rm[(2 * b + 1) * m + 2 * b + 1];
+    }
+    // Component thresholds for resample(gmm.weight, N) (pf2DRao.cpp:128).
+    std::vector<unsigned long long> thr(2 * (size_t)K);
+    unsigned long long acc = 0;
+    for (int k = 0; k < K; ++k) {
+        acc += fixed32(weights[k] / wmax);
+        const u128 th = (u128)acc * ((u128)(unsigned long long)h->N << 32);
+        thr[2 * k] = (unsigned long long)th;
+        thr[2 * k + 1] = (unsigned long long)(th >> 64);
+    }
+    h->pi_total = acc;
+    const size_t lds = sizeof(float) * (size_t)(4 * K + tab.size());
+    if (lds > 64 * 1024)
+        return set_error(PF_ERR_UNSUPPORTED, "model table of %zu bytes exceeds the 64 KiB LDS budget", lds);
+
+    // (re)allocate device buffers sized by the model
+    (void)(void)hipFree(h->table); (void)hipFree(h->thr); (void)hipFree(h->mu); (void)hipFree(h->sig);
+    (void)(void)hipFree(h->xA); (void)hipFree(h->PA); (void)hipFree(h->xB); (void)hipFree(h->PB);
+    h->table = nullptr; h->thr = nullptr; h->mu = nullptr; h->sig = nullptr;
+    h->xA = h->PA = h->xB = h->PB = nullptr;
+    (void)hipFree(h->zbuf); (void)hipFree(h->cntbuf); (void)hipFree(h->candbuf);
+    h->zbuf = nullptr; h->cntbuf = nullptr; h->candbuf = nullptr;
+    h->model = false;
+    h->tracks = false;
+    HIP_TRY(dalloc(&h->table, tab.size()));
+    HIP_TRY(dalloc(&h->thr, thr.size()));
+    HIP_TRY(dalloc(&h->mu, mu.size()));
+    HIP_TRY(dalloc(&h->sig, sg.size()));
+    HIP_TRY(dalloc(&h->xA, (size_t)d * h->ld));
+    HIP_TRY(dalloc(&h->xB, (size_t)d * h->ld));
+    HIP_TRY(dalloc(&h->PA, (size_t)NP * h->ld));
+    HIP_TRY(dalloc(&h->PB, (size_t)NP * h->ld));
+    HIP_TRY(hipMemcpyAsync(h->table, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->thr, thr.data(), sizeof(unsigned long long) * thr.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->mu, mu.data(), sizeof(float) * mu.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->sig, sg.data(), sizeof(float) * sg.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->table_floats = (int)tab.size();
+    h->clutter_mask = 0;
+    h->order_packed = 0;
+    for (int b = 0; b < J; ++b) h->order_packed |= (unsigned)b << (4 * b);
+    h->model = true;
+    return PF_OK;
+}
+
+int pf_set_joint_clutter(pf_handle *h, const int32_t *clutter)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!h->model) return set_error(PF_ERR_STATE, "load a model first");
+    unsigned mask = 0;
+    for (int b = 0; b < h->J; ++b)
+        if (clutter && clutter[b]) mask |= 1u << b;
+    // certain joints first, then PDAF joints, each in index order
+    unsigned order = 0;
+    int slot = 0;
+    for (int b = 0; b < h->J; ++b)
+        if (!((mask >> b) & 1u)) order |= (unsigned)b << (4 * slot++);
+    for (int b = 0; b < h->J; ++b)
+        if ((mask >> b) & 1u) order |= (unsigned)b << (4 * slot++);
+    h->clutter_mask = mask;
+    h->order_packed = order;
+    return PF_OK;
+}
+
+int pf_reset(pf_handle *h, const int32_t *bins)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!h->model) return set_error(PF_ERR_STATE, "load a model first");
+    if (!bins) return set_error(PF_ERR_ARG, "null bins");
+    for (int j = 0; j < h->N; ++j)
+        if (bins[j] < 0 || bins[j] >= h->K) return set_error(PF_ERR_ARG, "bins[%d]=%d out of range", j, bins[j]);
+    HIP_TRY(hipMemcpyAsync(h->bins, bins, sizeof(int) * h->N, hipMemcpyHostToDevice, h->stream));
+    st = dispatch_d<ResetL>(h->D, h);
+    if (st) return st;
+    hipLaunchKernelGGL(k_iota, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->anc, h->N);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->tracks = true;
+    return PF_OK;
+}
+
+int pf_set_tracks(pf_handle *h, const double *x, const double *P)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!h->model) return set_error(PF_ERR_STATE, "load a model first");
+    if (!x || !P) return set_error(PF_ERR_ARG, "null tracks");
+    std::vector<float> xs, Ps;
+    pack_tracks(h, x, P, xs, Ps);
+    HIP_TRY(hipMemcpyAsync(h->xA, xs.data(), sizeof(float) * xs.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->PA, Ps.data(), sizeof(float) * Ps.size(), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_iota, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->anc, h->N);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->tracks = true;
+    return PF_OK;
+}
+
+int pf_get_tracks(pf_handle *h, double *x, double *P)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!h->tracks) return set_error(PF_ERR_STATE, "tracks not initialised");
+    const int D = h->D, NP = npacked(D), N = h->N, ld = h->ld;
+    st = dispatch_d<GatherL>(D, h);
+    if (st) return st;
+    std::vector<float> xs((size_t)D * ld), Ps((size_t)NP * ld);
+    HIP_TRY(hipMemcpyAsync(xs.data(), h->xB, sizeof(float) * xs.size(), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(Ps.data(), h->PB, sizeof(float) * Ps.size(), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int j = 0; j < N; ++j) {
+        if (x)
+            for (int i = 0; i < D; ++i) x[(size_t)j * D + i] = xs[(size_t)i * ld + j];
+        if (P)
+            for (int i = 0; i < D; ++i)
+                for (int k = 0; k < D; ++k) P[(size_t)j * D * D + i * D + k] = Ps[(size_t)pk(D, i, k) * ld + j];
+    }
+    return PF_OK;
+}
+
+static int finish_host_step(pf_handle *h, float *lw, int32_t *anc, int32_t *comp, double *lml)
+{
+    int st = launch_resample_step(h, h->uni, h->lml1);
+    if (st) return st;
+    swap_state(h);
+    if (lw) HIP_TRY(hipMemcpyAsync(lw, h->lw, sizeof(float) * h->N, hipMemcpyDeviceToHost, h->stream));
+    if (anc) HIP_TRY(hipMemcpyAsync(anc, h->anc, sizeof(int) * h->N, hipMemcpyDeviceToHost, h->stream));
+    if (comp) HIP_TRY(hipMemcpyAsync(comp, h->comp, sizeof(int) * h->N, hipMemcpyDeviceToHost, h->stream));
+    if (lml) HIP_TRY(hipMemcpyAsync(lml, h->lml1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    int deg = 0;
+    st = read_status(h, &deg);
+    if (st) return st;
+    return deg ? PF_WARN_DEGENERATE : PF_OK;
+}
+
+int pf_step(pf_handle *h, const double *Z, uint32_t u_comp, uint32_t u_res, float *lw, int32_t *anc,
+            int32_t *comp, double *lml)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!h->tracks) return set_error(PF_ERR_STATE, "tracks not initialised (pf_reset or pf_set_tracks)");
+    if (!Z) return set_error(PF_ERR_ARG, "null measurement");
+    const int m = h->m, N = h->N;
+    if (!h->zbuf) HIP_TRY(dalloc(&h->zbuf, (size_t)m * N));
+    std::vector<float> zf((size_t)m * N);
+    for (size_t i = 0; i < zf.size(); ++i) zf[i] = (float)Z[i];
+    const uint32_t u[2] = {u_comp, u_res};
+    HIP_TRY(hipMemcpyAsync(h->zbuf, zf.data(), sizeof(float) * zf.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->uni, u, sizeof(u), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
+    StepArgs a = step_args(h, kModeRef, h->uni, h->zbuf, 0, N, nullptr, nullptr, comp ? h->comp : nullptr);
+    st = launch_step(h, a, kModeRef);
+    if (st) return st;
+    return finish_host_step(h, lw, anc, comp, lml);
+}
+
+int pf_step_candidates(pf_handle *h, const int32_t *count, const double *cand, uint32_t u_comp,
+                       uint32_t u_res, float *lw, int32_t *anc, int32_t *comp, double *lml)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!h->tracks) return set_error(PF_ERR_STATE, "tracks not initialised (pf_reset or pf_set_tracks)");
+    if (!count || !cand) return set_error(PF_ERR_ARG, "null candidates");
+    const int J = h->J, cap = h->cap;
+    for (int b = 0; b < J; ++b)
+        if (count[b] < 0 || count[b] > cap) return set_error(PF_ERR_ARG, "count[%d]=%d outside 0..%d", b, count[b], cap);
+    if (!h->cntbuf) HIP_TRY(dalloc(&h->cntbuf, (size_t)J));
+    if (!h->candbuf) HIP_TRY(dalloc(&h->candbuf, (size_t)J * cap * 2));
+    std::vector<float> cf((size_t)J * cap * 2);
+    for (size_t i = 0; i < cf.size(); ++i) cf[i] = (float)cand[i];
+    const uint32_t u[2] = {u_comp, u_res};
+    HIP_TRY(hipMemcpyAsync(h->cntbuf, count, sizeof(int) * J, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->candbuf, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->uni, u, sizeof(u), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
+    StepArgs a = step_args(h, kModeCand, h->uni, nullptr, 0, 0, h->cntbuf, h->candbuf, comp ? h->comp : nullptr);
+    st = launch_step(h, a, kModeCand);
+    if (st) return st;
+    return finish_host_step(h, lw, anc, comp, lml);
+}
+
+int pf_estimate(pf_handle *h, double *xbar)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!h->tracks) return set_error(PF_ERR_STATE, "tracks not initialised");
+    if (!xbar) return set_error(PF_ERR_ARG, "null output");
+    st = dispatch_d<EstimateL>(h->D, h);
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(xbar, h->est_out, sizeof(double) * h->D, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PF_OK;
+}
+
+static int resample_generic(pf_handle *h, const double *w, int L, int N, uint32_t u, int32_t *out, int kind)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!w || !out || L <= 0 || N <= 0 || L > (1 << 26) || N > (1 << 26))
+        return set_error(PF_ERR_ARG, "bad resample arguments (L=%d, N=%d)", L, N);
+    st = ensure_resample_scratch(h, L);
+    if (st) return st;
+    const size_t wbytes = sizeof(double) * (size_t)L;
+    st = ensure_gbuf(h, wbytes);
+    if (st) return st;
+    st = ensure_ibuf(h, sizeof(int) * (size_t)N + 64);
+    if (st) return st;
+    int *dout = h->ibuf;
+    uint32_t *duni = reinterpret_cast<uint32_t *>(h->ibuf + N);
+    const uint32_t uu[2] = {u, u};
+    HIP_TRY(hipMemcpyAsync(h->gbuf, w, wbytes, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(duni, uu, sizeof(uu), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
+    const int nbm = (L + 255) / 256;
+    if (kind == kLogF64)
+        hipLaunchKernelGGL((k_blockmax64<kLogF64>), dim3(nbm), dim3(256), 0, h->stream, h->gbuf, L, h->fs);
+    else
+        hipLaunchKernelGGL((k_blockmax64<kLinF64>), dim3(nbm), dim3(256), 0, h->stream, h->gbuf, L, h->fs);
+    WeightArgs wa{};
+    wa.w64 = h->gbuf;
+    wa.cloc = h->cloc;
+    wa.bsum = h->bsum;
+    wa.bexp = h->bexp;
+    wa.fs = h->fs;
+    wa.frame = h->frame;
+    wa.uni = duni;
+    wa.lml = nullptr;
+    wa.L = L;
+    wa.N = N;
+    wa.ustride = 2;
+    wa.uoff = 1;
+    wa.advance = 0;
+    const int nb = (L + kWeightsBlock - 1) / kWeightsBlock;
+    if (kind == kLogF64)
+        hipLaunchKernelGGL((k_weights<kLogF64>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, wa);
+    else
+        hipLaunchKernelGGL((k_weights<kLinF64>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, wa);
+    st = check_launch("k_weights");
+    if (st) return st;
+    AncArgs an{h->cloc, h->bsum, h->fs, dout, L, N, nb};
+    hipLaunchKernelGGL(k_ancestors, dim3((N + 255) / 256), dim3(256), 0, h->stream, an);
+    st = check_launch("k_ancestors");
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(out, dout, sizeof(int) * (size_t)N, hipMemcpyDeviceToHost, h->stream));
+    int deg = 0;
+    st = read_status(h, &deg);
+    if (st) return st;
+    return deg ? PF_WARN_DEGENERATE : PF_OK;
+}
+
+int pf_resample(pf_handle *h, const double *w, int32_t L, int32_t N, uint32_t u, int32_t *out)
+{
+    return resample_generic(h, w, L, N, u, out, kLinF64);
+}
+
+int pf_resample_log(pf_handle *h, const double *lw, int32_t L, int32_t N, uint32_t u, int32_t *out)
+{
+    return resample_generic(h, lw, L, N, u, out, kLogF64);
+}
+
+static int samples_common(pf_handle *h, const double *Hp, const double *M, int npose, int n)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!h->tracks) return set_error(PF_ERR_STATE, "tracks not initialised");
+    if (!Hp || !M || npose < 2 || n <= 0) return set_error(PF_ERR_ARG, "bad sample arguments");
+    return dispatch_d<EstimateL>(h->D, h);
+}
+
+int pf_get_samples(pf_handle *h, const double *Hp, const double *M, int32_t npose, int32_t n,
+                   double scale, const double *normals, double *out)
+{
+    int st = samples_common(h, Hp, M, npose, n);
+    if (st) return st;
+    if (!normals || !out) return set_error(PF_ERR_ARG, "null normals/output");
+    const int D = h->D;
+    const size_t need = sizeof(double) * (size_t)(2 * D + 2 + 2 * (size_t)n + 2 * (size_t)n);
+    st = ensure_gbuf(h, need);
+    if (st) return st;
+    double *dh = h->gbuf, *dm = dh + 2 * D, *dn = dm + 2, *dout = dn + 2 * (size_t)n;
+    HIP_TRY(hipMemcpyAsync(dh, Hp, sizeof(double) * 2 * D, hipMemcpyHostToDevice, h->stream));  // rows 0,1
+    HIP_TRY(hipMemcpyAsync(dm, M, sizeof(double) * 2, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dn, normals, sizeof(double) * 2 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_samples, dim3((n + 255) / 256), dim3(256), 0, h->stream, dh, dm, h->est_out, D, dn,
+                       n, scale, dout);
+    st = check_launch("k_samples");
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(out, dout, sizeof(double) * 2 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PF_OK;
+}
+
+int pf_get_sample_prob(pf_handle *h, const double *Hp, const double *M, int32_t npose,
+                       const double *in1, const double *in2, int32_t n, double scale, double *w1,
+                       double *w2)
+{
+    int st = samples_common(h, Hp, M, npose, n);
+    if (st) return st;
+    if (!in1 || !in2 || !w1 || !w2) return set_error(PF_ERR_ARG, "null inputs/outputs");
+    const int D = h->D;
+    const size_t need = sizeof(double) * (size_t)(2 * D + 2 + 6 * (size_t)n);
+    st = ensure_gbuf(h, need);
+    if (st) return st;
+    double *dh = h->gbuf, *dm = dh + 2 * D, *d1 = dm + 2, *d2 = d1 + 2 * (size_t)n;
+    double *o1 = d2 + 2 * (size_t)n, *o2 = o1 + n;
+    HIP_TRY(hipMemcpyAsync(dh, Hp, sizeof(double) * 2 * D, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dm, M, sizeof(double) * 2, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(d1, in1, sizeof(double) * 2 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(d2, in2, sizeof(double) * 2 * (size_t)n, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_sample_prob, dim3((n + 255) / 256), dim3(256), 0, h->stream, dh, dm, h->est_out,
+                       D, d1, d2, n, scale, o1, o2);
+    st = check_launch("k_sample_prob");
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(w1, o1, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(w2, o2, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PF_OK;
+}
+
+// ------------------------------------------------ device-resident sequences
+static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *count,
+                      const float *cand, const uint32_t *uniforms, double *lml, int flags)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!h->tracks) return set_error(PF_ERR_STATE, "tracks not initialised (pf_reset or pf_set_tracks)");
+    if (T <= 0 || !uniforms || !lml) return set_error(PF_ERR_ARG, "bad run arguments");
+    const bool timing = (flags & 2) != 0;
+    const bool graph = (flags & 1) != 0 && !timing;
+    const long long zstride = (long long)h->m * h->N;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
+    st = check_launch("k_set_int");
+    if (st) return st;
+    if (timing) {
+        const size_t need = 2 * (size_t)T + 2;
+        while (h->ev.size() < need) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            h->ev.push_back(e);
+        }
+    }
+    auto one_frame = [&](int f) -> int {
+        StepArgs a = step_args(h, mode, uniforms, Z, zstride, h->N, count, cand, nullptr);
+        if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f], h->stream));
+        int s = launch_step(h, a, mode);
+        if (s) return s;
+        if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f + 1], h->stream));
+        s = launch_resample_step(h, uniforms, lml);
+        if (s) return s;
+        swap_state(h);
+        return PF_OK;
+    };
+    if (!graph) {
+        if (timing) HIP_TRY(hipEventRecord(h->ev[2 * T], h->stream));
+        for (int f = 0; f < T; ++f) {
+            st = one_frame(f);
+            if (st) return st;
+        }
+        if (timing) {
+            HIP_TRY(hipEventRecord(h->ev[2 * T + 1], h->stream));
+            HIP_TRY(hipEventSynchronize(h->ev[2 * T + 1]));
+            double sum = 0.0;
+            for (int f = 0; f < T; ++f) {
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, h->ev[2 * f], h->ev[2 * f + 1]));
+                sum += ms;
+            }
+            float tot = 0.f;
+            HIP_TRY(hipEventElapsedTime(&tot, h->ev[2 * T], h->ev[2 * T + 1]));
+            h->last_step_ms = sum;
+            h->last_frame_ms = tot;
+            h->timed_launches = T;
+        }
+        return PF_OK;
+    }
+    // Graph: two frames (A->B, B->A) captured once, replayed T/2 times.
+    const int pairs = T / 2;
+    if (pairs > 0) {
+        hipGraph_t g;
+        hipGraphExec_t ge;
+        HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int s1 = one_frame(0);
+        int s2 = s1 ? s1 : one_frame(1);
+        hipError_t ec = hipStreamEndCapture(h->stream, &g);
+        if (s2) return s2;
+        if (ec != hipSuccess) return set_error(PF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        for (int p = 0; p < pairs; ++p) HIP_TRY(hipGraphLaunch(ge, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+    }
+    if (T & 1) {
+        st = one_frame(T - 1);
+        if (st) return st;
+    }
+    return PF_OK;
+}
+
+int pf_run_ref_dev(pf_handle *h, int32_t T, const float *Z, const uint32_t *uniforms, double *lml,
+                   int32_t use_graph)
+{
+    if (!Z) return set_error(PF_ERR_ARG, "null Z");
+    return run_frames(h, T, kModeRef, Z, nullptr, nullptr, uniforms, lml, use_graph);
+}
+
+int pf_run_candidates_dev(pf_handle *h, int32_t T, const int32_t *count, const float *cand,
+                          const uint32_t *uniforms, double *lml, int32_t use_graph)
+{
+    if (!count || !cand) return set_error(PF_ERR_ARG, "null candidates");
+    return run_frames(h, T, kModeCand, nullptr, count, cand, uniforms, lml, use_graph);
+}
+
+int pf_synchronize(pf_handle *h)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PF_OK;
+}
+
+int pf_get_stream(pf_handle *h, void **stream)
+{
+    if (!h || !stream) return set_error(PF_ERR_ARG, "null argument");
+    *stream = (void *)h->stream;
+    return PF_OK;
+}
+
+int pf_last_run_kernel_time(pf_handle *h, double *step_ms, double *frame_ms, int32_t *launches)
+{
+    if (!h) return set_error(PF_ERR_ARG, "null handle");
+    if (step_ms) *step_ms = h->last_step_ms;
+    if (frame_ms) *frame_ms = h->last_frame_ms;
+    if (launches) *launches = h->timed_launches;
+    return PF_OK;
+}
+
+// -------------------------------------------------------------- KF_model
+int pf_kf_predict(pf_handle *h, int32_t n, int32_t d, const double *F, const double *B,
+                  const double *Q, double *x, double *P)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (n <= 0 || d <= 0 || d > 32 || !F || !B || !Q || !x || !P)
+        return set_error(PF_ERR_ARG, "bad KF predict arguments");
+    const size_t mats = (size_t)(2 * d * d + d);
+    const size_t states = (size_t)n * (d + d * d);
+    st = ensure_gbuf(h, sizeof(double) * (mats + states));
+    if (st) return st;
+    double *dF = h->gbuf, *dQ = dF + d * d, *dB = dQ + d * d, *dx = dB + d, *dP = dx + (size_t)n * d;
+    HIP_TRY(hipMemcpyAsync(dF, F, sizeof(double) * d * d, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dQ, Q, sizeof(double) * d * d, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dB, B, sizeof(double) * d, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dx, x, sizeof(double) * (size_t)n * d, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dP, P, sizeof(double) * (size_t)n * d * d, hipMemcpyHostToDevice, h->stream));
+    const size_t lds = sizeof(double) * (size_t)(2 * d * d + d);
+    hipLaunchKernelGGL(k_kf_predict, dim3(n), dim3(256), lds, h->stream, d, dF, dB, dQ, dx, dP);
+    st = check_launch("k_kf_predict");
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(x, dx, sizeof(double) * (size_t)n * d, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(P, dP, sizeof(double) * (size_t)n * d * d, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PF_OK;
+}
+
+int pf_kf_update(pf_handle *h, int32_t n, int32_t d, int32_t m, const double *H, const double *BH,
+                 const double *R, const double *z, double *x, double *P)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (n <= 0 || d <= 0 || d > 32 || m <= 0 || m > 16 || !H || !BH || !R || !z || !x || !P)
+        return set_error(PF_ERR_ARG, "bad KF update arguments");
+    const size_t mats = (size_t)(m * d + m + m * m);
+    const size_t states = (size_t)n * (m + d + d * d);
+    st = ensure_gbuf(h, sizeof(double) * (mats + states));
+    if (st) return st;
+    st = ensure_ibuf(h, sizeof(int) * 16);
+    if (st) return st;
+    double *dH = h->gbuf, *dBH = dH + m * d, *dR = dBH + m, *dz = dR + m * m, *dx = dz + (size_t)n * m;
+    double *dP = dx + (size_t)n * d;
+    HIP_TRY(hipMemcpyAsync(dH, H, sizeof(double) * m * d, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dBH, BH, sizeof(double) * m, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dR, R, sizeof(double) * m * m, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dz, z, sizeof(double) * (size_t)n * m, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dx, x, sizeof(double) * (size_t)n * d, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(dP, P, sizeof(double) * (size_t)n * d * d, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->ibuf, 0);
+    const size_t lds = sizeof(double) * (size_t)(d * d + 2 * m * d + m * m + d + m);
+    hipLaunchKernelGGL(k_kf_update, dim3(n), dim3(256), lds, h->stream, d, m, dH, dBH, dR, dz, dx, dP, h->ibuf);
+    st = check_launch("k_kf_update");
+    if (st) return st;
+    int flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, h->ibuf, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(x, dx, sizeof(double) * (size_t)n * d, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(P, dP, sizeof(double) * (size_t)n * d * d, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (flag) return set_error(PF_ERR_NOT_SPD, "innovation covariance not positive definite");
+    return PF_OK;
+}
+
+void pf_philox_u32(uint64_t seed, uint32_t stream, uint64_t first, int64_t count, uint32_t *out)
+{
+    philox_u32(seed, stream, first, count, out);
+}
+
+}  // extern "C"

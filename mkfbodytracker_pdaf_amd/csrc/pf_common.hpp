@@ -1,0 +1,69 @@
+// pf_common.hpp -- helpers shared by host and device code of the particle
+// filter library: packed symmetric indexing, the deterministic exp used by the
+// resampler, fixed-point weights and 128-bit threshold arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mkfbt {
+
+typedef unsigned __int128 u128;
+
+// Packed upper-triangular index of (i, j), i <= j, row-major, n x n.
+__host__ __device__ constexpr int pk(int n, int i, int j)
+{
+    return i <= j ? i * n - (i * (i - 1)) / 2 + (j - i) : j * n - (j * (j - 1)) / 2 + (i - j);
+}
+
+__host__ __device__ constexpr int npacked(int n) { return n * (n + 1) / 2; }
+
+// exp(x) for x <= 0 from IEEE-exact operations only (rint, fma, ldexp):
+// Cody-Waite reduction with the fdlibm split of ln 2 and the degree-12 Taylor
+// polynomial on |r| <= ln2/2.  Host and device builds return identical bits,
+// which makes the fixed-point resampling weights below reproducible anywhere.
+__host__ __device__ inline double det_exp(double x)
+{
+    if (!(x >= -745.0)) return 0.0;
+    if (x > 709.0) x = 709.0;
+    const double n = rint(x * 1.4426950408889634074);
+    double r = fma(-n, 6.93147180369123816490e-01, x);
+    r = fma(-n, 1.90821492927058770002e-10, r);
+    double p = 1.0 / 479001600.0;
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+
+// v in [0, 1] -> floor(v * 2^32), exactly.
+__host__ __device__ inline uint64_t fixed32(double v)
+{
+    if (!(v > 0.0)) return 0;
+    const double t = floor(ldexp(v, 32));
+    if (t >= 4294967296.0) return (uint64_t)1 << 32;
+    return (uint64_t)(uint32_t)t;
+}
+
+// Fixed-point weight of a log-weight relative to the maximum.
+__host__ __device__ inline uint64_t qlog(double lw, double lmax)
+{
+    return (isfinite(lw) && isfinite(lmax)) ? fixed32(det_exp(lw - lmax)) : 0;
+}
+
+// Systematic threshold of output i: (i * 2^32 + U) * total.
+__host__ __device__ inline u128 sys_target(uint32_t i, uint32_t U, uint64_t total)
+{
+    return (u128)((((uint64_t)i) << 32) | U) * total;
+}
+
+}  // namespace mkfbt

@@ -1,0 +1,274 @@
+"""ctypes bindings for the FP64 CPU oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY.  The oracle is the parity checker for the HIP
+implementation; only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module.
+
+Every wrapper mirrors one C function of ``oracle/pf_oracle.h``; the C file
+cites the reference lines each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    """Compile liboracle.so with the Makefile beside this file."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "liboracle.so"], check=True)
+
+
+class _Model(C.Structure):
+    _fields_ = [("K", C.c_int), ("d", C.c_int), ("m", C.c_int),
+                ("pi", C.c_void_p), ("gamma", C.c_void_p), ("mu", C.c_void_p),
+                ("Sigma", C.c_void_p), ("H", C.c_void_p), ("BH", C.c_void_p),
+                ("R", C.c_void_p)]
+
+
+class _Frame(C.Structure):
+    _fields_ = [("J", C.c_int), ("cap", C.c_int), ("clutter", C.c_void_p),
+                ("count", C.c_void_p), ("cand", C.c_void_p),
+                ("pd", C.c_double), ("gate", C.c_double), ("lam", C.c_double)]
+
+
+class _Pf2dGmm(C.Structure):
+    _fields_ = [("K", C.c_int), ("d", C.c_int), ("pi", C.c_void_p), ("mu", C.c_void_p),
+                ("Sinv", C.c_void_p), ("cnorm", C.c_void_p)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_philox4x32_10.argtypes = [_up, _up, _up]
+        L.orc_philox_u32.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_int64, _up]
+        L.orc_det_exp.argtypes = [C.c_double]
+        L.orc_det_exp.restype = C.c_double
+        L.orc_resample_linear.argtypes = [_dp, C.c_int, C.c_int, C.c_uint32, _ip]
+        L.orc_resample_log.argtypes = [_dp, C.c_int, C.c_int, C.c_uint32, _ip]
+        L.orc_resample_wheel.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_void_p, _ip]
+        L.orc_kf_predict.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_kf_update.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_mvn_logpdf.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_mvn_logpdf.restype = C.c_double
+        L.orc_load_gaussian.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp,
+                                        C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_reset_tracker.argtypes = [C.POINTER(_Model), C.c_int, _ip, _dp, _dp]
+        L.orc_rbpf_update_ref.argtypes = [C.POINTER(_Model), C.c_int, _dp, _dp, _dp,
+                                          C.c_uint32, C.c_uint32, _dp, _ip, _ip,
+                                          C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+        L.orc_rbpf_update_pdaf.argtypes = [C.POINTER(_Model), C.c_int, _dp, _dp, C.POINTER(_Frame),
+                                           C.c_uint32, C.c_uint32, _dp, _ip, _ip,
+                                           C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+        L.orc_rbpf_estimate.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_get_samples.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, C.c_double, _dp, _dp]
+        L.orc_get_sample_prob.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp, C.c_double, _dp]
+        L.orc_pf2d_load_gaussian.argtypes = [C.c_int, _dp, _dp, C.POINTER(C.c_double)]
+        L.orc_pf2d_init.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.orc_pf2d_update.argtypes = [C.POINTER(_Pf2dGmm), C.c_int, _dp, _dp, _dp, _ip, _dp,
+                                      C.c_uint32, _dp, _dp, C.c_int, C.c_int]
+        L.orc_pf2d_estimate.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ------------------------------------------------------------------ RNG
+def philox4x32_10(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(np.ascontiguousarray(ctr, np.uint32), np.ascontiguousarray(key, np.uint32), out)
+    return out
+
+
+def philox_u32(seed: int, stream: int, first: int, count: int):
+    out = np.zeros(count, np.uint32)
+    lib().orc_philox_u32(seed, stream, first, count, out)
+    return out
+
+
+def det_exp(x: float) -> float:
+    return lib().orc_det_exp(float(x))
+
+
+# ----------------------------------------------------------- resampling
+def resample_linear(w, N: int, U: int):
+    w = _f64(w)
+    out = np.zeros(N, np.int32)
+    st = lib().orc_resample_linear(w, w.size, N, int(U), out)
+    return out, st
+
+
+def resample_log(lw, N: int, U: int):
+    lw = _f64(lw)
+    out = np.zeros(N, np.int32)
+    st = lib().orc_resample_log(lw, lw.size, N, int(U), out)
+    return out, st
+
+
+def resample_wheel(w, N: int, u: float, fallback=None):
+    w = _f64(w)
+    out = np.zeros(N, np.int32)
+    fb = None if fallback is None else _i32(fallback)
+    st = lib().orc_resample_wheel(w, w.size, N, float(u), None if fb is None else fb.ctypes.data, out)
+    return out, st
+
+
+# -------------------------------------------------------------- KF_model
+def kf_predict(F, B, Q, x, P):
+    x = _f64(x).copy(); P = _f64(P).copy()
+    lib().orc_kf_predict(x.size, _f64(F), _f64(B), _f64(Q), x, P)
+    return x, P
+
+
+def kf_update(H, BH, R, z, x, P):
+    x = _f64(x).copy(); P = _f64(P).copy()
+    H = _f64(H)
+    st = lib().orc_kf_update(x.size, H.shape[0], H, _f64(BH), _f64(R), _f64(z), x, P)
+    if st != 0:
+        raise ArithmeticError("S singular")
+    return x, P
+
+
+def mvn_logpdf(x, u, S):
+    x = _f64(x)
+    return lib().orc_mvn_logpdf(x.size, x, _f64(u), _f64(S))
+
+
+def load_gaussian(u, s, pca_proj, pca_mean, g, sel, R):
+    d = len(u); npose = pca_proj.shape[1]; m = len(sel)
+    F = np.zeros((d, d)); B = np.zeros(d); Q = np.zeros((d, d))
+    H = np.zeros((m, d)); BH = np.zeros(m); Ro = np.zeros((m, m))
+    lib().orc_load_gaussian(d, npose, m, _i32(sel), _f64(u), _f64(s), _f64(pca_proj),
+                            _f64(pca_mean), float(g), _f64(R), F, B, Q, H, BH, Ro)
+    return dict(F=F, B=B, Q=Q, H=H, BH=BH, R=Ro)
+
+
+# ------------------------------------------------------------------ RBPF
+class Model:
+    """FP64 RBPF model (my_gmm + shared KF_model H/BH/R)."""
+
+    def __init__(self, pi, gamma, mu, Sigma, H, BH, R):
+        self.pi = _f64(pi); self.gamma = _f64(gamma); self.mu = _f64(mu)
+        self.Sigma = _f64(Sigma); self.H = _f64(H); self.BH = _f64(BH); self.R = _f64(R)
+        self.K, self.d = self.mu.shape
+        self.m = self.H.shape[0]
+        self._s = _Model(self.K, self.d, self.m, self.pi.ctypes.data, self.gamma.ctypes.data,
+                         self.mu.ctypes.data, self.Sigma.ctypes.data, self.H.ctypes.data,
+                         self.BH.ctypes.data, self.R.ctypes.data)
+
+    def reset(self, bins):
+        bins = _i32(bins)
+        N = bins.size
+        x = np.zeros((N, self.d)); P = np.zeros((N, self.d, self.d))
+        lib().orc_reset_tracker(C.byref(self._s), N, bins, x, P)
+        return x, P
+
+    def update_ref(self, x, P, Z, U1, U2):
+        """One ParticleFilter::update in reference mode.  Returns a dict."""
+        x = _f64(x).copy(); P = _f64(P).copy()
+        N = x.shape[0]
+        lw = np.zeros(N); anc = np.zeros(N, np.int32); comp = np.zeros(N, np.int32)
+        xu = np.zeros_like(x); Pu = np.zeros_like(P)
+        lml = C.c_double(0.0)
+        st = lib().orc_rbpf_update_ref(C.byref(self._s), N, x, P, _f64(Z), int(U1), int(U2),
+                                       lw, anc, comp, C.byref(lml), xu.ctypes.data, Pu.ctypes.data)
+        return dict(x=x, P=P, lw=lw, anc=anc, comp=comp, lml=lml.value, xu=xu, Pu=Pu, status=st)
+
+    def update_pdaf(self, x, P, clutter, count, cand, U1, U2, pd, gate, lam):
+        x = _f64(x).copy(); P = _f64(P).copy()
+        N = x.shape[0]
+        clutter = _i32(clutter); count = _i32(count); cand = _f64(cand)
+        J, cap = cand.shape[0], cand.shape[1]
+        fr = _Frame(J, cap, clutter.ctypes.data, count.ctypes.data, cand.ctypes.data,
+                    float(pd), float(gate), float(lam))
+        lw = np.zeros(N); anc = np.zeros(N, np.int32); comp = np.zeros(N, np.int32)
+        xu = np.zeros_like(x); Pu = np.zeros_like(P)
+        lml = C.c_double(0.0)
+        st = lib().orc_rbpf_update_pdaf(C.byref(self._s), N, x, P, C.byref(fr), int(U1), int(U2),
+                                        lw, anc, comp, C.byref(lml), xu.ctypes.data, Pu.ctypes.data)
+        if st < 0:
+            raise ValueError("bad arguments")
+        return dict(x=x, P=P, lw=lw, anc=anc, comp=comp, lml=lml.value, xu=xu, Pu=Pu, status=st)
+
+
+def rbpf_estimate(x):
+    x = _f64(x)
+    out = np.zeros(x.shape[1])
+    lib().orc_rbpf_estimate(x.shape[0], x.shape[1], x, out)
+    return out
+
+
+def get_samples(Hp, M, xbar, n, scale, normals):
+    Hp = _f64(Hp); out = np.zeros((2, n))
+    lib().orc_get_samples(Hp.shape[1], Hp.shape[0], Hp, _f64(M), _f64(xbar), n, float(scale), _f64(normals), out)
+    return out
+
+
+def get_sample_prob(Hp, M, xbar, inp, scale):
+    Hp = _f64(Hp); inp = _f64(inp); n = inp.shape[1]
+    w = np.zeros(n)
+    lib().orc_get_sample_prob(Hp.shape[1], Hp.shape[0], Hp, _f64(M), _f64(xbar), n, inp, float(scale), w)
+    return w
+
+
+# ------------------------------------------------------------------ pf2D
+def pf2d_load_gaussian(S):
+    S = _f64(S); d = S.shape[0]
+    Sinv = np.zeros((d, d)); c = C.c_double(0.0)
+    if lib().orc_pf2d_load_gaussian(d, S, Sinv, C.byref(c)) != 0:
+        raise ArithmeticError("not SPD")
+    return Sinv, c.value
+
+
+def pf2d_init(N, d, side, unif):
+    out = np.zeros((N, d))
+    lib().orc_pf2d_init(N, d, int(side), _f64(unif), out)
+    return out
+
+
+class Pf2dGmm:
+    def __init__(self, pi, mu, Sinv, cnorm):
+        self.pi = _f64(pi); self.mu = _f64(mu); self.Sinv = _f64(Sinv); self.cnorm = _f64(cnorm)
+        self.K, self.d = self.mu.shape
+        self._s = _Pf2dGmm(self.K, self.d, self.pi.ctypes.data, self.mu.ctypes.data,
+                           self.Sinv.ctypes.data, self.cnorm.ctypes.data)
+
+    def update(self, particles, meas, U, normals, reinit_unif, side, flags=0):
+        p = _f64(particles).copy()
+        N = p.shape[0]
+        w = np.zeros(N); lw = np.zeros(N); anc = np.zeros(N, np.int32)
+        st = lib().orc_pf2d_update(C.byref(self._s), N, p, w, lw, anc, _f64(meas), int(U),
+                                   _f64(normals), _f64(reinit_unif), int(side), int(flags))
+        return dict(particles=p, weights=w, lw=lw, anc=anc, status=st)
+
+
+def pf2d_estimate(particles, weights):
+    p = _f64(particles); out = np.zeros(p.shape[1])
+    lib().orc_pf2d_estimate(p.shape[0], p.shape[1], p, _f64(weights), out)
+    return out
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))

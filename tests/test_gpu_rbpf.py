@@ -1,0 +1,246 @@
+"""GPU parity of the RBPF hot path (pf2DRao / KF_model / my_gmm) against the
+FP64 oracle, through the C ABI.
+
+Tolerances (FP32 device state vs FP64 oracle):
+  * ancestors and component indicators: bit-exact on identical weights and uniforms;
+  * particle means and covariances: normwise relative error <= 1e-4 per particle;
+  * log-weights: |dlw| <= 1e-4 * max(1, |lw|) ... stated per test.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from mkfbodytracker_pdaf_amd import synthetic
+from mkfbodytracker_pdaf_amd.rbpf import Rbpf
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(N, seed=3, **model_kw):
+    mdl = synthetic.make_model(seed=seed, **model_kw)
+    rb = Rbpf(N, cand_capacity=8, clutter_density=3.0 / (640 * 480))
+    rb.load_model(mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean,
+                  sel=mdl.sel, R=mdl.R)
+    om = oracle.Model(mdl.weights, mdl.gamma, mdl.means, mdl.Sigma, mdl.H(), mdl.BH(), mdl.R)
+    return mdl, rb, om
+
+
+def _relerr(a, b):
+    n = a.shape[0]
+    a = a.reshape(n, -1); b = b.reshape(n, -1)
+    return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-30)
+
+
+# ------------------------------------------------------------------ resampling
+@pytest.mark.parametrize("L,N", [(1, 1), (1, 7), (7, 1), (15, 500), (500, 500), (1000, 333),
+                                 (4097, 4096), (100000, 100000), (257, 70000)])
+def test_resample_linear_bitexact(gpu, L, N):
+    rb = Rbpf(max(N, 1))
+    rng = np.random.default_rng(L * 7 + N)
+    for trial in range(3):
+        w = rng.random(L) ** 3
+        if trial == 1 and L > 4:
+            w[rng.random(L) < 0.5] = 0.0          # runs of zero weights
+        if trial == 2 and L > 4:
+            w[rng.integers(0, L)] = np.nan        # a non-finite weight counts as zero
+        U = int(rng.integers(0, 2 ** 32))
+        g, st = rb.resample(w, N, U)
+        o, sto = oracle.resample_linear(w, N, U)
+        assert st == sto
+        assert np.array_equal(g, o), (L, N, trial)
+        assert np.all(np.diff(g) >= 0)
+        assert np.all(np.nan_to_num(w, nan=0.0)[g] > 0)   # zero weights are never selected
+    rb.close()
+
+
+def test_resample_edge_cases(gpu):
+    rb = Rbpf(64)
+    w = np.zeros(10)
+    g, st = rb.resample(w, 20, 123)
+    o, sto = oracle.resample_linear(w, 20, 123)
+    assert st == 1 and sto == 1 and np.array_equal(g, o)      # degenerate -> uniform fallback
+    w = np.zeros(10); w[6] = 3.0
+    g, st = rb.resample(w, 20, 0)
+    assert st == 0 and np.all(g == 6)
+    for U in (0, 1, 2 ** 32 - 1):                              # u at both ends of [0, 1)
+        w = np.random.default_rng(U % 97).random(33)
+        g, _ = rb.resample(w, 50, U)
+        o, _ = oracle.resample_linear(w, 50, U)
+        assert np.array_equal(g, o)
+    rb.close()
+
+
+@pytest.mark.parametrize("L", [10, 1000, 65537])
+def test_resample_log_bitexact(gpu, L):
+    rb = Rbpf(L)
+    rng = np.random.default_rng(L)
+    lw = rng.normal(-50, 10, L)
+    lw[rng.random(L) < 0.1] = -np.inf
+    U = int(rng.integers(0, 2 ** 32))
+    g, st = rb.resample(lw, L, U, log=True)
+    o, sto = oracle.resample_log(lw, L, U)
+    assert st == sto == 0
+    assert np.array_equal(g, o)
+    rb.close()
+
+
+# ------------------------------------------------------------------ one step
+@pytest.mark.parametrize("N", [1, 64, 1000, 4099])
+def test_step_reference_mode_teacher_forced(gpu, N):
+    mdl, rb, om = _setup(N)
+    bins, _ = oracle.resample_linear(mdl.weights, N, 987654321)
+    x0, P0 = om.reset(bins)
+    rb.set_tracks(x0, P0)
+    seq = synthetic.simulate(mdl, 3, seed=11)
+    for t in range(3):
+        Z = synthetic.per_particle_measurements(seq.z[t], N, seed=t)
+        U1, U2 = int(seq.uniforms[t, 0]), int(seq.uniforms[t, 1])
+        st, out = rb.step(Z, U1, U2)
+        ref = om.update_ref(x0, P0, Z, U1, U2)
+        assert np.array_equal(out["comp"], ref["comp"])
+        # log-weights: FP32 vs FP64, absolute 1e-4 relative to |lw|
+        assert np.all(np.abs(out["lw"] - ref["lw"]) <= 1e-4 * np.maximum(1.0, np.abs(ref["lw"])))
+        # ancestors: bit-exact when the oracle resampler gets the GPU's weights
+        anc_o, _ = oracle.resample_log(out["lw"].astype(np.float64), N, U2)
+        assert np.array_equal(out["anc"], anc_o)
+        x, P = rb.get_tracks()
+        assert _relerr(x, ref["xu"][out["anc"]]).max() <= 1e-4
+        assert _relerr(P, ref["Pu"][out["anc"]]).max() <= 1e-4
+        assert abs(out["lml"] - ref["lml"]) <= 1e-4 * max(1.0, abs(ref["lml"]))
+        x0, P0 = x, P          # teacher forcing: next frame starts from the GPU tracks
+    rb.close()
+
+
+@pytest.mark.parametrize("N", [64, 2048])
+def test_step_candidate_mode_teacher_forced(gpu, N):
+    mdl, rb, om = _setup(N)
+    rb.set_joint_clutter(mdl.clutter)
+    bins, _ = oracle.resample_linear(mdl.weights, N, 5)
+    x0, P0 = om.reset(bins)
+    rb.set_tracks(x0, P0)
+    seq = synthetic.simulate(mdl, 4, seed=12, cand_per_joint=4, cap=8)
+    lam = 3.0 / (640 * 480)
+    for t in range(4):
+        U1, U2 = int(seq.uniforms[t, 0]), int(seq.uniforms[t, 1])
+        st, out = rb.step_candidates(seq.count[t], seq.cand[t], U1, U2)
+        ref = om.update_pdaf(x0, P0, mdl.clutter, seq.count[t], seq.cand[t], U1, U2,
+                             0.95, 9.210340371976184, lam)
+        assert np.array_equal(out["comp"], ref["comp"])
+        assert np.all(np.abs(out["lw"] - ref["lw"]) <= 1e-4 * np.maximum(1.0, np.abs(ref["lw"])))
+        anc_o, _ = oracle.resample_log(out["lw"].astype(np.float64), N, U2)
+        assert np.array_equal(out["anc"], anc_o)
+        x, P = rb.get_tracks()
+        assert _relerr(x, ref["xu"][out["anc"]]).max() <= 1e-4
+        assert _relerr(P, ref["Pu"][out["anc"]]).max() <= 1e-4
+        x0, P0 = x, P
+    rb.close()
+
+
+def test_candidate_mode_no_detections(gpu):
+    """A cluttered joint with no detection leaves the state at its prediction."""
+    N = 128
+    mdl, rb, om = _setup(N)
+    rb.set_joint_clutter([1, 1, 1])
+    bins, _ = oracle.resample_linear(mdl.weights, N, 5)
+    x0, P0 = om.reset(bins)
+    rb.set_tracks(x0, P0)
+    count = np.zeros(3, np.int32)
+    cand = np.zeros((3, 8, 2))
+    st, out = rb.step_candidates(count, cand, 1, 2)
+    ref = om.update_pdaf(x0, P0, [1, 1, 1], count, cand, 1, 2, 0.95, 9.210340371976184, 3.0 / (640 * 480))
+    pg = 1 - np.exp(-0.5 * 9.210340371976184)
+    assert np.allclose(out["lw"], 3 * np.log(1 - 0.95 * pg), rtol=1e-5)
+    x, P = rb.get_tracks()
+    assert _relerr(x, ref["xu"][out["anc"]]).max() <= 1e-5
+    rb.close()
+
+
+# ---------------------------------------------------------- free-running run
+def test_free_running_config1(gpu):
+    """Plain method, one sequence of 300 frames, N=1000 (reference mode):
+    identical to the oracle until the first ancestor divergence, estimates
+    close afterwards."""
+    N, T = 1000, 300
+    mdl, rb, om = _setup(N)
+    bins, _ = oracle.resample_linear(mdl.weights, N, 42)
+    x0, P0 = om.reset(bins)
+    rb.set_tracks(x0, P0)
+    xo, Po = x0, P0
+    seq = synthetic.simulate(mdl, T, seed=21)
+    first_div = None
+    est_err = []
+    for t in range(T):
+        Z = synthetic.per_particle_measurements(seq.z[t], N, seed=100 + t)
+        U1, U2 = int(seq.uniforms[t, 0]), int(seq.uniforms[t, 1])
+        st, out = rb.step(Z, U1, U2)
+        ref = om.update_ref(xo, Po, Z, U1, U2)
+        xo, Po = ref["x"], ref["P"]
+        if first_div is None and not np.array_equal(out["anc"], ref["anc"]):
+            first_div = t
+        if first_div is None:
+            x, P = rb.get_tracks()
+            assert _relerr(x, xo).max() <= 1e-4
+            assert _relerr(P, Po).max() <= 1e-4
+        eg = rb.estimate()
+        eo = oracle.rbpf_estimate(xo)
+        est_err.append(np.abs(mdl.H() @ (eg - eo)).max())   # pixel-space estimate difference
+    print("first ancestor divergence at frame", first_div)
+    assert first_div is None or first_div >= 5
+    # after divergence both are valid filters of the same data: pixel estimates agree closely
+    assert np.median(est_err) < 5.0
+    rb.close()
+
+
+def test_estimate_matches_oracle(gpu):
+    N = 3000
+    mdl, rb, om = _setup(N)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((N, mdl.d)) * 10
+    P = np.repeat(np.eye(mdl.d)[None], N, 0)
+    rb.set_tracks(x, P)
+    e = rb.estimate()
+    xf = x.astype(np.float32).astype(np.float64)
+    assert np.allclose(e, oracle.rbpf_estimate(xf), rtol=1e-12, atol=1e-9)
+    rb.close()
+
+
+def test_proposal_sampling(gpu):
+    N, n = 500, 5000
+    mdl, rb, om = _setup(N)
+    bins, _ = oracle.resample_linear(mdl.weights, N, 77)
+    x0, P0 = om.reset(bins)
+    rb.set_tracks(x0, P0)
+    Hp = mdl.pca_proj.T.copy()
+    M = mdl.pca_mean.copy()
+    normals = np.random.default_rng(1).standard_normal((2, n))
+    xbar = oracle.rbpf_estimate(x0.astype(np.float32).astype(np.float64))
+    s = rb.get_samples(Hp, M, n, 42.0, normals)
+    so = oracle.get_samples(Hp, M, xbar, n, 42.0, normals)
+    assert np.allclose(s, so, rtol=1e-9, atol=1e-9)
+    w1, w2 = rb.get_sample_prob(Hp, M, s, s[::-1].copy(), 42.0)
+    assert np.allclose(w1, oracle.get_sample_prob(Hp, M, xbar, s, 42.0), rtol=1e-9, atol=1e-300)
+    assert np.allclose(w2, oracle.get_sample_prob(Hp, M, xbar, s[::-1].copy(), 42.0), rtol=1e-9, atol=1e-300)
+    rb.close()
+
+
+def test_kf_model_predict_update(gpu):
+    rng = np.random.default_rng(5)
+    d, m, n = 12, 6, 7
+    rb = Rbpf(1)
+    A = rng.standard_normal((d, d)) * 0.3
+    F = np.eye(d) * 0.9 + 0.01 * A
+    B = rng.standard_normal(d)
+    Qh = rng.standard_normal((d, d)); Q = Qh @ Qh.T + np.eye(d)
+    H = rng.standard_normal((m, d)); BH = rng.standard_normal(m)
+    R = 100 * np.eye(m)
+    xs = rng.standard_normal((n, d)); Ps = np.stack([np.eye(d) * (i + 1) for i in range(n)])
+    z = rng.standard_normal((n, m))
+    xg, Pg = rb.kf_predict(F, B, Q, xs, Ps)
+    xg2, Pg2 = rb.kf_update(H, BH, R, z, xg, Pg)
+    for i in range(n):
+        xo, Po = oracle.kf_predict(F, B, Q, xs[i], Ps[i])
+        assert np.allclose(xg[i], xo, rtol=1e-12, atol=1e-12) and np.allclose(Pg[i], Po, rtol=1e-12, atol=1e-10)
+        xo2, Po2 = oracle.kf_update(H, BH, R, z[i], xo, Po)
+        assert np.allclose(xg2[i], xo2, rtol=1e-9, atol=1e-9)
+        assert np.allclose(Pg2[i], Po2, rtol=1e-9, atol=1e-9)
+    rb.close()
