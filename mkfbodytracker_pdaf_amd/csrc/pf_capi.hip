@@ -57,7 +57,6 @@ struct pf_handle {
     // model
     float *table = nullptr;
     int table_floats = 0;
-    unsigned long long *thr = nullptr;
     unsigned long long pi_total = 0;
     float *mu = nullptr, *sig = nullptr;
     std::vector<double> h_mu, h_sig, h_pi, h_gamma, h_H, h_BH, h_R;
@@ -146,7 +145,7 @@ int check_launch(const char *what)
 template <int D>
 int launch_step_d(pf_handle *h, const StepArgs &a, int mode, int block)
 {
-    const size_t lds = sizeof(float) * (size_t)(4 * h->K + h->table_floats);
+    const size_t lds = sizeof(float) * (size_t)h->table_floats;
     const int grid = (h->N + block - 1) / block;
     if (mode == kModeRef)
         hipLaunchKernelGGL((k_step<D, kModeRef>), dim3(grid), dim3(block), lds, h->stream, a);
@@ -204,8 +203,9 @@ struct EstimateL {
     }
 };
 
-// Resampling of the step's FP32 log-weights: weights + ancestors.
-int launch_resample_step(pf_handle *h, const uint32_t *uni, double *lml)
+// Resampling of the step's FP32 log-weights: weights + ancestors.  parity
+// selects which of the two frame counters is read (the other receives t + 1).
+int launch_resample_step(pf_handle *h, const uint32_t *uni, double *lml, int parity)
 {
     WeightArgs w{};
     w.lw32 = h->lw;
@@ -213,25 +213,34 @@ int launch_resample_step(pf_handle *h, const uint32_t *uni, double *lml)
     w.bsum = h->bsum;
     w.bexp = h->bexp;
     w.fs = h->fs;
-    w.frame = h->frame;
-    w.uni = uni;
-    w.lml = lml;
+    w.frame = h->frame + parity;
+    w.frame_next = h->frame + (1 - parity);
     w.L = h->N;
-    w.N = h->N;
-    w.ustride = 2;
-    w.uoff = 1;
-    w.advance = 1;
     const int nb = (h->N + kWeightsBlock - 1) / kWeightsBlock;
     hipLaunchKernelGGL((k_weights<kLogF32>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, w);
     int st = check_launch("k_weights");
     if (st) return st;
-    AncArgs an{h->cloc, h->bsum, h->fs, h->anc, h->N, h->N, nb};
-    hipLaunchKernelGGL(k_ancestors, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, an);
+    AncArgs an{};
+    an.cloc = h->cloc;
+    an.bsum = h->bsum;
+    an.bexp = h->bexp;
+    an.fs = h->fs;
+    an.frame = h->frame + parity;
+    an.uni = uni;
+    an.lml = lml;
+    an.anc = h->anc;
+    an.L = h->N;
+    an.N = h->N;
+    an.nb = nb;
+    an.ustride = 2;
+    an.uoff = 1;
+    an.kind = kLogF32;
+    hipLaunchKernelGGL(k_ancestors, dim3(nb), dim3(kWeightsBlock), 0, h->stream, an);
     return check_launch("k_ancestors");
 }
 
 StepArgs step_args(pf_handle *h, int mode, const uint32_t *uni, const float *Z, long long zstride,
-                   int zld, const int *count, const float *cand, int *comp_out)
+                   int zld, const int *count, const float *cand, int *comp_out, int parity)
 {
     StepArgs a{};
     a.xin = h->xA;
@@ -243,8 +252,7 @@ StepArgs step_args(pf_handle *h, int mode, const uint32_t *uni, const float *Z, 
     a.comp_out = comp_out;
     a.fs = h->fs;
     a.table = h->table;
-    a.thr = h->thr;
-    a.frame = h->frame;
+    a.frame = h->frame + parity;
     a.uni = uni;
     a.Z = Z;
     a.z_frame_stride = zstride;
@@ -331,8 +339,8 @@ int pf_create(pf_handle **out, const pf_config *cfg)
     pf_config c;
     pf_config_default(&c);
     if (cfg) c = *cfg;
-    if (c.n_particles <= 0 || c.n_particles > (1 << 26))
-        return set_error(PF_ERR_ARG, "n_particles %d out of range (1..2^26)", c.n_particles);
+    if (c.n_particles <= 0 || c.n_particles > (1 << 22))
+        return set_error(PF_ERR_ARG, "n_particles %d out of range (1..2^22)", c.n_particles);
     if (c.cand_capacity <= 0) c.cand_capacity = 16;
     if (c.cand_capacity > 256) return set_error(PF_ERR_ARG, "cand_capacity %d > 256", c.cand_capacity);
     if (c.block_size != 0 && (c.block_size % 64 != 0 || c.block_size > 256 || c.block_size < 64))
@@ -356,7 +364,7 @@ int pf_create(pf_handle **out, const pf_config *cfg)
     }
     hipError_t e = hipSuccess;
     e = e ? e : dalloc(&h->fs, 1);
-    e = e ? e : dalloc(&h->frame, 1);
+    e = e ? e : dalloc(&h->frame, 2);
     e = e ? e : dalloc(&h->uni, 2);
     e = e ? e : dalloc(&h->lml1, 1);
     e = e ? e : dalloc(&h->anc, (size_t)h->ld);
@@ -384,7 +392,7 @@ int pf_destroy(pf_handle *h)
     if (!h) return PF_OK;
     (void)hipSetDevice(h->cfg.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void *ptrs[] = {h->xA, h->PA, h->xB, h->PB, h->anc, h->comp, h->bins, h->lw, h->table, h->thr,
+    void *ptrs[] = {h->xA, h->PA, h->xB, h->PB, h->anc, h->comp, h->bins, h->lw, h->table,
                     h->mu, h->sig, h->fs, h->frame, h->uni, h->lml1, h->cloc, h->bsum, h->bexp,
                     h->zbuf, h->cntbuf, h->candbuf, h->est_partial, h->est_out, h->gbuf, h->ibuf};
     for (void *p : ptrs)
@@ -504,33 +512,37 @@ int pf_load_model(pf_handle *h, int32_t K, int32_t d, int32_t npose, int32_t m, 
         thr[2 * k + 1] = (unsigned long long)(th >> 64);
     }
     h->pi_total = acc;
-    const size_t lds = sizeof(float) * (size_t)(4 * K + tab.size());
+    // LDS image of the step kernel: thresholds (as 4K floats) then the table,
+    // padded to whole float4s.
+    std::vector<float> blob((size_t)4 * K + tab.size());
+    std::memcpy(blob.data(), thr.data(), sizeof(unsigned long long) * thr.size());
+    std::memcpy(blob.data() + 4 * K, tab.data(), sizeof(float) * tab.size());
+    blob.resize((blob.size() + 3) & ~(size_t)3, 0.f);
+    const size_t lds = sizeof(float) * blob.size();
     if (lds > 64 * 1024)
         return set_error(PF_ERR_UNSUPPORTED, "model table of %zu bytes exceeds the 64 KiB LDS budget", lds);
 
     // (re)allocate device buffers sized by the model
-    (void)(void)hipFree(h->table); (void)hipFree(h->thr); (void)hipFree(h->mu); (void)hipFree(h->sig);
+    (void)hipFree(h->table); (void)hipFree(h->mu); (void)hipFree(h->sig);
     (void)(void)hipFree(h->xA); (void)hipFree(h->PA); (void)hipFree(h->xB); (void)hipFree(h->PB);
-    h->table = nullptr; h->thr = nullptr; h->mu = nullptr; h->sig = nullptr;
+    h->table = nullptr; h->mu = nullptr; h->sig = nullptr;
     h->xA = h->PA = h->xB = h->PB = nullptr;
     (void)hipFree(h->zbuf); (void)hipFree(h->cntbuf); (void)hipFree(h->candbuf);
     h->zbuf = nullptr; h->cntbuf = nullptr; h->candbuf = nullptr;
     h->model = false;
     h->tracks = false;
-    HIP_TRY(dalloc(&h->table, tab.size()));
-    HIP_TRY(dalloc(&h->thr, thr.size()));
+    HIP_TRY(dalloc(&h->table, blob.size()));
     HIP_TRY(dalloc(&h->mu, mu.size()));
     HIP_TRY(dalloc(&h->sig, sg.size()));
     HIP_TRY(dalloc(&h->xA, (size_t)d * h->ld));
     HIP_TRY(dalloc(&h->xB, (size_t)d * h->ld));
     HIP_TRY(dalloc(&h->PA, (size_t)NP * h->ld));
     HIP_TRY(dalloc(&h->PB, (size_t)NP * h->ld));
-    HIP_TRY(hipMemcpyAsync(h->table, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->thr, thr.data(), sizeof(unsigned long long) * thr.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->table, blob.data(), sizeof(float) * blob.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->mu, mu.data(), sizeof(float) * mu.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->sig, sg.data(), sizeof(float) * sg.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    h->table_floats = (int)tab.size();
+    h->table_floats = (int)blob.size();
     h->clutter_mask = 0;
     h->order_packed = 0;
     for (int b = 0; b < J; ++b) h->order_packed |= (unsigned)b << (4 * b);
@@ -615,7 +627,7 @@ int pf_get_tracks(pf_handle *h, double *x, double *P)
 
 static int finish_host_step(pf_handle *h, float *lw, int32_t *anc, int32_t *comp, double *lml)
 {
-    int st = launch_resample_step(h, h->uni, h->lml1);
+    int st = launch_resample_step(h, h->uni, h->lml1, 0);
     if (st) return st;
     swap_state(h);
     if (lw) HIP_TRY(hipMemcpyAsync(lw, h->lw, sizeof(float) * h->N, hipMemcpyDeviceToHost, h->stream));
@@ -643,7 +655,7 @@ int pf_step(pf_handle *h, const double *Z, uint32_t u_comp, uint32_t u_res, floa
     HIP_TRY(hipMemcpyAsync(h->zbuf, zf.data(), sizeof(float) * zf.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->uni, u, sizeof(u), hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
-    StepArgs a = step_args(h, kModeRef, h->uni, h->zbuf, 0, N, nullptr, nullptr, comp ? h->comp : nullptr);
+    StepArgs a = step_args(h, kModeRef, h->uni, h->zbuf, 0, N, nullptr, nullptr, comp ? h->comp : nullptr, 0);
     st = launch_step(h, a, kModeRef);
     if (st) return st;
     return finish_host_step(h, lw, anc, comp, lml);
@@ -668,7 +680,7 @@ int pf_step_candidates(pf_handle *h, const int32_t *count, const double *cand, u
     HIP_TRY(hipMemcpyAsync(h->candbuf, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->uni, u, sizeof(u), hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
-    StepArgs a = step_args(h, kModeCand, h->uni, nullptr, 0, 0, h->cntbuf, h->candbuf, comp ? h->comp : nullptr);
+    StepArgs a = step_args(h, kModeCand, h->uni, nullptr, 0, 0, h->cntbuf, h->candbuf, comp ? h->comp : nullptr, 0);
     st = launch_step(h, a, kModeCand);
     if (st) return st;
     return finish_host_step(h, lw, anc, comp, lml);
@@ -718,13 +730,8 @@ static int resample_generic(pf_handle *h, const double *w, int L, int N, uint32_
     wa.bexp = h->bexp;
     wa.fs = h->fs;
     wa.frame = h->frame;
-    wa.uni = duni;
-    wa.lml = nullptr;
+    wa.frame_next = nullptr;
     wa.L = L;
-    wa.N = N;
-    wa.ustride = 2;
-    wa.uoff = 1;
-    wa.advance = 0;
     const int nb = (L + kWeightsBlock - 1) / kWeightsBlock;
     if (kind == kLogF64)
         hipLaunchKernelGGL((k_weights<kLogF64>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, wa);
@@ -732,8 +739,22 @@ static int resample_generic(pf_handle *h, const double *w, int L, int N, uint32_
         hipLaunchKernelGGL((k_weights<kLinF64>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, wa);
     st = check_launch("k_weights");
     if (st) return st;
-    AncArgs an{h->cloc, h->bsum, h->fs, dout, L, N, nb};
-    hipLaunchKernelGGL(k_ancestors, dim3((N + 255) / 256), dim3(256), 0, h->stream, an);
+    AncArgs an{};
+    an.cloc = h->cloc;
+    an.bsum = h->bsum;
+    an.bexp = h->bexp;
+    an.fs = h->fs;
+    an.frame = h->frame;
+    an.uni = duni;
+    an.lml = nullptr;
+    an.anc = dout;
+    an.L = L;
+    an.N = N;
+    an.nb = nb;
+    an.ustride = 2;
+    an.uoff = 1;
+    an.kind = kind;
+    hipLaunchKernelGGL(k_ancestors, dim3(nb), dim3(kWeightsBlock), 0, h->stream, an);
     st = check_launch("k_ancestors");
     if (st) return st;
     HIP_TRY(hipMemcpyAsync(out, dout, sizeof(int) * (size_t)N, hipMemcpyDeviceToHost, h->stream));
@@ -835,12 +856,13 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
         }
     }
     auto one_frame = [&](int f) -> int {
-        StepArgs a = step_args(h, mode, uniforms, Z, zstride, h->N, count, cand, nullptr);
+        const int parity = f & 1;
+        StepArgs a = step_args(h, mode, uniforms, Z, zstride, h->N, count, cand, nullptr, parity);
         if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f], h->stream));
         int s = launch_step(h, a, mode);
         if (s) return s;
         if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f + 1], h->stream));
-        s = launch_resample_step(h, uniforms, lml);
+        s = launch_resample_step(h, uniforms, lml, parity);
         if (s) return s;
         swap_state(h);
         return PF_OK;

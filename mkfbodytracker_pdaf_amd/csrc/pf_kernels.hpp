@@ -24,12 +24,8 @@ enum StepMode { kModeRef = 0, kModeCand = 1 };
 
 struct FrameScratch {
     unsigned long long maxkey;   // ordered key of the largest finite weight (0 = none)
-    unsigned int done;           // completion counter of k_weights blocks
     unsigned int done_est;       // completion counter of k_estimate blocks
     unsigned int status;         // bit 0: a degenerate (all-zero) resampling happened
-    uint32_t U;                  // uniform of the pending resampling
-    unsigned long long total;    // fixed-point weight total of the pending resampling
-    double ref;                  // max log-weight (or max linear weight)
 };
 
 struct StepArgs {
@@ -42,7 +38,6 @@ struct StepArgs {
     int *__restrict__ comp_out;           // optional
     FrameScratch *fs;
     const float *__restrict__ table;
-    const unsigned long long *__restrict__ thr;  // 2K words
     const int *__restrict__ frame;
     const uint32_t *__restrict__ uni;     // 2 per frame: U_comp, U_res
     // reference mode: Z[t][r][zld]
@@ -52,7 +47,7 @@ struct StepArgs {
     const int *__restrict__ count;
     const float *__restrict__ cand;
     unsigned long long pi_total;
-    int table_floats;
+    int table_floats;   // LDS image: 2K threshold words, then the float table (multiple of 4)
     int K, N, ld, m, J, cap, zld;
     unsigned int order_packed;   // joint processing order, 4 bits per slot
     unsigned int clutter_mask;
@@ -165,6 +160,45 @@ struct Joint {
     }
 };
 
+// Copies n4 float4 of the model table into LDS: all loads of a chunk are
+// issued before the first LDS write so the copy costs one memory latency.
+__device__ __forceinline__ void stage_table(float4 *dst, const float4 *__restrict__ src, int n4)
+{
+    constexpr int kChunk = 8;
+    for (int base = 0; base < n4; base += kChunk * (int)blockDim.x) {
+        float4 r[kChunk];
+#pragma unroll
+        for (int it = 0; it < kChunk; ++it) {
+            const int i = base + it * (int)blockDim.x + (int)threadIdx.x;
+            if (i < n4) r[it] = src[i];
+        }
+#pragma unroll
+        for (int it = 0; it < kChunk; ++it) {
+            const int i = base + it * (int)blockDim.x + (int)threadIdx.x;
+            if (i < n4) dst[i] = r[it];
+        }
+    }
+    __syncthreads();
+}
+
+// Structure-of-arrays rows through buffer instructions: one resource per
+// array, the row offset in an SGPR (soffset) and the particle offset in a
+// 32-bit VGPR (voffset), so no per-load 64-bit address arithmetic.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t soa_rsrc(const float *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float soa_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+
+__device__ __forceinline__ void soa_st(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_int(v), r, (int)voff, (int)soff, 0);
+}
+
 __device__ __forceinline__ int draw_component(const unsigned long long *thr, int K, uint32_t j,
                                               uint32_t U, unsigned long long total)
 {
@@ -183,12 +217,10 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
 {
     constexpr int NP = npacked(D);
     constexpr int CS = 2 + D + NP;
-    extern __shared__ float sm[];
-    unsigned long long *sthr = reinterpret_cast<unsigned long long *>(sm);
-    float *stab = sm + 4 * a.K;
-    for (int i = threadIdx.x; i < 2 * a.K; i += blockDim.x) sthr[i] = a.thr[i];
-    for (int i = threadIdx.x; i < a.table_floats; i += blockDim.x) stab[i] = a.table[i];
-    __syncthreads();
+    extern __shared__ float4 sm4[];
+    stage_table(sm4, reinterpret_cast<const float4 *>(a.table), a.table_floats >> 2);
+    const unsigned long long *sthr = reinterpret_cast<const unsigned long long *>(sm4);
+    const float *stab = reinterpret_cast<const float *>(sm4) + 4 * a.K;
 
     const int t = *a.frame;
     const uint32_t U1 = a.uni[2 * t];
@@ -196,11 +228,14 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
     float lwj = -INFINITY;
     if (j < a.N) {
         const int src = a.anc ? a.anc[j] : j;
+        const unsigned rowb = (unsigned)a.ld * 4u;
+        const unsigned voff = (unsigned)src * 4u;
+        const __amdgpu_buffer_rsrc_t rx = soa_rsrc(a.xin, rowb * D), rP = soa_rsrc(a.Pin, rowb * NP);
         float x[D], P[NP];
 #pragma unroll
-        for (int i = 0; i < D; ++i) x[i] = a.xin[(size_t)i * a.ld + src];
+        for (int i = 0; i < D; ++i) x[i] = soa_ld(rx, voff, rowb * i);
 #pragma unroll
-        for (int p = 0; p < NP; ++p) P[p] = a.Pin[(size_t)p * a.ld + src];
+        for (int p = 0; p < NP; ++p) P[p] = soa_ld(rP, voff, rowb * p);
 
         const int k = draw_component(sthr, a.K, (uint32_t)j, U1, a.pi_total);
         if (a.comp_out) a.comp_out[j] = k;
@@ -277,10 +312,12 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
             }
         }
         lwj = acc;
+        const unsigned doff = (unsigned)j * 4u;
+        const __amdgpu_buffer_rsrc_t wx = soa_rsrc(a.xout, rowb * D), wP = soa_rsrc(a.Pout, rowb * NP);
 #pragma unroll
-        for (int i = 0; i < D; ++i) a.xout[(size_t)i * a.ld + j] = x[i];
+        for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[i]);
 #pragma unroll
-        for (int p = 0; p < NP; ++p) a.Pout[(size_t)p * a.ld + j] = P[p];
+        for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[p]);
         a.lw[j] = lwj;
     }
     // block maximum of the finite log-weights -> one atomic per block
@@ -302,21 +339,23 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
 //   q_i = floor(2^32 * w_i / max w) (or det_exp(lw_i - max lw)),
 //   C = inclusive prefix sum (exact, order independent),
 //   a_i = min{idx : C[idx] * N * 2^32 > (i * 2^32 + U) * C[L-1]}.
-// k_weights writes the block-local prefix sums and, in its last block, the
-// block prefix, the total and the frame's log mean weight (FP64).
+// Equivalently input idx owns the outputs [g(C[idx-1]), g(C[idx])) with
+//   g(c) = min{i in [0, N] : (i * 2^32 + U) * T >= c * N * 2^32}.
+// k_weights: per-block inclusive scans of q (no cross-block traffic).
+// k_ancestors: block prefix from the block sums, g() of every input, then the
+// block writes its contiguous output range (binary search in LDS).
 enum WeightKind { kLogF32 = 0, kLogF64 = 1, kLinF64 = 2 };
 
 struct WeightArgs {
     const float *lw32;
     const double *w64;
     unsigned long long *cloc;     // L block-local inclusive prefix sums
-    unsigned long long *bsum;     // nb block sums -> inclusive block prefix (last block)
+    unsigned long long *bsum;     // nb block sums
     double *bexp;                 // nb block sums of the FP64 weights
     FrameScratch *fs;
-    int *frame;                   // frame counter (advanced by the last block if advance)
-    const uint32_t *uni;          // U = uni[t * ustride + uoff]
-    double *lml;                  // optional: lml[t] = log mean weight
-    int L, N, ustride, uoff, advance;
+    const int *frame;             // frame counter read this frame
+    int *frame_next;              // frame counter of the next frame (block 0 writes t + 1)
+    int L;
 };
 
 template <int KIND>
@@ -324,8 +363,6 @@ __global__ void __launch_bounds__(kWeightsBlock) k_weights(WeightArgs a)
 {
     __shared__ unsigned long long s_wave[4];
     __shared__ double s_dwave[4];
-    __shared__ int s_last;
-    const int t = *a.frame;
     const unsigned long long key = a.fs->maxkey;
     const bool degenerate = (key == 0ull);
     const double ref = degenerate ? 0.0 : key_value(key);
@@ -335,12 +372,8 @@ __global__ void __launch_bounds__(kWeightsBlock) k_weights(WeightArgs a)
     if (i < a.L) {
         if (degenerate) {
             q = 1;
-        } else if (KIND == kLogF32) {
-            const double x = (double)a.lw32[i];
-            v = isfinite(x) ? det_exp(x - ref) : 0.0;
-            q = qlog(x, ref);
-        } else if (KIND == kLogF64) {
-            const double x = a.w64[i];
+        } else if (KIND == kLogF32 || KIND == kLogF64) {
+            const double x = (KIND == kLogF32) ? (double)a.lw32[i] : a.w64[i];
             v = isfinite(x) ? det_exp(x - ref) : 0.0;
             q = qlog(x, ref);
         } else {
@@ -350,7 +383,6 @@ __global__ void __launch_bounds__(kWeightsBlock) k_weights(WeightArgs a)
             q = ok ? fixed32(x / ref) : 0;
         }
     }
-    // block inclusive scan of q
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     unsigned long long s = q;
 #pragma unroll
@@ -370,90 +402,89 @@ __global__ void __launch_bounds__(kWeightsBlock) k_weights(WeightArgs a)
         a.bsum[blockIdx.x] = s;
         a.bexp[blockIdx.x] = (s_dwave[0] + s_dwave[1]) + (s_dwave[2] + s_dwave[3]);
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = (atomicAdd(&a.fs->done, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-
-    // last block: inclusive scan of the block sums, totals, frame bookkeeping
-    const int nb = gridDim.x;
-    const int per = (nb + kWeightsBlock - 1) / kWeightsBlock;
-    const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
-    unsigned long long loc = 0;
-    double dloc = 0.0;
-    for (int b = b0; b < b1; ++b) {
-        loc += ((volatile unsigned long long *)a.bsum)[b];
-        dloc += ((volatile double *)a.bexp)[b];
-    }
-    unsigned long long ls = loc;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long n = __shfl_up(ls, o, 64);
-        if (lane >= o) ls += n;
-    }
-    const double dsum = wave_sum(dloc);
-    __syncthreads();
-    if (lane == 63) s_wave[wid] = ls;
-    if (lane == 0) s_dwave[wid] = dsum;
-    __syncthreads();
-    unsigned long long woff = 0;
-    for (int w = 0; w < wid; ++w) woff += s_wave[w];
-    unsigned long long run = woff + ls - loc;  // exclusive prefix of this thread's chunk
-    for (int b = b0; b < b1; ++b) {
-        run += ((volatile unsigned long long *)a.bsum)[b];
-        a.bsum[b] = run;
-    }
-    if (threadIdx.x == kWeightsBlock - 1) {
-        const unsigned long long total = run;
-        const double se = (s_dwave[0] + s_dwave[1]) + (s_dwave[2] + s_dwave[3]);
-        a.fs->total = total;
-        a.fs->U = a.uni[(size_t)t * a.ustride + a.uoff];
-        a.fs->ref = ref;
-        if (degenerate) a.fs->status |= 1u;
-        if (a.lml) {
-            double l;
-            if (degenerate) l = -INFINITY;
-            else if (KIND == kLinF64) l = log(se / (double)a.N);
-            else l = ref + log(se) - log((double)a.N);
-            a.lml[t] = l;
-        }
-        a.fs->maxkey = 0ull;
-        a.fs->done = 0u;
-        if (a.advance) *a.frame = t + 1;
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.frame_next) *a.frame_next = *a.frame + 1;
 }
 
 struct AncArgs {
     const unsigned long long *cloc;
-    const unsigned long long *bsum;   // inclusive block prefix
-    const FrameScratch *fs;
+    const unsigned long long *bsum;
+    const double *bexp;
+    FrameScratch *fs;
+    const int *frame;
+    const uint32_t *uni;          // U = uni[t * ustride + uoff]
+    double *lml;                  // optional: lml[t] = log mean weight
     int *anc;
-    int L, N, nb;
+    int L, N, nb, ustride, uoff, kind;
 };
 
-__global__ void __launch_bounds__(256) k_ancestors(AncArgs a)
+// g(c): first output whose systematic target reaches cumulative weight c.
+__device__ __forceinline__ int first_output(unsigned long long c, uint32_t U, unsigned long long T, int N)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N) return;
-    const unsigned long long total = a.fs->total;
-    const u128 target = sys_target((uint32_t)i, a.fs->U, total);
-    const u128 scale = (u128)(unsigned long long)a.N << 32;
-    int lo = 0, hi = a.nb - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((u128)a.bsum[mid] * scale > target) hi = mid;
-        else lo = mid + 1;
+    const u128 cs = (u128)c * ((u128)(unsigned long long)N << 32);
+    double est = (double)c * (double)N / (double)T - (double)U * 2.3283064365386963e-10;
+    long long i0 = (long long)ceil(est);
+    if (i0 < 0) i0 = 0;
+    if (i0 > N) i0 = N;
+    while (i0 > 0 && sys_target((uint32_t)(i0 - 1), U, T) >= cs) --i0;
+    while (i0 < N && sys_target((uint32_t)i0, U, T) < cs) ++i0;
+    return (int)i0;
+}
+
+__global__ void __launch_bounds__(kWeightsBlock) k_ancestors(AncArgs a)
+{
+    __shared__ unsigned long long s_pre[4], s_tot[4];
+    __shared__ double s_e[4];
+    __shared__ int s_hi[kWeightsBlock];
+    __shared__ int s_lo;
+    const int t = *a.frame;
+    const uint32_t U = a.uni[(size_t)t * a.ustride + a.uoff];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int b = blockIdx.x;
+    // prefix of the block sums before this block, and the total
+    unsigned long long pre = 0, tot = 0;
+    double e = 0.0;
+    for (int k = threadIdx.x; k < a.nb; k += kWeightsBlock) {
+        const unsigned long long v = a.bsum[k];
+        tot += v;
+        if (k < b) pre += v;
+        if (b == 0) e += a.bexp[k];
     }
-    const unsigned long long base = lo > 0 ? a.bsum[lo - 1] : 0ull;
-    int l2 = lo * kWeightsBlock, h2 = min(a.L, (lo + 1) * kWeightsBlock) - 1;
-    while (l2 < h2) {
-        const int mid = (l2 + h2) >> 1;
-        if ((u128)(base + a.cloc[mid]) * scale > target) h2 = mid;
-        else l2 = mid + 1;
+    pre = wave_sum(pre);
+    tot = wave_sum(tot);
+    e = wave_sum(e);
+    if (lane == 0) { s_pre[wid] = pre; s_tot[wid] = tot; s_e[wid] = e; }
+    __syncthreads();
+    pre = (s_pre[0] + s_pre[1]) + (s_pre[2] + s_pre[3]);
+    tot = (s_tot[0] + s_tot[1]) + (s_tot[2] + s_tot[3]);
+    const int i0 = b * kWeightsBlock;
+    const int idx = i0 + threadIdx.x;
+    const int n_in = min(kWeightsBlock, a.L - i0);
+    if (threadIdx.x < n_in) s_hi[threadIdx.x] = first_output(pre + a.cloc[idx], U, tot, a.N);
+    if (threadIdx.x == 0) s_lo = first_output(pre, U, tot, a.N);
+    __syncthreads();
+    const int lo = s_lo, hi = s_hi[n_in - 1];
+    for (int i = lo + threadIdx.x; i < hi; i += kWeightsBlock) {
+        int l = 0, h = n_in - 1;           // smallest k with s_hi[k] > i
+        while (l < h) {
+            const int mid = (l + h) >> 1;
+            if (s_hi[mid] > i) h = mid;
+            else l = mid + 1;
+        }
+        a.anc[i] = i0 + l;
     }
-    a.anc[i] = l2;
+    if (b == 0 && threadIdx.x == 0) {
+        const unsigned long long key = a.fs->maxkey;
+        const double se = (s_e[0] + s_e[1]) + (s_e[2] + s_e[3]);
+        if (key == 0ull) a.fs->status |= 1u;
+        if (a.lml) {
+            double l;
+            if (key == 0ull) l = -INFINITY;
+            else if (a.kind == kLinF64) l = log(se / (double)a.N);
+            else l = key_value(key) + log(se) - log((double)a.N);
+            a.lml[t] = l;
+        }
+        a.fs->maxkey = 0ull;
+    }
 }
 
 // block maximum of FP64 host weights (for the stand-alone resample entry points)
@@ -532,17 +563,18 @@ __global__ void __launch_bounds__(256) k_estimate(const float *x, const int *anc
         if (lane == 0) s_w[wid][i] = r;
     }
     __syncthreads();
-    if (threadIdx.x < D) {
-        double r = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += s_w[w][threadIdx.x];
-        partial[(size_t)blockIdx.x * D + threadIdx.x] = r;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < D; ++i) {
+            double r = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += s_w[w][i];
+            partial[(size_t)blockIdx.x * D + i] = r;
+        }
+        __threadfence();
+        s_last = (atomicAdd(&fs->done_est, 1u) == gridDim.x - 1);
+        if (s_last) __threadfence();
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = (atomicAdd(&fs->done_est, 1u) == gridDim.x - 1);
     __syncthreads();
     if (!s_last) return;
-    __threadfence();
     if (threadIdx.x < D) {
         double r = 0.0;
         for (int b = 0; b < (int)gridDim.x; ++b) r += ((volatile double *)partial)[(size_t)b * D + threadIdx.x];
