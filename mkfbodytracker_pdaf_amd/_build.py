@@ -10,7 +10,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-LIB = os.path.join(HERE, "libmkfbt.so")
+LIB = os.environ.get("MKFBT_LIB") or os.path.join(HERE, "libmkfbt.so")
 SOURCES = ["pf_capi.hip", "pf_io.cpp", "pf_host.cpp", "pf2d.hip", "pf_adjoint.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -60,6 +60,11 @@ struct pf_handle {
     unsigned long long pi_total = 0;
     float *mu = nullptr, *sig = nullptr;
     std::vector<double> h_mu, h_sig, h_pi, h_gamma, h_H, h_BH, h_R;
+    double wmax = 0.0;
+    // state basis x' = T x (T = I when H has no usable row-selection basis)
+    bool sel_mode = false;
+    std::vector<double> h_T, h_Tinv;
+    double *Tdev = nullptr, *Tinv = nullptr;   // d x d each, device FP64
     // frame bookkeeping
     FrameScratch *fs = nullptr;
     int *frame = nullptr;
@@ -142,16 +147,35 @@ int check_launch(const char *what)
 }
 
 // ---------------------------------------------------------- kernel dispatch
+template <int D, int SELJ>
+int launch_step_dj(pf_handle *h, const StepArgs &a, int mode, int block)
+{
+    const size_t lds = (size_t)step_lds_bytes(h->table_floats, a.anc_mode == kAncFused ? a.pend.nb : 0);
+    const int grid = (h->N + block - 1) / block;
+    if (mode == kModeRef)
+        hipLaunchKernelGGL((k_step<D, kModeRef, SELJ>), dim3(grid), dim3(block), lds, h->stream, a);
+    else
+        hipLaunchKernelGGL((k_step<D, kModeCand, SELJ>), dim3(grid), dim3(block), lds, h->stream, a);
+    return check_launch("k_step");
+}
+
+// Selection-basis kernels exist for every joint count J with 2J <= D.
 template <int D>
 int launch_step_d(pf_handle *h, const StepArgs &a, int mode, int block)
 {
-    const size_t lds = sizeof(float) * (size_t)h->table_floats;
-    const int grid = (h->N + block - 1) / block;
-    if (mode == kModeRef)
-        hipLaunchKernelGGL((k_step<D, kModeRef>), dim3(grid), dim3(block), lds, h->stream, a);
-    else
-        hipLaunchKernelGGL((k_step<D, kModeCand>), dim3(grid), dim3(block), lds, h->stream, a);
-    return check_launch("k_step");
+    if (!h->sel_mode) return launch_step_dj<D, 0>(h, a, mode, block);
+    switch (h->J) {
+    case 1: return launch_step_dj<D, 1>(h, a, mode, block);
+    case 2: return launch_step_dj<D, 2>(h, a, mode, block);
+    case 3: return launch_step_dj<D, 3>(h, a, mode, block);
+    case 4: return launch_step_dj<D, 4>(h, a, mode, block);
+    case 5: if constexpr (D >= 10) return launch_step_dj<D, 5>(h, a, mode, block); break;
+    case 6: if constexpr (D >= 12) return launch_step_dj<D, 6>(h, a, mode, block); break;
+    case 7: if constexpr (D >= 14) return launch_step_dj<D, 7>(h, a, mode, block); break;
+    case 8: if constexpr (D >= 16) return launch_step_dj<D, 8>(h, a, mode, block); break;
+    default: break;
+    }
+    return launch_step_dj<D, 0>(h, a, mode, block);
 }
 
 int launch_step(pf_handle *h, const StepArgs &a, int mode)
@@ -186,11 +210,20 @@ struct ResetL {
 };
 
 template <int D>
-struct GatherL {
-    static void run(pf_handle *h)
+struct ExportL {
+    static void run(pf_handle *h, double *x, double *P)
     {
-        hipLaunchKernelGGL((k_gather<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->xA,
-                           h->PA, h->anc, h->xB, h->PB, h->N, h->ld);
+        hipLaunchKernelGGL((k_export<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->xA, h->PA,
+                           h->anc, h->Tinv, x, P, h->N, h->ld);
+    }
+};
+
+template <int D>
+struct ImportL {
+    static void run(pf_handle *h, const double *x, const double *P)
+    {
+        hipLaunchKernelGGL((k_import<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, x, P, h->Tdev,
+                           h->xA, h->PA, h->N, h->ld);
     }
 };
 
@@ -199,13 +232,25 @@ struct EstimateL {
     static void run(pf_handle *h)
     {
         hipLaunchKernelGGL((k_estimate<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->xA,
-                           h->anc, h->N, h->ld, h->est_partial, h->fs, h->est_out);
+                           h->anc, h->N, h->ld, h->est_partial, h->fs, h->Tinv, h->est_out);
     }
 };
 
-// Resampling of the step's FP32 log-weights: weights + ancestors.  parity
-// selects which of the two frame counters is read (the other receives t + 1).
-int launch_resample_step(pf_handle *h, const uint32_t *uni, double *lml, int parity)
+Pending pending_of(const pf_handle *h, int L)
+{
+    Pending p{};
+    p.cloc = h->cloc;
+    p.bsum = h->bsum;
+    p.bexp = h->bexp;
+    p.L = L;
+    p.nb = (L + kWeightsBlock - 1) / kWeightsBlock;
+    return p;
+}
+
+// Fixed-point weights and block scans of the step's FP32 log-weights.
+// parity selects which of the two frame counters is read (the other
+// receives t + 1) and which maximum slot holds the frame's maximum.
+int launch_weights(pf_handle *h, const uint32_t *uni, int parity)
 {
     WeightArgs w{};
     w.lw32 = h->lw;
@@ -215,34 +260,41 @@ int launch_resample_step(pf_handle *h, const uint32_t *uni, double *lml, int par
     w.fs = h->fs;
     w.frame = h->frame + parity;
     w.frame_next = h->frame + (1 - parity);
+    w.uni = uni;
     w.L = h->N;
+    w.parity = parity;
+    w.ustride = 2;
+    w.uoff = 1;
     const int nb = (h->N + kWeightsBlock - 1) / kWeightsBlock;
     hipLaunchKernelGGL((k_weights<kLogF32>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, w);
-    int st = check_launch("k_weights");
-    if (st) return st;
+    return check_launch("k_weights");
+}
+
+// Materialise the pending resampling of L weights into N ancestors.
+int launch_ancestors(pf_handle *h, int L, int N, int *out, int parity, int kind_linear, double *lml)
+{
     AncArgs an{};
-    an.cloc = h->cloc;
-    an.bsum = h->bsum;
-    an.bexp = h->bexp;
+    an.pend = pending_of(h, L);
     an.fs = h->fs;
-    an.frame = h->frame + parity;
-    an.uni = uni;
+    an.anc = out;
     an.lml = lml;
-    an.anc = h->anc;
-    an.L = h->N;
-    an.N = h->N;
-    an.nb = nb;
-    an.ustride = 2;
-    an.uoff = 1;
-    an.kind = kLogF32;
-    hipLaunchKernelGGL(k_ancestors, dim3(nb), dim3(kWeightsBlock), 0, h->stream, an);
+    an.N = N;
+    an.parity = parity;
+    an.kind_linear = kind_linear;
+    const size_t lds = 8 * (size_t)resolve_lds_words(an.pend.nb);
+    hipLaunchKernelGGL(k_ancestors, dim3((N + 255) / 256), dim3(256), lds, h->stream, an);
     return check_launch("k_ancestors");
 }
 
 StepArgs step_args(pf_handle *h, int mode, const uint32_t *uni, const float *Z, long long zstride,
-                   int zld, const int *count, const float *cand, int *comp_out, int parity)
+                   int zld, const int *count, const float *cand, int *comp_out, int parity,
+                   int anc_mode, double *lml)
 {
     StepArgs a{};
+    a.parity = parity;
+    a.anc_mode = anc_mode;
+    a.lml = lml;
+    a.pend = pending_of(h, h->N);
     a.xin = h->xA;
     a.Pin = h->PA;
     a.xout = h->xB;
@@ -297,22 +349,206 @@ int read_status(pf_handle *h, int *degenerate)
     return PF_OK;
 }
 
-// Packed-SoA FP32 <-> row-major double conversions on the host.
-void pack_tracks(const pf_handle *h, const double *x, const double *P, std::vector<float> &xs,
-                 std::vector<float> &Ps)
+
+// ------------------------------------------------------------ model tables
+// Gauss-Jordan inverse with partial pivoting (FP64, n <= 32).
+bool invert(int n, const std::vector<double> &A, std::vector<double> &Ai)
 {
-    const int D = h->D, NP = npacked(D), N = h->N, ld = h->ld;
-    xs.assign((size_t)D * ld, 0.f);
-    Ps.assign((size_t)NP * ld, 0.f);
-    for (int j = 0; j < N; ++j) {
-        for (int i = 0; i < D; ++i) xs[(size_t)i * ld + j] = (float)x[(size_t)j * D + i];
-        for (int i = 0; i < D; ++i)
-            for (int k = i; k < D; ++k)
-                Ps[(size_t)pk(D, i, k) * ld + j] =
-                    (float)(0.5 * (P[(size_t)j * D * D + i * D + k] + P[(size_t)j * D * D + k * D + i]));
+    std::vector<double> a = A;
+    Ai.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Ai[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)p * n + c])) p = r;
+        if (a[(size_t)p * n + c] == 0.0) return false;
+        if (p != c)
+            for (int j = 0; j < n; ++j) {
+                std::swap(a[(size_t)c * n + j], a[(size_t)p * n + j]);
+                std::swap(Ai[(size_t)c * n + j], Ai[(size_t)p * n + j]);
+            }
+        const double piv = a[(size_t)c * n + c];
+        for (int j = 0; j < n; ++j) {
+            a[(size_t)c * n + j] /= piv;
+            Ai[(size_t)c * n + j] /= piv;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = a[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; ++j) {
+                a[(size_t)r * n + j] -= f * a[(size_t)c * n + j];
+                Ai[(size_t)r * n + j] -= f * Ai[(size_t)c * n + j];
+            }
+        }
     }
+    return true;
 }
 
+// State basis in which the measurement matrix is a row selection.  Because
+// every particle shares H and F = g I commutes with any change of basis
+// (my_gmm.cpp:56,71), the filter can run on x' = T x, P' = T P T^T with
+//   T = [H rows in processing order ; s * orthonormal complement of row(H)],
+// so joint slot s observes rows 2s, 2s+1 of x' directly and H P costs
+// nothing.  Likelihoods and weights are basis independent.  Returns false
+// (T = I, dense H kept) if H is rank deficient or T is badly conditioned.
+bool make_basis(const pf_handle *h, std::vector<double> &T, std::vector<double> &Ti)
+{
+    const int d = h->D, m = h->m, J = h->J;
+    T.assign((size_t)d * d, 0.0);
+    if (m > d) return false;
+    std::vector<double> rows((size_t)m * d);
+    for (int sl = 0; sl < J; ++sl) {
+        const int b = (int)((h->order_packed >> (4 * sl)) & 15u);
+        for (int r = 0; r < 2; ++r)
+            for (int c = 0; c < d; ++c) rows[(size_t)(2 * sl + r) * d + c] = h->h_H[(size_t)(2 * b + r) * d + c];
+    }
+    // orthonormal basis of span(rows) then its completion (MGS, two passes)
+    std::vector<std::vector<double>> q;
+    double fro = 0.0;
+    for (double v : rows) fro += v * v;
+    const double scale = std::sqrt(fro / m);
+    auto orth = [&](std::vector<double> v, double ref) -> bool {
+        for (int pass = 0; pass < 2; ++pass)
+            for (const auto &u : q) {
+                double dot = 0.0;
+                for (int c = 0; c < d; ++c) dot += u[c] * v[c];
+                for (int c = 0; c < d; ++c) v[c] -= dot * u[c];
+            }
+        double n2 = 0.0;
+        for (int c = 0; c < d; ++c) n2 += v[c] * v[c];
+        if (!(std::sqrt(n2) > 1e-8 * ref)) return false;
+        const double nv = std::sqrt(n2);
+        for (int c = 0; c < d; ++c) v[c] /= nv;
+        q.push_back(v);
+        return true;
+    };
+    for (int r = 0; r < m; ++r) {
+        std::vector<double> v(rows.begin() + (size_t)r * d, rows.begin() + (size_t)(r + 1) * d);
+        double n2 = 0.0;
+        for (double x : v) n2 += x * x;
+        if (!orth(v, std::sqrt(n2))) return false;
+    }
+    for (int i = 0; i < d && (int)q.size() < d; ++i) {
+        std::vector<double> e((size_t)d, 0.0);
+        e[i] = 1.0;
+        orth(e, 1e-2);
+    }
+    if ((int)q.size() != d) return false;
+    for (int r = 0; r < m; ++r)
+        for (int c = 0; c < d; ++c) T[(size_t)r * d + c] = rows[(size_t)r * d + c];
+    for (int r = m; r < d; ++r)
+        for (int c = 0; c < d; ++c) T[(size_t)r * d + c] = scale * q[r][c];
+    if (!invert(d, T, Ti)) return false;
+    double nt = 0.0, ni = 0.0;
+    for (double v : T) nt += v * v;
+    for (double v : Ti) ni += v * v;
+    return std::sqrt(nt * ni) < 1e6;
+}
+
+// Builds the LDS model image (thresholds, component table, H, BH, R blocks),
+// the reset table and the basis matrices, and uploads them.
+int build_tables(pf_handle *h)
+{
+    const int d = h->D, K = h->K, m = h->m, J = h->J, NP = npacked(d), CS = 2 + d + NP;
+    std::vector<double> T, Ti;
+    h->sel_mode = make_basis(h, T, Ti);
+    if (!h->sel_mode) {
+        T.assign((size_t)d * d, 0.0);
+        for (int i = 0; i < d; ++i) T[(size_t)i * d + i] = 1.0;
+        Ti = T;
+    }
+    h->h_T = T;
+    h->h_Tinv = Ti;
+    auto tmul = [&](const double *v, double *out) {               // out = T v
+        for (int i = 0; i < d; ++i) {
+            double acc = 0.0;
+            for (int k = 0; k < d; ++k) acc += T[(size_t)i * d + k] * v[k];
+            out[i] = acc;
+        }
+    };
+    auto tsandwich = [&](const double *S, double *out) {          // out = T S T^T
+        std::vector<double> TS((size_t)d * d, 0.0);
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j < d; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < d; ++k) acc += T[(size_t)i * d + k] * S[(size_t)k * d + j];
+                TS[(size_t)i * d + j] = acc;
+            }
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j < d; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < d; ++k) acc += TS[(size_t)i * d + k] * T[(size_t)j * d + k];
+                out[(size_t)i * d + j] = acc;
+            }
+    };
+    std::vector<float> tab((size_t)K * CS + (size_t)m * d + m + 3 * J);
+    std::vector<float> mu((size_t)K * d), sg((size_t)K * NP);
+    std::vector<double> mut(d), sgt((size_t)d * d), sym((size_t)d * d);
+    for (int k = 0; k < K; ++k) {                       // my_gmm.cpp:51-58, in the state basis
+        const double g = h->h_gamma[k];
+        const double *Sk = &h->h_sig[(size_t)k * d * d];
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j < d; ++j) sym[(size_t)i * d + j] = 0.5 * (Sk[(size_t)i * d + j] + Sk[(size_t)j * d + i]);
+        tmul(&h->h_mu[(size_t)k * d], mut.data());
+        tsandwich(sym.data(), sgt.data());
+        float *c = &tab[(size_t)k * CS];
+        c[0] = (float)g;
+        c[1] = (float)(g * g);
+        for (int i = 0; i < d; ++i) {
+            c[2 + i] = (float)((1.0 - g) * mut[i]);
+            mu[(size_t)k * d + i] = (float)mut[i];
+        }
+        for (int i = 0; i < d; ++i)
+            for (int q = i; q < d; ++q) {
+                const double v = 0.5 * (sgt[(size_t)i * d + q] + sgt[(size_t)q * d + i]);
+                c[2 + d + pk(d, i, q)] = (float)((1.0 - g * g) * v);
+                sg[(size_t)k * NP + pk(d, i, q)] = (float)v;
+            }
+    }
+    float *Hs = &tab[(size_t)K * CS];
+    for (int i = 0; i < m * d; ++i) Hs[i] = (float)h->h_H[i];
+    for (int a = 0; a < m; ++a) Hs[m * d + a] = (float)h->h_BH[a];
+    const std::vector<double> &Rm = h->h_R;
+    for (int b = 0; b < J; ++b) {
+        Hs[m * d + m + 3 * b] = (float)Rm[(2 * b) * m + 2 * b];
+        Hs[m * d + m + 3 * b + 1] = (float)Rm[(2 * b) * m + 2 * b + 1];
+        Hs[m * d + m + 3 * b + 2] = (float)Rm[(2 * b + 1) * m + 2 * b + 1];
+    }
+    // Component thresholds for resample(gmm.weight, N) (pf2DRao.cpp:128).
+    std::vector<unsigned long long> thr(2 * (size_t)K);
+    unsigned long long acc = 0;
+    for (int k = 0; k < K; ++k) {
+        acc += fixed32(h->h_pi[k] / h->wmax);
+        const u128 th = (u128)acc * ((u128)(unsigned long long)h->N << 32);
+        thr[2 * k] = (unsigned long long)th;
+        thr[2 * k + 1] = (unsigned long long)(th >> 64);
+    }
+    h->pi_total = acc;
+    std::vector<float> blob((size_t)4 * K + tab.size());
+    std::memcpy(blob.data(), thr.data(), sizeof(unsigned long long) * thr.size());
+    std::memcpy(blob.data() + 4 * K, tab.data(), sizeof(float) * tab.size());
+    blob.resize((blob.size() + 3) & ~(size_t)3, 0.f);
+    const size_t lds = sizeof(float) * blob.size();
+    if (lds > 48 * 1024)
+        return set_error(PF_ERR_UNSUPPORTED, "model table of %zu bytes exceeds the 48 KiB LDS budget", lds);
+    (void)hipFree(h->table); (void)hipFree(h->mu); (void)hipFree(h->sig);
+    (void)hipFree(h->Tdev); (void)hipFree(h->Tinv);
+    h->table = nullptr; h->mu = nullptr; h->sig = nullptr; h->Tdev = nullptr; h->Tinv = nullptr;
+    HIP_TRY(dalloc(&h->table, blob.size()));
+    HIP_TRY(dalloc(&h->mu, mu.size()));
+    HIP_TRY(dalloc(&h->sig, sg.size()));
+    HIP_TRY(dalloc(&h->Tdev, (size_t)d * d));
+    HIP_TRY(dalloc(&h->Tinv, (size_t)d * d));
+    HIP_TRY(hipMemcpyAsync(h->table, blob.data(), sizeof(float) * blob.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->mu, mu.data(), sizeof(float) * mu.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->sig, sg.data(), sizeof(float) * sg.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->Tdev, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->Tinv, Ti.data(), sizeof(double) * Ti.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->table_floats = (int)blob.size();
+    return PF_OK;
+}
 }  // namespace
 
 // =================================================================== C ABI
@@ -339,8 +575,9 @@ int pf_create(pf_handle **out, const pf_config *cfg)
     pf_config c;
     pf_config_default(&c);
     if (cfg) c = *cfg;
-    if (c.n_particles <= 0 || c.n_particles > (1 << 22))
-        return set_error(PF_ERR_ARG, "n_particles %d out of range (1..2^22)", c.n_particles);
+    if (c.n_particles <= 0 || c.n_particles > kMaxScanBlocks * kWeightsBlock)
+        return set_error(PF_ERR_ARG, "n_particles %d out of range (1..%d)", c.n_particles,
+                         kMaxScanBlocks * kWeightsBlock);
     if (c.cand_capacity <= 0) c.cand_capacity = 16;
     if (c.cand_capacity > 256) return set_error(PF_ERR_ARG, "cand_capacity %d > 256", c.cand_capacity);
     if (c.block_size != 0 && (c.block_size % 64 != 0 || c.block_size > 256 || c.block_size < 64))
@@ -394,7 +631,8 @@ int pf_destroy(pf_handle *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->xA, h->PA, h->xB, h->PB, h->anc, h->comp, h->bins, h->lw, h->table,
                     h->mu, h->sig, h->fs, h->frame, h->uni, h->lml1, h->cloc, h->bsum, h->bexp,
-                    h->zbuf, h->cntbuf, h->candbuf, h->est_partial, h->est_out, h->gbuf, h->ibuf};
+                    h->zbuf, h->cntbuf, h->candbuf, h->est_partial, h->est_out, h->gbuf, h->ibuf,
+                    h->Tdev, h->Tinv};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
@@ -462,7 +700,7 @@ int pf_load_model(pf_handle *h, int32_t K, int32_t d, int32_t npose, int32_t m, 
         }
     }
 
-    const int NP = npacked(d), CS = 2 + d + NP, J = m / 2;
+    const int NP = npacked(d), J = m / 2;
     h->D = d; h->K = K; h->m = m; h->J = J; h->npose = npose;
     h->h_mu.assign(means, means + (size_t)K * d);
     h->h_sig.assign(covs, covs + (size_t)K * d * d);
@@ -475,77 +713,24 @@ int pf_load_model(pf_handle *h, int32_t K, int32_t d, int32_t npose, int32_t m, 
         h->h_BH[a] = pca_mean[sel[a]];
     }
     h->h_R = Rm;
-
-    std::vector<float> tab((size_t)K * CS + (size_t)m * d + m + 3 * J);
-    std::vector<float> mu((size_t)K * d), sg((size_t)K * NP);
-    for (int k = 0; k < K; ++k) {                       // my_gmm.cpp:51-58
-        const double g = gamma[k];
-        float *c = &tab[(size_t)k * CS];
-        c[0] = (float)g;
-        c[1] = (float)(g * g);
-        for (int i = 0; i < d; ++i) {
-            c[2 + i] = (float)((1.0 - g) * means[(size_t)k * d + i]);
-            mu[(size_t)k * d + i] = (float)means[(size_t)k * d + i];
-        }
-        for (int i = 0; i < d; ++i)
-            for (int q = i; q < d; ++q) {
-                const double s = 0.5 * (covs[(size_t)k * d * d + i * d + q] + covs[(size_t)k * d * d + q * d + i]);
-                c[2 + d + pk(d, i, q)] = (float)((1.0 - g * g) * s);
-                sg[(size_t)k * NP + pk(d, i, q)] = (float)s;
-            }
-    }
-    float *Hs = &tab[(size_t)K * CS];
-    for (int i = 0; i < m * d; ++i) Hs[i] = (float)h->h_H[i];
-    for (int a = 0; a < m; ++a) Hs[m * d + a] = (float)h->h_BH[a];
-    for (int b = 0; b < J; ++b) {
-        Hs[m * d + m + 3 * b] = (float)Rm[(2 * b) * m + 2 * b];
-        Hs[m * d + m + 3 * b + 1] = (float)Rm[(2 * b) * m + 2 * b + 1];
-        Hs[m * d + m + 3 * b + 2] = (float)Rm[(2 * b + 1) * m + 2 * b + 1];
-    }
-    // Component thresholds for resample(gmm.weight, N) (pf2DRao.cpp:128).
-    std::vector<unsigned long long> thr(2 * (size_t)K);
-    unsigned long long acc = 0;
-    for (int k = 0; k < K; ++k) {
-        acc += fixed32(weights[k] / wmax);
-        const u128 th = (u128)acc * ((u128)(unsigned long long)h->N << 32);
-        thr[2 * k] = (unsigned long long)th;
-        thr[2 * k + 1] = (unsigned long long)(th >> 64);
-    }
-    h->pi_total = acc;
-    // LDS image of the step kernel: thresholds (as 4K floats) then the table,
-    // padded to whole float4s.
-    std::vector<float> blob((size_t)4 * K + tab.size());
-    std::memcpy(blob.data(), thr.data(), sizeof(unsigned long long) * thr.size());
-    std::memcpy(blob.data() + 4 * K, tab.data(), sizeof(float) * tab.size());
-    blob.resize((blob.size() + 3) & ~(size_t)3, 0.f);
-    const size_t lds = sizeof(float) * blob.size();
-    if (lds > 64 * 1024)
-        return set_error(PF_ERR_UNSUPPORTED, "model table of %zu bytes exceeds the 64 KiB LDS budget", lds);
+    h->wmax = wmax;
 
     // (re)allocate device buffers sized by the model
-    (void)hipFree(h->table); (void)hipFree(h->mu); (void)hipFree(h->sig);
-    (void)(void)hipFree(h->xA); (void)hipFree(h->PA); (void)hipFree(h->xB); (void)hipFree(h->PB);
-    h->table = nullptr; h->mu = nullptr; h->sig = nullptr;
+    (void)hipFree(h->xA); (void)hipFree(h->PA); (void)hipFree(h->xB); (void)hipFree(h->PB);
     h->xA = h->PA = h->xB = h->PB = nullptr;
     (void)hipFree(h->zbuf); (void)hipFree(h->cntbuf); (void)hipFree(h->candbuf);
     h->zbuf = nullptr; h->cntbuf = nullptr; h->candbuf = nullptr;
     h->model = false;
     h->tracks = false;
-    HIP_TRY(dalloc(&h->table, blob.size()));
-    HIP_TRY(dalloc(&h->mu, mu.size()));
-    HIP_TRY(dalloc(&h->sig, sg.size()));
     HIP_TRY(dalloc(&h->xA, (size_t)d * h->ld));
     HIP_TRY(dalloc(&h->xB, (size_t)d * h->ld));
     HIP_TRY(dalloc(&h->PA, (size_t)NP * h->ld));
     HIP_TRY(dalloc(&h->PB, (size_t)NP * h->ld));
-    HIP_TRY(hipMemcpyAsync(h->table, blob.data(), sizeof(float) * blob.size(), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->mu, mu.data(), sizeof(float) * mu.size(), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->sig, sg.data(), sizeof(float) * sg.size(), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    h->table_floats = (int)blob.size();
     h->clutter_mask = 0;
     h->order_packed = 0;
     for (int b = 0; b < J; ++b) h->order_packed |= (unsigned)b << (4 * b);
+    st = build_tables(h);
+    if (st) return st;
     h->model = true;
     return PF_OK;
 }
@@ -565,9 +750,11 @@ int pf_set_joint_clutter(pf_handle *h, const int32_t *clutter)
         if (!((mask >> b) & 1u)) order |= (unsigned)b << (4 * slot++);
     for (int b = 0; b < h->J; ++b)
         if ((mask >> b) & 1u) order |= (unsigned)b << (4 * slot++);
+    if (h->tracks && (mask != h->clutter_mask))
+        return set_error(PF_ERR_STATE, "joint kinds must be set before the tracks are initialised");
     h->clutter_mask = mask;
     h->order_packed = order;
-    return PF_OK;
+    return build_tables(h);
 }
 
 int pf_reset(pf_handle *h, const int32_t *bins)
@@ -593,10 +780,14 @@ int pf_set_tracks(pf_handle *h, const double *x, const double *P)
     if (st) return st;
     if (!h->model) return set_error(PF_ERR_STATE, "load a model first");
     if (!x || !P) return set_error(PF_ERR_ARG, "null tracks");
-    std::vector<float> xs, Ps;
-    pack_tracks(h, x, P, xs, Ps);
-    HIP_TRY(hipMemcpyAsync(h->xA, xs.data(), sizeof(float) * xs.size(), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->PA, Ps.data(), sizeof(float) * Ps.size(), hipMemcpyHostToDevice, h->stream));
+    const int D = h->D, N = h->N;
+    const size_t nx = (size_t)N * D, np = (size_t)N * D * D;
+    st = ensure_gbuf(h, sizeof(double) * (nx + np));
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(h->gbuf, x, sizeof(double) * nx, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->gbuf + nx, P, sizeof(double) * np, hipMemcpyHostToDevice, h->stream));
+    st = dispatch_d<ImportL>(D, h, (const double *)h->gbuf, (const double *)(h->gbuf + nx));
+    if (st) return st;
     hipLaunchKernelGGL(k_iota, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->anc, h->N);
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->tracks = true;
@@ -608,26 +799,23 @@ int pf_get_tracks(pf_handle *h, double *x, double *P)
     int st = check_handle(h);
     if (st) return st;
     if (!h->tracks) return set_error(PF_ERR_STATE, "tracks not initialised");
-    const int D = h->D, NP = npacked(D), N = h->N, ld = h->ld;
-    st = dispatch_d<GatherL>(D, h);
+    const int D = h->D, N = h->N;
+    const size_t nx = (size_t)N * D, np = (size_t)N * D * D;
+    st = ensure_gbuf(h, sizeof(double) * (nx + np));
     if (st) return st;
-    std::vector<float> xs((size_t)D * ld), Ps((size_t)NP * ld);
-    HIP_TRY(hipMemcpyAsync(xs.data(), h->xB, sizeof(float) * xs.size(), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipMemcpyAsync(Ps.data(), h->PB, sizeof(float) * Ps.size(), hipMemcpyDeviceToHost, h->stream));
+    st = dispatch_d<ExportL>(D, h, h->gbuf, h->gbuf + nx);
+    if (st) return st;
+    if (x) HIP_TRY(hipMemcpyAsync(x, h->gbuf, sizeof(double) * nx, hipMemcpyDeviceToHost, h->stream));
+    if (P) HIP_TRY(hipMemcpyAsync(P, h->gbuf + nx, sizeof(double) * np, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    for (int j = 0; j < N; ++j) {
-        if (x)
-            for (int i = 0; i < D; ++i) x[(size_t)j * D + i] = xs[(size_t)i * ld + j];
-        if (P)
-            for (int i = 0; i < D; ++i)
-                for (int k = 0; k < D; ++k) P[(size_t)j * D * D + i * D + k] = Ps[(size_t)pk(D, i, k) * ld + j];
-    }
     return PF_OK;
 }
 
 static int finish_host_step(pf_handle *h, float *lw, int32_t *anc, int32_t *comp, double *lml)
 {
-    int st = launch_resample_step(h, h->uni, h->lml1, 0);
+    int st = launch_weights(h, h->uni, 0);
+    if (st) return st;
+    st = launch_ancestors(h, h->N, h->N, h->anc, 0, 0, h->lml1);
     if (st) return st;
     swap_state(h);
     if (lw) HIP_TRY(hipMemcpyAsync(lw, h->lw, sizeof(float) * h->N, hipMemcpyDeviceToHost, h->stream));
@@ -655,7 +843,8 @@ int pf_step(pf_handle *h, const double *Z, uint32_t u_comp, uint32_t u_res, floa
     HIP_TRY(hipMemcpyAsync(h->zbuf, zf.data(), sizeof(float) * zf.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->uni, u, sizeof(u), hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
-    StepArgs a = step_args(h, kModeRef, h->uni, h->zbuf, 0, N, nullptr, nullptr, comp ? h->comp : nullptr, 0);
+    StepArgs a = step_args(h, kModeRef, h->uni, h->zbuf, 0, N, nullptr, nullptr, comp ? h->comp : nullptr, 0,
+                           kAncArray, nullptr);
     st = launch_step(h, a, kModeRef);
     if (st) return st;
     return finish_host_step(h, lw, anc, comp, lml);
@@ -680,7 +869,8 @@ int pf_step_candidates(pf_handle *h, const int32_t *count, const double *cand, u
     HIP_TRY(hipMemcpyAsync(h->candbuf, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->uni, u, sizeof(u), hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
-    StepArgs a = step_args(h, kModeCand, h->uni, nullptr, 0, 0, h->cntbuf, h->candbuf, comp ? h->comp : nullptr, 0);
+    StepArgs a = step_args(h, kModeCand, h->uni, nullptr, 0, 0, h->cntbuf, h->candbuf, comp ? h->comp : nullptr, 0,
+                           kAncArray, nullptr);
     st = launch_step(h, a, kModeCand);
     if (st) return st;
     return finish_host_step(h, lw, anc, comp, lml);
@@ -703,7 +893,7 @@ static int resample_generic(pf_handle *h, const double *w, int L, int N, uint32_
 {
     int st = check_handle(h);
     if (st) return st;
-    if (!w || !out || L <= 0 || N <= 0 || L > (1 << 26) || N > (1 << 26))
+    if (!w || !out || L <= 0 || N <= 0 || L > kMaxScanBlocks * kWeightsBlock || N > (1 << 26))
         return set_error(PF_ERR_ARG, "bad resample arguments (L=%d, N=%d)", L, N);
     st = ensure_resample_scratch(h, L);
     if (st) return st;
@@ -731,7 +921,11 @@ static int resample_generic(pf_handle *h, const double *w, int L, int N, uint32_
     wa.fs = h->fs;
     wa.frame = h->frame;
     wa.frame_next = nullptr;
+    wa.uni = duni;
     wa.L = L;
+    wa.parity = 0;
+    wa.ustride = 2;
+    wa.uoff = 1;
     const int nb = (L + kWeightsBlock - 1) / kWeightsBlock;
     if (kind == kLogF64)
         hipLaunchKernelGGL((k_weights<kLogF64>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, wa);
@@ -739,23 +933,7 @@ static int resample_generic(pf_handle *h, const double *w, int L, int N, uint32_
         hipLaunchKernelGGL((k_weights<kLinF64>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, wa);
     st = check_launch("k_weights");
     if (st) return st;
-    AncArgs an{};
-    an.cloc = h->cloc;
-    an.bsum = h->bsum;
-    an.bexp = h->bexp;
-    an.fs = h->fs;
-    an.frame = h->frame;
-    an.uni = duni;
-    an.lml = nullptr;
-    an.anc = dout;
-    an.L = L;
-    an.N = N;
-    an.nb = nb;
-    an.ustride = 2;
-    an.uoff = 1;
-    an.kind = kind;
-    hipLaunchKernelGGL(k_ancestors, dim3(nb), dim3(kWeightsBlock), 0, h->stream, an);
-    st = check_launch("k_ancestors");
+    st = launch_ancestors(h, L, N, dout, 0, kind == kLinF64, nullptr);
     if (st) return st;
     HIP_TRY(hipMemcpyAsync(out, dout, sizeof(int) * (size_t)N, hipMemcpyDeviceToHost, h->stream));
     int deg = 0;
@@ -855,14 +1033,18 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
             h->ev.push_back(e);
         }
     }
+    // Frame f reads frame counter f & 1 and writes the other one.  Frame 0
+    // reads the materialised ancestor array; later frames resolve their
+    // ancestors from the previous frame's weights inside the step kernel.
     auto one_frame = [&](int f) -> int {
         const int parity = f & 1;
-        StepArgs a = step_args(h, mode, uniforms, Z, zstride, h->N, count, cand, nullptr, parity);
+        const int amode = (f == 0) ? kAncArray : kAncFused;
+        StepArgs a = step_args(h, mode, uniforms, Z, zstride, h->N, count, cand, nullptr, parity, amode, lml);
         if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f], h->stream));
         int s = launch_step(h, a, mode);
         if (s) return s;
         if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f + 1], h->stream));
-        s = launch_resample_step(h, uniforms, lml, parity);
+        s = launch_weights(h, uniforms, parity);
         if (s) return s;
         swap_state(h);
         return PF_OK;
@@ -873,43 +1055,48 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
             st = one_frame(f);
             if (st) return st;
         }
-        if (timing) {
-            HIP_TRY(hipEventRecord(h->ev[2 * T + 1], h->stream));
-            HIP_TRY(hipEventSynchronize(h->ev[2 * T + 1]));
-            double sum = 0.0;
-            for (int f = 0; f < T; ++f) {
-                float ms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&ms, h->ev[2 * f], h->ev[2 * f + 1]));
-                sum += ms;
-            }
-            float tot = 0.f;
-            HIP_TRY(hipEventElapsedTime(&tot, h->ev[2 * T], h->ev[2 * T + 1]));
-            h->last_step_ms = sum;
-            h->last_frame_ms = tot;
-            h->timed_launches = T;
-        }
-        return PF_OK;
-    }
-    // Graph: two frames (A->B, B->A) captured once, replayed T/2 times.
-    const int pairs = T / 2;
-    if (pairs > 0) {
-        hipGraph_t g;
-        hipGraphExec_t ge;
-        HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        int s1 = one_frame(0);
-        int s2 = s1 ? s1 : one_frame(1);
-        hipError_t ec = hipStreamEndCapture(h->stream, &g);
-        if (s2) return s2;
-        if (ec != hipSuccess) return set_error(PF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
-        HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        for (int p = 0; p < pairs; ++p) HIP_TRY(hipGraphLaunch(ge, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        (void)hipGraphExecDestroy(ge);
-        (void)hipGraphDestroy(g);
-    }
-    if (T & 1) {
-        st = one_frame(T - 1);
+        if (timing) HIP_TRY(hipEventRecord(h->ev[2 * T + 1], h->stream));
+    } else {
+        st = one_frame(0);
         if (st) return st;
+        // Frames 1..T-1: an (odd, even) pair captured once and replayed.
+        const int pairs = (T - 1) / 2;
+        if (pairs > 0) {
+            hipGraph_t g;
+            hipGraphExec_t ge;
+            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            int s1 = one_frame(1);
+            int s2 = s1 ? s1 : one_frame(2);
+            hipError_t ec = hipStreamEndCapture(h->stream, &g);
+            if (s2) return s2;
+            if (ec != hipSuccess) return set_error(PF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+            HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            for (int p = 0; p < pairs; ++p) HIP_TRY(hipGraphLaunch(ge, h->stream));
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            (void)hipGraphExecDestroy(ge);
+            (void)hipGraphDestroy(g);
+        }
+        if ((T - 1) & 1) {
+            st = one_frame(T - 1);
+            if (st) return st;
+        }
+    }
+    // Materialise the last frame's ancestors and its log mean weight.
+    st = launch_ancestors(h, h->N, h->N, h->anc, (T - 1) & 1, 0, lml + (T - 1));
+    if (st) return st;
+    if (timing) {
+        HIP_TRY(hipEventSynchronize(h->ev[2 * T + 1]));
+        double sum = 0.0;
+        for (int f = 0; f < T; ++f) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, h->ev[2 * f], h->ev[2 * f + 1]));
+            sum += ms;
+        }
+        float tot = 0.f;
+        HIP_TRY(hipEventElapsedTime(&tot, h->ev[2 * T], h->ev[2 * T + 1]));
+        h->last_step_ms = sum;
+        h->last_frame_ms = tot;
+        h->timed_launches = T;
     }
     return PF_OK;
 }

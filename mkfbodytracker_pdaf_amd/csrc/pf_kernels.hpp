@@ -7,10 +7,20 @@
 //           Two copies: the step reads copy A at the ancestor index and writes
 //           copy B at the particle index (the resampling gather of
 //           pf2DRao.cpp:153-156 is folded into the next frame's loads).
-//   table   per component k: [g, g^2, (1-g) mu_k (D), (1-g^2) Sigma_k packed],
-//           then the shared H (m x D), BH (m) and the 2x2 joint blocks of R
-//           (r00, r01, r11 per joint).  Staged into LDS by every block.
-//   thr     component thresholds C_pi[k] * N * 2^32 (128-bit, lo/hi words).
+//   table   LDS image of the model: 2K words of component thresholds
+//           C_pi[k] * N * 2^32 (128-bit), then per component k
+//           [g, g^2, (1-g) mu_k (D), (1-g^2) Sigma_k packed], then the shared
+//           H (m x D), BH (m) and the 2x2 joint blocks of R (r00, r01, r11).
+//
+// Frame pipeline (two launches per frame):
+//   k_step    resolves each particle's ancestor from the previous frame's
+//             pending resampling (or reads a materialised ancestor array),
+//             gathers, predicts, updates, writes the log-weight and the block
+//             maximum (atomicMax of an order-preserving key).
+//   k_weights fixed-point weights q_i and per-block inclusive scans; block 0
+//             records the resampling uniform and advances the frame counter.
+// k_ancestors materialises the pending ancestors (end of a run, host steps,
+// stand-alone resampling).
 #pragma once
 
 #include "pf_common.hpp"
@@ -18,14 +28,27 @@
 namespace mkfbt {
 
 constexpr int kMaxJoints = 8;
-constexpr int kWeightsBlock = 256;
+constexpr int kWeightsBlock = 256;     // input block of the resampler scans
+constexpr int kMaxScanBlocks = 4096;   // => at most 2^20 resampled weights
 
 enum StepMode { kModeRef = 0, kModeCand = 1 };
+enum AncMode { kAncArray = 0, kAncFused = 1 };
 
 struct FrameScratch {
-    unsigned long long maxkey;   // ordered key of the largest finite weight (0 = none)
-    unsigned int done_est;       // completion counter of k_estimate blocks
-    unsigned int status;         // bit 0: a degenerate (all-zero) resampling happened
+    unsigned long long maxkey[2];  // per frame parity: ordered key of the largest finite weight (0 = none)
+    unsigned int done_est;         // completion counter of k_estimate blocks
+    unsigned int status;           // bit 0: a degenerate (all-zero) resampling happened
+    uint32_t U_pending;            // uniform of the pending resampling
+    uint32_t pad;
+};
+
+// The pending systematic resampling of one weight vector of length L into N
+// outputs: block-local inclusive prefix sums and block sums of q.
+struct Pending {
+    const unsigned long long *__restrict__ cloc;
+    const unsigned long long *__restrict__ bsum;
+    const double *__restrict__ bexp;
+    int L, nb;
 };
 
 struct StepArgs {
@@ -33,13 +56,15 @@ struct StepArgs {
     const float *__restrict__ Pin;
     float *__restrict__ xout;
     float *__restrict__ Pout;
-    const int *__restrict__ anc;          // nullptr: identity
+    int *__restrict__ anc;                // read (kAncArray) or written (kAncFused)
     float *__restrict__ lw;
     int *__restrict__ comp_out;           // optional
     FrameScratch *fs;
     const float *__restrict__ table;
     const int *__restrict__ frame;
     const uint32_t *__restrict__ uni;     // 2 per frame: U_comp, U_res
+    double *lml;                          // kAncFused: lml[t-1] of the consumed resampling
+    Pending pend;                         // kAncFused: previous frame's weights
     // reference mode: Z[t][r][zld]
     const float *__restrict__ Z;
     long long z_frame_stride;
@@ -47,8 +72,10 @@ struct StepArgs {
     const int *__restrict__ count;
     const float *__restrict__ cand;
     unsigned long long pi_total;
-    int table_floats;   // LDS image: 2K threshold words, then the float table (multiple of 4)
+    int table_floats;   // floats of the LDS model image (multiple of 4)
     int K, N, ld, m, J, cap, zld;
+    int parity;         // frame parity: selects maxkey[parity]
+    int anc_mode;
     unsigned int order_packed;   // joint processing order, 4 bits per slot
     unsigned int clutter_mask;
     float pdl;    // P_D / lambda
@@ -84,9 +111,128 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
+// ------------------------------------------------------------- resampling
+// Systematic resampling (pf2DRao.cpp:175-210) with fixed-point weights:
+//   q_i = floor(2^32 * w_i / max w) (or det_exp(lw_i - max lw)),
+//   C = inclusive prefix sum (exact, order independent),
+//   a_i = min{idx : C[idx] * N * 2^32 > (i * 2^32 + U) * C[L-1]}.
+// The prefix sum is held as block-local scans (cloc) plus block sums (bsum);
+// resolve_ancestor() finds a_i for one block of outputs.
+
+// LDS workspace of resolve_ancestor: nb + 1 prefix words, 2 staged input
+// blocks, wave totals and a few broadcast slots.
+__host__ __device__ constexpr int resolve_lds_words(int nb) { return nb + 1 + 2 * kWeightsBlock + 8; }
+
+// a_j for outputs j in [j0, j0 + blockDim) (j = j0 + threadIdx.x); every
+// thread of the block must call it.  w (u64 LDS words) is the workspace.
+__device__ int resolve_ancestor(const Pending &pd, int N, uint32_t U, int j0, unsigned long long *w,
+                                unsigned long long *total_out, double *bexp_sum_out)
+{
+    const int tid = threadIdx.x, nt = blockDim.x, nb = pd.nb;
+    const int lane = tid & 63, wid = tid >> 6, nwaves = (nt + 63) >> 6;
+    unsigned long long *pre = w;                      // nb + 1
+    unsigned long long *stg = w + nb + 1;             // 2 * kWeightsBlock
+    unsigned long long *misc = stg + 2 * kWeightsBlock;  // 8
+    // 1. exclusive prefix of the block sums (chunked scan across the block)
+    const int per = (nb + nt - 1) / nt;
+    const int c0 = min(nb, tid * per), c1 = min(nb, c0 + per);
+    unsigned long long loc = 0;
+    double eloc = 0.0;
+    for (int k = c0; k < c1; ++k) {
+        loc += pd.bsum[k];
+        if (bexp_sum_out) eloc += pd.bexp[k];
+    }
+    unsigned long long sc = loc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long n = __shfl_up(sc, o, 64);
+        if (lane >= o) sc += n;
+    }
+    if (bexp_sum_out) eloc = wave_sum(eloc);
+    if (lane == 63) misc[wid] = sc;
+    if (bexp_sum_out && lane == 0) reinterpret_cast<double *>(stg)[wid] = eloc;
+    __syncthreads();
+    unsigned long long run = sc - loc;
+    for (int k = 0; k < wid; ++k) run += misc[k];
+    unsigned long long total = 0;
+    for (int k = 0; k < nwaves; ++k) total += misc[k];
+    if (bexp_sum_out) {
+        double es = 0.0;
+        for (int k = 0; k < nwaves; ++k) es += reinterpret_cast<double *>(stg)[k];
+        *bexp_sum_out = es;
+    }
+    for (int k = c0; k < c1; ++k) {
+        pre[k] = run;
+        run += pd.bsum[k];
+    }
+    if (tid == 0) pre[nb] = total;
+    *total_out = total;
+    __syncthreads();
+    // 2. the input block of each output: largest b with pre[b] * S <= t_j
+    const int j = j0 + tid;
+    const int jc = min(j, N - 1);
+    const u128 S = (u128)(unsigned long long)N << 32;
+    const u128 tj = sys_target((uint32_t)jc, U, total);
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((u128)pre[mid] * S <= tj) lo = mid;
+        else hi = mid - 1;
+    }
+    const int bj = lo;
+    const int jlast = min(j0 + nt, N) - 1;
+    if (j == j0) misc[4] = (unsigned long long)bj;
+    if (j == jlast) misc[5] = (unsigned long long)bj;
+    __syncthreads();
+    const int blo = (int)misc[4], bhi = (int)misc[5];
+    int a;
+    if (bhi - blo <= 1) {
+        // 3a. stage C = pre + cloc of the (at most two) input blocks in LDS
+        const int base = blo * kWeightsBlock;
+        const int n = min(pd.L, (bhi + 1) * kWeightsBlock) - base;
+        for (int k = tid; k < n; k += nt) stg[k] = pre[blo + k / kWeightsBlock] + pd.cloc[base + k];
+        __syncthreads();
+        int l = (bj - blo) * kWeightsBlock, h = min(n, (bj - blo + 1) * kWeightsBlock) - 1;
+        while (l < h) {
+            const int mid = (l + h) >> 1;
+            if ((u128)stg[mid] * S > tj) h = mid;
+            else l = mid + 1;
+        }
+        a = base + l;
+    } else {
+        // 3b. sparse weights: search the input block in global memory
+        const unsigned long long pb = pre[bj];
+        int l = bj * kWeightsBlock, h = min(pd.L, (bj + 1) * kWeightsBlock) - 1;
+        while (l < h) {
+            const int mid = (l + h) >> 1;
+            if ((u128)(pb + pd.cloc[mid]) * S > tj) h = mid;
+            else l = mid + 1;
+        }
+        a = l;
+    }
+    return a;
+}
+
+// Frame statistics of a consumed resampling (block 0, thread 0): the log
+// mean weight, the degenerate flag, and the reset of the maximum slot.
+__device__ __forceinline__ void finish_resampling(FrameScratch *fs, int parity, double bexp_sum, int N,
+                                                  int kind_linear, double *lml_slot)
+{
+    const unsigned long long key = fs->maxkey[parity];
+    if (key == 0ull) fs->status |= 1u;
+    if (lml_slot) {
+        double l;
+        if (key == 0ull) l = -INFINITY;
+        else if (kind_linear) l = log(bexp_sum / (double)N);
+        else l = key_value(key) + log(bexp_sum) - log((double)N);
+        *lml_slot = l;
+    }
+    fs->maxkey[parity] = 0ull;
+}
+
 // --------------------------------------------------------------- step kernel
 // One lane = one particle.  Fuses, for particle j of frame t:
-//   ancestor gather (pf2DRao.cpp:155), component draw resample(gmm.weight, N)
+//   ancestor (pf2DRao.cpp:152-156), component draw resample(gmm.weight, N)
 //   (:128, systematic, thresholds precomputed), KF_model::predict with
 //   F = g I (KF_model.cpp:11-15, my_gmm.cpp:53-58), the innovation likelihood
 //   (:138) and KF_model::update (KF_model.cpp:17-25), processed as sequential
@@ -141,6 +287,30 @@ struct Joint {
         }
     }
 
+    // Same quantities when the state basis makes H_b the selection of rows
+    // r0, r0 + 1: H_b P is the pair of rows of P, y the pair of entries of x.
+    __device__ static __forceinline__ void prepare_sel(const float (&x)[D], const float (&P)[NP], int r0,
+                                                       float bh0, float bh1, float r00, float r01, float r11,
+                                                       float (&w0)[D], float (&w1)[D], float &y0, float &y1,
+                                                       float &il00, float &l10, float &il11, float &logl)
+    {
+        const int r1 = r0 + 1;
+        y0 = x[r0] + bh0;
+        y1 = x[r1] + bh1;
+        const float s00 = P[pk(D, r0, r0)] + r00, s01 = P[pk(D, r0, r1)] + r01, s11 = P[pk(D, r1, r1)] + r11;
+        const float l00 = sqrtf(s00);
+        il00 = 1.f / l00;
+        l10 = s01 * il00;
+        const float l11 = sqrtf(s11 - l10 * l10);
+        il11 = 1.f / l11;
+        logl = logf(l00 * l11);
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            w0[c] = P[pk(D, r0, c)] * il00;
+            w1[c] = (P[pk(D, r1, c)] - l10 * w0[c]) * il11;
+        }
+    }
+
     // x += W^T v ; P -= W^T M W with M = [[m00, m01], [m01, m11]].
     __device__ static __forceinline__ void apply(float (&x)[D], float (&P)[NP], const float (&w0)[D],
                                                  const float (&w1)[D], float v0, float v1, float m00,
@@ -156,7 +326,10 @@ struct Joint {
 #pragma unroll
         for (int i = 0; i < D; ++i)
 #pragma unroll
-            for (int k = i; k < D; ++k) P[pk(D, i, k)] -= fmaf(w0[i], u0[k], w1[i] * u1[k]);
+            for (int k = i; k < D; ++k) {
+                float p = fmaf(-w0[i], u0[k], P[pk(D, i, k)]);
+                P[pk(D, i, k)] = fmaf(-w1[i], u1[k], p);
+            }
     }
 };
 
@@ -164,7 +337,7 @@ struct Joint {
 // issued before the first LDS write so the copy costs one memory latency.
 __device__ __forceinline__ void stage_table(float4 *dst, const float4 *__restrict__ src, int n4)
 {
-    constexpr int kChunk = 8;
+    constexpr int kChunk = 4;
     for (int base = 0; base < n4; base += kChunk * (int)blockDim.x) {
         float4 r[kChunk];
 #pragma unroll
@@ -178,7 +351,6 @@ __device__ __forceinline__ void stage_table(float4 *dst, const float4 *__restric
             if (i < n4) dst[i] = r[it];
         }
     }
-    __syncthreads();
 }
 
 // Structure-of-arrays rows through buffer instructions: one resource per
@@ -212,31 +384,134 @@ __device__ __forceinline__ int draw_component(const unsigned long long *thr, int
     return k;
 }
 
-template <int D, int MODE>
+// Detections of joint b in frame t (candidate mode) or the per-particle
+// measurement (reference mode).  Returns false if a certain joint has no
+// detection this frame (the joint is skipped).
+template <int MODE>
+__device__ __forceinline__ bool joint_inputs(const StepArgs &a, int t, int b, int &cnt, const float *&cz,
+                                             bool &clutter)
+{
+    clutter = (MODE == kModeCand) && ((a.clutter_mask >> b) & 1u);
+    cnt = 1;
+    cz = nullptr;
+    if (MODE == kModeCand) {
+        cnt = a.count[(size_t)t * a.J + b];
+        cz = a.cand + ((size_t)t * a.J + b) * a.cap * 2;
+        if (cnt <= 0 && !clutter) return false;
+        if (cnt > a.cap) cnt = a.cap;
+    }
+    return true;
+}
+
+// Likelihood of one prepared joint block and the parameters of its update
+// x += W^T v, P -= W^T M W (M = I for a certain measurement); returns the
+// joint's log-likelihood term.
+template <int MODE>
+__device__ __forceinline__ float joint_measure(const StepArgs &a, int t, int j, int b, bool clutter, int cnt,
+                                               const float *cz, float y0, float y1, float il00, float l10,
+                                               float il11, float logl, float &v0, float &v1, float &m00,
+                                               float &m01, float &m11)
+{
+    const float LOG2PI = 1.8378770664093453f;
+    if (!clutter) {
+        float z0, z1;
+        if (MODE == kModeRef) {
+            const float *Zt = a.Z + (size_t)t * a.z_frame_stride;
+            z0 = Zt[(size_t)(2 * b) * a.zld + j];
+            z1 = Zt[(size_t)(2 * b + 1) * a.zld + j];
+        } else {
+            z0 = cz[0];
+            z1 = cz[1];
+        }
+        v0 = (z0 - y0) * il00;
+        v1 = ((z1 - y1) - l10 * v0) * il11;
+        m00 = 1.f;
+        m01 = 0.f;
+        m11 = 1.f;
+        return -0.5f * fmaf(v0, v0, v1 * v1) - logl - LOG2PI;
+    }
+    // PDAF over the gated detections (Bar-Shalom & Fortmann), in whitened
+    // coordinates u_c = L^-1 (z_c - y).
+    const float lnorm = -logl - LOG2PI;
+    float se = 0.f, sv0 = 0.f, sv1 = 0.f, q00 = 0.f, q01 = 0.f, q11 = 0.f;
+    for (int c = 0; c < cnt; ++c) {
+        const float n0 = cz[2 * c] - y0, n1 = cz[2 * c + 1] - y1;
+        const float u0 = n0 * il00;
+        const float u1 = (n1 - l10 * u0) * il11;
+        const float d2 = fmaf(u0, u0, u1 * u1);
+        if (d2 <= a.gate) {
+            const float e = a.pdl * __expf(lnorm - 0.5f * d2);
+            se += e;
+            sv0 = fmaf(e, u0, sv0);
+            sv1 = fmaf(e, u1, sv1);
+            q00 = fmaf(e * u0, u0, q00);
+            q01 = fmaf(e * u0, u1, q01);
+            q11 = fmaf(e * u1, u1, q11);
+        }
+    }
+    const float ell = a.b0 + se;
+    const float inv = 1.f / ell;
+    v0 = sv0 * inv;
+    v1 = sv1 * inv;
+    const float bsum = se * inv;  // 1 - beta_0
+    m00 = bsum - (q00 * inv - v0 * v0);
+    m01 = -(q01 * inv - v0 * v1);
+    m11 = bsum - (q11 * inv - v1 * v1);
+    return logf(ell);
+}
+
+// LDS bytes of k_step: model image + ancestor workspace (fused mode).
+__host__ __device__ constexpr int step_lds_bytes(int table_floats, int nb_pending)
+{
+    return 4 * table_floats + (nb_pending > 0 ? 8 * resolve_lds_words(nb_pending) : 0);
+}
+
+template <int D, int MODE, int SELJ>
 __global__ void __launch_bounds__(256) k_step(StepArgs a)
 {
     constexpr int NP = npacked(D);
     constexpr int CS = 2 + D + NP;
     extern __shared__ float4 sm4[];
-    stage_table(sm4, reinterpret_cast<const float4 *>(a.table), a.table_floats >> 2);
-    const unsigned long long *sthr = reinterpret_cast<const unsigned long long *>(sm4);
-    const float *stab = reinterpret_cast<const float *>(sm4) + 4 * a.K;
-
     const int t = *a.frame;
     const uint32_t U1 = a.uni[2 * t];
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    float lwj = -INFINITY;
-    if (j < a.N) {
-        const int src = a.anc ? a.anc[j] : j;
-        const unsigned rowb = (unsigned)a.ld * 4u;
-        const unsigned voff = (unsigned)src * 4u;
+    const int j0 = blockIdx.x * blockDim.x;
+    const int j = j0 + threadIdx.x;
+    const bool live = j < a.N;
+    int src;
+    if (a.anc_mode == kAncFused) {
+        // Consume the previous frame's resampling: a_j, then the frame's
+        // statistics (block 0).
+        unsigned long long *wsp = reinterpret_cast<unsigned long long *>(
+            reinterpret_cast<float *>(sm4) + a.table_floats);
+        unsigned long long total;
+        double es = 0.0;
+        src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, &total,
+                               blockIdx.x == 0 ? &es : nullptr);
+        if (live) a.anc[j] = src;
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            finish_resampling(a.fs, a.parity ^ 1, es, a.N, 0, a.lml ? a.lml + (t - 1) : nullptr);
+    } else {
+        src = live ? a.anc[j] : 0;
+    }
+    // Gathered state loads, then the model staging; the LDS barrier waits for
+    // both, so they share one memory latency.
+    const unsigned rowb = (unsigned)a.ld * 4u;
+    const unsigned voff = (unsigned)src * 4u;
+    float x[D], P[NP];
+    if (live) {
         const __amdgpu_buffer_rsrc_t rx = soa_rsrc(a.xin, rowb * D), rP = soa_rsrc(a.Pin, rowb * NP);
-        float x[D], P[NP];
 #pragma unroll
         for (int i = 0; i < D; ++i) x[i] = soa_ld(rx, voff, rowb * i);
 #pragma unroll
         for (int p = 0; p < NP; ++p) P[p] = soa_ld(rP, voff, rowb * p);
+    }
+    stage_table(sm4, reinterpret_cast<const float4 *>(a.table), a.table_floats >> 2);
+    __syncthreads();
+    const unsigned long long *sthr = reinterpret_cast<const unsigned long long *>(sm4);
+    const float *stab = reinterpret_cast<const float *>(sm4) + 4 * a.K;
 
+    float lwj = -INFINITY;
+    if (live) {
         const int k = draw_component(sthr, a.K, (uint32_t)j, U1, a.pi_total);
         if (a.comp_out) a.comp_out[j] = k;
         const float *ck = stab + k * CS;
@@ -250,64 +525,36 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
         const float *BHs = Hs + a.m * D;
         const float *Rs = BHs + a.m;
         float acc = 0.f;
-        const float LOG2PI = 1.8378770664093453f;
-        for (int oi = 0; oi < a.J; ++oi) {
-            const int b = (MODE == kModeRef) ? oi : (int)((a.order_packed >> (4 * oi)) & 15u);
-            const bool clutter = (MODE == kModeCand) && ((a.clutter_mask >> b) & 1u);
-            int cnt = 1;
-            const float *cz = nullptr;
-            if (MODE == kModeCand) {
-                cnt = a.count[(size_t)t * a.J + b];
-                cz = a.cand + ((size_t)t * a.J + b) * a.cap * 2;
-                if (cnt <= 0 && !clutter) continue;
-                if (cnt > a.cap) cnt = a.cap;
+        if constexpr (SELJ > 0) {
+#pragma unroll
+            for (int oi = 0; oi < SELJ; ++oi) {
+                const int b = (int)((a.order_packed >> (4 * oi)) & 15u);
+                int cnt;
+                const float *cz;
+                bool clutter;
+                if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter)) continue;
+                float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
+                Joint<D>::prepare_sel(x, P, 2 * oi, BHs[2 * b], BHs[2 * b + 1], Rs[3 * b], Rs[3 * b + 1],
+                                      Rs[3 * b + 2], w0, w1, y0, y1, il00, l10, il11, logl);
+                float v0, v1, m00, m01, m11;
+                acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
+                                           v1, m00, m01, m11);
+                Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
             }
-            float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
-            Joint<D>::prepare(x, P, Hs + (2 * b) * D, Hs + (2 * b + 1) * D, BHs[2 * b], BHs[2 * b + 1],
-                              Rs[3 * b], Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0, y1, il00, l10,
-                              il11, logl);
-            if (!clutter) {
-                float z0, z1;
-                if (MODE == kModeRef) {
-                    const float *Zt = a.Z + (size_t)t * a.z_frame_stride;
-                    z0 = Zt[(size_t)(2 * b) * a.zld + j];
-                    z1 = Zt[(size_t)(2 * b + 1) * a.zld + j];
-                } else {
-                    z0 = cz[0];
-                    z1 = cz[1];
-                }
-                const float v0 = (z0 - y0) * il00;
-                const float v1 = ((z1 - y1) - l10 * (z0 - y0) * il00) * il11;
-                acc += -0.5f * fmaf(v0, v0, v1 * v1) - logl - LOG2PI;
-                Joint<D>::apply(x, P, w0, w1, v0, v1, 1.f, 0.f, 1.f);
-            } else {
-                // PDAF over the gated detections (Bar-Shalom & Fortmann), in
-                // whitened coordinates v_c = L^-1 (z_c - y).
-                const float lnorm = -logl - LOG2PI;
-                float se = 0.f, sv0 = 0.f, sv1 = 0.f, q00 = 0.f, q01 = 0.f, q11 = 0.f;
-                for (int c = 0; c < cnt; ++c) {
-                    const float n0 = cz[2 * c] - y0, n1 = cz[2 * c + 1] - y1;
-                    const float v0 = n0 * il00;
-                    const float v1 = (n1 - l10 * v0) * il11;
-                    const float d2 = fmaf(v0, v0, v1 * v1);
-                    if (d2 <= a.gate) {
-                        const float e = a.pdl * __expf(lnorm - 0.5f * d2);
-                        se += e;
-                        sv0 = fmaf(e, v0, sv0);
-                        sv1 = fmaf(e, v1, sv1);
-                        q00 = fmaf(e * v0, v0, q00);
-                        q01 = fmaf(e * v0, v1, q01);
-                        q11 = fmaf(e * v1, v1, q11);
-                    }
-                }
-                const float ell = a.b0 + se;
-                const float inv = 1.f / ell;
-                const float v0 = sv0 * inv, v1 = sv1 * inv;
-                const float bsum = se * inv;  // 1 - beta_0
-                const float m00 = bsum - (q00 * inv - v0 * v0);
-                const float m01 = -(q01 * inv - v0 * v1);
-                const float m11 = bsum - (q11 * inv - v1 * v1);
-                acc += logf(ell);
+        } else {
+            for (int oi = 0; oi < a.J; ++oi) {
+                const int b = (MODE == kModeRef) ? oi : (int)((a.order_packed >> (4 * oi)) & 15u);
+                int cnt;
+                const float *cz;
+                bool clutter;
+                if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter)) continue;
+                float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
+                Joint<D>::prepare(x, P, Hs + (2 * b) * D, Hs + (2 * b + 1) * D, BHs[2 * b], BHs[2 * b + 1],
+                                  Rs[3 * b], Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0, y1, il00, l10,
+                                  il11, logl);
+                float v0, v1, m00, m01, m11;
+                acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
+                                           v1, m00, m01, m11);
                 Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
             }
         }
@@ -330,20 +577,11 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
     if (threadIdx.x == 0) {
         double m = wmax[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, wmax[w]);
-        if (isfinite(m)) atomicMax(&a.fs->maxkey, order_key(m));
+        if (isfinite(m)) atomicMax(&a.fs->maxkey[a.parity], order_key(m));
     }
 }
 
-// ------------------------------------------------------------- resampling
-// Systematic resampling (pf2DRao.cpp:175-210) with fixed-point weights:
-//   q_i = floor(2^32 * w_i / max w) (or det_exp(lw_i - max lw)),
-//   C = inclusive prefix sum (exact, order independent),
-//   a_i = min{idx : C[idx] * N * 2^32 > (i * 2^32 + U) * C[L-1]}.
-// Equivalently input idx owns the outputs [g(C[idx-1]), g(C[idx])) with
-//   g(c) = min{i in [0, N] : (i * 2^32 + U) * T >= c * N * 2^32}.
-// k_weights: per-block inclusive scans of q (no cross-block traffic).
-// k_ancestors: block prefix from the block sums, g() of every input, then the
-// block writes its contiguous output range (binary search in LDS).
+// Weights of the resampler: q_i and per-block inclusive scans.
 enum WeightKind { kLogF32 = 0, kLogF64 = 1, kLinF64 = 2 };
 
 struct WeightArgs {
@@ -355,7 +593,8 @@ struct WeightArgs {
     FrameScratch *fs;
     const int *frame;             // frame counter read this frame
     int *frame_next;              // frame counter of the next frame (block 0 writes t + 1)
-    int L;
+    const uint32_t *uni;          // U = uni[t * ustride + uoff]
+    int L, parity, ustride, uoff;
 };
 
 template <int KIND>
@@ -363,7 +602,7 @@ __global__ void __launch_bounds__(kWeightsBlock) k_weights(WeightArgs a)
 {
     __shared__ unsigned long long s_wave[4];
     __shared__ double s_dwave[4];
-    const unsigned long long key = a.fs->maxkey;
+    const unsigned long long key = a.fs->maxkey[a.parity];
     const bool degenerate = (key == 0ull);
     const double ref = degenerate ? 0.0 : key_value(key);
     const int i = blockIdx.x * kWeightsBlock + threadIdx.x;
@@ -402,89 +641,34 @@ __global__ void __launch_bounds__(kWeightsBlock) k_weights(WeightArgs a)
         a.bsum[blockIdx.x] = s;
         a.bexp[blockIdx.x] = (s_dwave[0] + s_dwave[1]) + (s_dwave[2] + s_dwave[3]);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && a.frame_next) *a.frame_next = *a.frame + 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int t = *a.frame;
+        a.fs->U_pending = a.uni[(size_t)t * a.ustride + a.uoff];
+        if (a.frame_next) *a.frame_next = t + 1;
+    }
 }
 
+// Materialises the pending resampling: anc[j] for every output j, and the
+// statistics of the resampled frame (block 0).
 struct AncArgs {
-    const unsigned long long *cloc;
-    const unsigned long long *bsum;
-    const double *bexp;
+    Pending pend;
     FrameScratch *fs;
-    const int *frame;
-    const uint32_t *uni;          // U = uni[t * ustride + uoff]
-    double *lml;                  // optional: lml[t] = log mean weight
     int *anc;
-    int L, N, nb, ustride, uoff, kind;
+    double *lml;     // optional: written with the frame's log mean weight
+    int N, parity, kind_linear;
 };
 
-// g(c): first output whose systematic target reaches cumulative weight c.
-__device__ __forceinline__ int first_output(unsigned long long c, uint32_t U, unsigned long long T, int N)
+__global__ void __launch_bounds__(256) k_ancestors(AncArgs a)
 {
-    const u128 cs = (u128)c * ((u128)(unsigned long long)N << 32);
-    double est = (double)c * (double)N / (double)T - (double)U * 2.3283064365386963e-10;
-    long long i0 = (long long)ceil(est);
-    if (i0 < 0) i0 = 0;
-    if (i0 > N) i0 = N;
-    while (i0 > 0 && sys_target((uint32_t)(i0 - 1), U, T) >= cs) --i0;
-    while (i0 < N && sys_target((uint32_t)i0, U, T) < cs) ++i0;
-    return (int)i0;
-}
-
-__global__ void __launch_bounds__(kWeightsBlock) k_ancestors(AncArgs a)
-{
-    __shared__ unsigned long long s_pre[4], s_tot[4];
-    __shared__ double s_e[4];
-    __shared__ int s_hi[kWeightsBlock];
-    __shared__ int s_lo;
-    const int t = *a.frame;
-    const uint32_t U = a.uni[(size_t)t * a.ustride + a.uoff];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int b = blockIdx.x;
-    // prefix of the block sums before this block, and the total
-    unsigned long long pre = 0, tot = 0;
-    double e = 0.0;
-    for (int k = threadIdx.x; k < a.nb; k += kWeightsBlock) {
-        const unsigned long long v = a.bsum[k];
-        tot += v;
-        if (k < b) pre += v;
-        if (b == 0) e += a.bexp[k];
-    }
-    pre = wave_sum(pre);
-    tot = wave_sum(tot);
-    e = wave_sum(e);
-    if (lane == 0) { s_pre[wid] = pre; s_tot[wid] = tot; s_e[wid] = e; }
-    __syncthreads();
-    pre = (s_pre[0] + s_pre[1]) + (s_pre[2] + s_pre[3]);
-    tot = (s_tot[0] + s_tot[1]) + (s_tot[2] + s_tot[3]);
-    const int i0 = b * kWeightsBlock;
-    const int idx = i0 + threadIdx.x;
-    const int n_in = min(kWeightsBlock, a.L - i0);
-    if (threadIdx.x < n_in) s_hi[threadIdx.x] = first_output(pre + a.cloc[idx], U, tot, a.N);
-    if (threadIdx.x == 0) s_lo = first_output(pre, U, tot, a.N);
-    __syncthreads();
-    const int lo = s_lo, hi = s_hi[n_in - 1];
-    for (int i = lo + threadIdx.x; i < hi; i += kWeightsBlock) {
-        int l = 0, h = n_in - 1;           // smallest k with s_hi[k] > i
-        while (l < h) {
-            const int mid = (l + h) >> 1;
-            if (s_hi[mid] > i) h = mid;
-            else l = mid + 1;
-        }
-        a.anc[i] = i0 + l;
-    }
-    if (b == 0 && threadIdx.x == 0) {
-        const unsigned long long key = a.fs->maxkey;
-        const double se = (s_e[0] + s_e[1]) + (s_e[2] + s_e[3]);
-        if (key == 0ull) a.fs->status |= 1u;
-        if (a.lml) {
-            double l;
-            if (key == 0ull) l = -INFINITY;
-            else if (a.kind == kLinF64) l = log(se / (double)a.N);
-            else l = key_value(key) + log(se) - log((double)a.N);
-            a.lml[t] = l;
-        }
-        a.fs->maxkey = 0ull;
-    }
+    extern __shared__ unsigned long long wsp[];
+    const int j0 = blockIdx.x * blockDim.x;
+    const int j = j0 + threadIdx.x;
+    unsigned long long total;
+    double es = 0.0;
+    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, &total,
+                                     blockIdx.x == 0 ? &es : nullptr);
+    if (j < a.N) a.anc[j] = src;
+    if (blockIdx.x == 0 && threadIdx.x == 0) finish_resampling(a.fs, a.parity, es, a.N, a.kind_linear, a.lml);
 }
 
 // block maximum of FP64 host weights (for the stand-alone resample entry points)
@@ -498,7 +682,7 @@ __global__ void __launch_bounds__(256) k_blockmax64(const double *w, int L, Fram
         if (isfinite(x) && (KIND == kLogF64 || x > 0.0)) v = x;
     }
     v = wave_max(v);
-    if ((threadIdx.x & 63) == 0 && isfinite(v)) atomicMax(&fs->maxkey, order_key(v));
+    if ((threadIdx.x & 63) == 0 && isfinite(v)) atomicMax(&fs->maxkey[0], order_key(v));
 }
 
 // ---------------------------------------------------------------- helpers
@@ -528,26 +712,81 @@ __global__ void __launch_bounds__(256) k_reset(const int *bins, const float *mu,
     for (int p = 0; p < NP; ++p) P[(size_t)p * ld + j] = sig[b * NP + p];
 }
 
-// Gather tracks[j] = updated[anc_j] (the copy of pf2DRao.cpp:153-156).
+// Tracks in: x' = T x, P' = T sym(P) T^T (FP64), stored FP32 SoA packed.
 template <int D>
-__global__ void __launch_bounds__(256) k_gather(const float *xin, const float *Pin, const int *anc,
-                                                float *xout, float *Pout, int N, int ld)
+__global__ void __launch_bounds__(256) k_import(const double *x, const double *P, const double *T, float *xs,
+                                                float *Ps, int N, int ld)
+{
+    __shared__ double sT[D * D];
+    for (int i = threadIdx.x; i < D * D; i += blockDim.x) sT[i] = T[i];
+    __syncthreads();
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const double *xj = x + (size_t)j * D, *Pj = P + (size_t)j * D * D;
+    for (int i = 0; i < D; ++i) {
+        double acc = 0.0;
+        for (int k = 0; k < D; ++k) acc = fma(sT[i * D + k], xj[k], acc);
+        xs[(size_t)i * ld + j] = (float)acc;
+    }
+    for (int i = 0; i < D; ++i) {
+        double r[D];
+        for (int k = 0; k < D; ++k) {
+            double acc = 0.0;
+            for (int l = 0; l < D; ++l) acc = fma(sT[i * D + l], 0.5 * (Pj[l * D + k] + Pj[k * D + l]), acc);
+            r[k] = acc;
+        }
+        for (int q = i; q < D; ++q) {
+            double acc = 0.0;
+            for (int k = 0; k < D; ++k) acc = fma(r[k], sT[q * D + k], acc);
+            Ps[(size_t)pk(D, i, q) * ld + j] = (float)acc;
+        }
+    }
+}
+
+// Tracks out: tracks[j] = updated[anc_j] (the copy of pf2DRao.cpp:153-156),
+// back in the model basis: x = Ti x', P = Ti P' Ti^T (FP64, row-major).
+template <int D>
+__global__ void __launch_bounds__(256) k_export(const float *xs, const float *Ps, const int *anc,
+                                                const double *Ti, double *x, double *P, int N, int ld)
 {
     constexpr int NP = npacked(D);
+    __shared__ double sT[D * D];
+    for (int i = threadIdx.x; i < D * D; i += blockDim.x) sT[i] = Ti[i];
+    __syncthreads();
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     const int s = anc ? anc[j] : j;
-#pragma unroll
-    for (int i = 0; i < D; ++i) xout[(size_t)i * ld + j] = xin[(size_t)i * ld + s];
-#pragma unroll
-    for (int p = 0; p < NP; ++p) Pout[(size_t)p * ld + j] = Pin[(size_t)p * ld + s];
+    double xp[D], pp[NP];
+    for (int i = 0; i < D; ++i) xp[i] = (double)xs[(size_t)i * ld + s];
+    for (int p = 0; p < NP; ++p) pp[p] = (double)Ps[(size_t)p * ld + s];
+    double *xo = x + (size_t)j * D, *Po = P + (size_t)j * D * D;
+    for (int i = 0; i < D; ++i) {
+        double acc = 0.0;
+        for (int k = 0; k < D; ++k) acc = fma(sT[i * D + k], xp[k], acc);
+        xo[i] = acc;
+    }
+    for (int i = 0; i < D; ++i) {
+        double r[D];
+        for (int k = 0; k < D; ++k) {
+            double acc = 0.0;
+            for (int l = 0; l < D; ++l) acc = fma(sT[i * D + l], pp[pk(D, l, k)], acc);
+            r[k] = acc;
+        }
+        for (int q = i; q < D; ++q) {
+            double acc = 0.0;
+            for (int k = 0; k < D; ++k) acc = fma(r[k], sT[q * D + k], acc);
+            Po[i * D + q] = acc;
+            Po[q * D + i] = acc;
+        }
+    }
 }
 
 // getEstimator (pf2DRao.cpp:23-31): xbar = sum_j x[anc_j] / N, FP64
 // accumulation, fixed reduction order (block partials, then the last block).
 template <int D>
 __global__ void __launch_bounds__(256) k_estimate(const float *x, const int *anc, int N, int ld,
-                                                  double *partial, FrameScratch *fs, double *out)
+                                                  double *partial, FrameScratch *fs, const double *Ti,
+                                                  double *out)
 {
     __shared__ double s_w[4][D];
     __shared__ int s_last;
@@ -575,10 +814,17 @@ __global__ void __launch_bounds__(256) k_estimate(const float *x, const int *anc
     }
     __syncthreads();
     if (!s_last) return;
+    __shared__ double s_mean[D];
     if (threadIdx.x < D) {
         double r = 0.0;
         for (int b = 0; b < (int)gridDim.x; ++b) r += ((volatile double *)partial)[(size_t)b * D + threadIdx.x];
-        out[threadIdx.x] = r / (double)N;
+        s_mean[threadIdx.x] = r / (double)N;
+    }
+    __syncthreads();
+    if (threadIdx.x < D) {          // back to the model basis: xbar = Ti xbar'
+        double r = 0.0;
+        for (int k = 0; k < D; ++k) r = fma(Ti[threadIdx.x * D + k], s_mean[k], r);
+        out[threadIdx.x] = r;
     }
     if (threadIdx.x == 0) fs->done_est = 0u;
 }

@@ -199,8 +199,8 @@ def test_estimate_matches_oracle(gpu):
     P = np.repeat(np.eye(mdl.d)[None], N, 0)
     rb.set_tracks(x, P)
     e = rb.estimate()
-    xf = x.astype(np.float32).astype(np.float64)
-    assert np.allclose(e, oracle.rbpf_estimate(xf), rtol=1e-12, atol=1e-9)
+    # the device stores FP32 states (in its state basis): FP32 storage error bound
+    assert np.max(np.abs(e - oracle.rbpf_estimate(x))) <= 1e-6 * np.abs(x).max()
     rb.close()
 
 
@@ -213,7 +213,7 @@ def test_proposal_sampling(gpu):
     Hp = mdl.pca_proj.T.copy()
     M = mdl.pca_mean.copy()
     normals = np.random.default_rng(1).standard_normal((2, n))
-    xbar = oracle.rbpf_estimate(x0.astype(np.float32).astype(np.float64))
+    xbar = rb.estimate()        # the estimator has its own test; proposals are checked given it
     s = rb.get_samples(Hp, M, n, 42.0, normals)
     so = oracle.get_samples(Hp, M, xbar, n, 42.0, normals)
     assert np.allclose(s, so, rtol=1e-9, atol=1e-9)
@@ -244,3 +244,50 @@ def test_kf_model_predict_update(gpu):
         assert np.allclose(xg2[i], xo2, rtol=1e-9, atol=1e-9)
         assert np.allclose(Pg2[i], Po2, rtol=1e-9, atol=1e-9)
     rb.close()
+
+
+# ------------------------------------------------- device-resident sequences
+@pytest.mark.parametrize("mode", ["candidates", "reference"])
+@pytest.mark.parametrize("flags", [0, 1])
+@pytest.mark.parametrize("T", [1, 2, 7])
+def test_device_run_matches_host_steps(gpu, mode, flags, T):
+    """pf_run_*_dev (ancestors resolved inside the next step kernel, optional
+    hipGraph replay) gives bit-identical tracks, ancestors and per-frame log
+    mean weights to the same frames driven one host call at a time."""
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    N = 3000
+    mdl, rb, om = _setup(N)
+    mdl2, rb2, _ = _setup(N)
+    seq = synthetic.simulate(mdl, T, seed=31, cand_per_joint=4, cap=8)
+    if mode == "candidates":
+        rb.set_joint_clutter(mdl.clutter)
+        rb2.set_joint_clutter(mdl.clutter)
+    bins, _ = oracle.resample_linear(mdl.weights, N, 99)
+    rb.reset(bins)
+    rb2.reset(bins)
+    lml_host = []
+    Zs = []
+    for t in range(T):
+        U1, U2 = int(seq.uniforms[t, 0]), int(seq.uniforms[t, 1])
+        if mode == "candidates":
+            st, out = rb.step_candidates(seq.count[t], seq.cand[t], U1, U2)
+        else:
+            Z = synthetic.per_particle_measurements(seq.z[t], N, seed=t)
+            Zs.append(Z)
+            st, out = rb.step(Z, U1, U2)
+        lml_host.append(out["lml"])
+    x1, P1 = rb.get_tracks()
+    d_uni = DeviceArray(seq.uniforms.astype(np.uint32))
+    d_lml = DeviceArray.empty(T, np.float64)
+    if mode == "candidates":
+        d_cnt = DeviceArray(seq.count.astype(np.int32))
+        d_cand = DeviceArray(seq.cand.astype(np.float32))
+        rb2.run_candidates_dev(T, d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, flags)
+    else:
+        d_Z = DeviceArray(np.stack(Zs).astype(np.float32))
+        rb2.run_ref_dev(T, d_Z.ptr, d_uni.ptr, d_lml.ptr, flags)
+    rb2.synchronize()
+    x2, P2 = rb2.get_tracks()
+    assert np.array_equal(x1, x2) and np.array_equal(P1, P2)
+    assert np.array_equal(d_lml.get(), np.array(lml_host))
+    rb.close(); rb2.close()
