@@ -29,6 +29,8 @@ EXPORTS = [
     "pf_get_sample_prob", "pf_run_ref_dev", "pf_run_candidates_dev", "pf_synchronize",
     "pf_get_stream", "pf_last_run_kernel_time", "pf_kf_predict", "pf_kf_update",
     "pf_yml_read_matrix", "pf_read_camera_info", "pf_load_model_yml", "pf_philox_u32",
+    "pf2d_create", "pf2d_destroy", "pf2d_load_gmm", "pf2d_init", "pf2d_set_particles",
+    "pf2d_get_particles", "pf2d_update", "pf2d_estimate",
 ]
 
 
@@ -91,6 +93,14 @@ def lib():
     _sig(L, "pf_read_camera_info", C.c_int, C.c_char_p, C.POINTER(i32), C.POINTER(i32), _vp, _vp, _vp)
     _sig(L, "pf_load_model_yml", C.c_int, _vp, C.c_char_p, C.c_char_p, i32, _vp, _vp)
     _sig(L, "pf_philox_u32", None, u64, u32, u64, i64, _vp)
+    _sig(L, "pf2d_create", C.c_int, C.POINTER(_vp), i32, i32, i32, i32)
+    _sig(L, "pf2d_destroy", C.c_int, _vp)
+    _sig(L, "pf2d_load_gmm", C.c_int, _vp, i32, _vp, _vp, _vp)
+    _sig(L, "pf2d_init", C.c_int, _vp, _vp)
+    _sig(L, "pf2d_set_particles", C.c_int, _vp, _vp, _vp)
+    _sig(L, "pf2d_get_particles", C.c_int, _vp, _vp, _vp)
+    _sig(L, "pf2d_update", C.c_int, _vp, _vp, u32, _vp, _vp, _vp, _vp)
+    _sig(L, "pf2d_estimate", C.c_int, _vp, _vp)
     _lib = L
     return L
 

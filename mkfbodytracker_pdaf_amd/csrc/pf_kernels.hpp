@@ -658,7 +658,7 @@ struct AncArgs {
     int N, parity, kind_linear;
 };
 
-__global__ void __launch_bounds__(256) k_ancestors(AncArgs a)
+static __global__ void __launch_bounds__(256) k_ancestors(AncArgs a)
 {
     extern __shared__ unsigned long long wsp[];
     const int j0 = blockIdx.x * blockDim.x;
@@ -686,12 +686,12 @@ __global__ void __launch_bounds__(256) k_blockmax64(const double *w, int L, Fram
 }
 
 // ---------------------------------------------------------------- helpers
-__global__ void k_set_int(int *p, int v)
+static __global__ void k_set_int(int *p, int v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
-__global__ void k_iota(int *p, int n)
+static __global__ void k_iota(int *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = i;
@@ -830,7 +830,7 @@ __global__ void __launch_bounds__(256) k_estimate(const float *x, const int *anc
 }
 
 // getSamples (pf2DRao.cpp:85-103): out = y[0:2] + 0.8 scale n, y = Hp xbar + M.
-__global__ void k_samples(const double *hp2, const double *m2, const double *xbar, int d,
+static __global__ void k_samples(const double *hp2, const double *m2, const double *xbar, int d,
                           const double *normals, int n, double scale, double *out)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -846,7 +846,7 @@ __global__ void k_samples(const double *hp2, const double *m2, const double *xba
 }
 
 // getSampleProb (pf2DRao.cpp:105-122): N(in_c; y[0:2], 0.8 scale I) for two inputs.
-__global__ void k_sample_prob(const double *hp2, const double *m2, const double *xbar, int d,
+static __global__ void k_sample_prob(const double *hp2, const double *m2, const double *xbar, int d,
                               const double *in1, const double *in2, int n, double scale,
                               double *w1, double *w2)
 {
@@ -868,7 +868,7 @@ __global__ void k_sample_prob(const double *hp2, const double *m2, const double 
 // KF_model::predict / update for n states sharing one model (generic F, H):
 // one block per state, one thread per matrix element, FP64 (the class API of
 // KF_model.h:8-16 is double precision; this is not the filter's hot loop).
-__global__ void k_kf_predict(int d, const double *F, const double *B, const double *Q, double *x,
+static __global__ void k_kf_predict(int d, const double *F, const double *B, const double *Q, double *x,
                              double *P)
 {
     extern __shared__ double s[];
@@ -899,7 +899,7 @@ __global__ void k_kf_predict(int d, const double *F, const double *B, const doub
     }
 }
 
-__global__ void k_kf_update(int d, int m, const double *H, const double *BH, const double *R,
+static __global__ void k_kf_update(int d, int m, const double *H, const double *BH, const double *R,
                             const double *z, double *x, double *P, int *status)
 {
     extern __shared__ double s[];
