@@ -23,6 +23,22 @@ def sources():
     return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
+ROOT = os.path.dirname(HERE)
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_host_api")
+
+
+def build_cpp_test() -> str:
+    """Compile the C++ drop-in check (plain g++, linked against libmkfbt.so)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC,
+           "-L", HERE, "-lmkfbt", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../mkfbodytracker_pdaf_amd",
+           "-o", CPP_TEST_BIN]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + res.stderr[-4000:])
+    return CPP_TEST_BIN
+
+
 def build(verbose: bool = False) -> str:
     """Compile every source under csrc/ into libmkfbt.so (always rebuilds)."""
     tmp = LIB + ".tmp"
@@ -33,6 +49,8 @@ def build(verbose: bool = False) -> str:
     if verbose and res.stderr:
         print(res.stderr)
     os.replace(tmp, LIB)
+    if LIB.endswith("libmkfbt.so") and os.path.exists(CPP_TEST_SRC):
+        build_cpp_test()
     return LIB
 
 
