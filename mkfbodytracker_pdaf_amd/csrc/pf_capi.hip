@@ -1036,9 +1036,14 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
     // Frame f reads frame counter f & 1 and writes the other one.  Frame 0
     // reads the materialised ancestor array; later frames resolve their
     // ancestors from the previous frame's weights inside the step kernel.
+    // The fused ancestor resolution needs its LDS workspace next to the model
+    // image; if both do not fit one workgroup, ancestors are materialised by
+    // a separate k_ancestors launch each frame.
+    const int nbp = (h->N + kWeightsBlock - 1) / kWeightsBlock;
+    const bool fused = step_lds_bytes(h->table_floats, nbp) <= kMaxLdsBytes;
     auto one_frame = [&](int f) -> int {
         const int parity = f & 1;
-        const int amode = (f == 0) ? kAncArray : kAncFused;
+        const int amode = (f == 0 || !fused) ? kAncArray : kAncFused;
         StepArgs a = step_args(h, mode, uniforms, Z, zstride, h->N, count, cand, nullptr, parity, amode, lml);
         if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f], h->stream));
         int s = launch_step(h, a, mode);
@@ -1046,6 +1051,10 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
         if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f + 1], h->stream));
         s = launch_weights(h, uniforms, parity);
         if (s) return s;
+        if (!fused && f + 1 < T) {
+            s = launch_ancestors(h, h->N, h->N, h->anc, parity, 0, lml + f);
+            if (s) return s;
+        }
         swap_state(h);
         return PF_OK;
     };

@@ -30,6 +30,7 @@ namespace mkfbt {
 constexpr int kMaxJoints = 8;
 constexpr int kWeightsBlock = 256;     // input block of the resampler scans
 constexpr int kMaxScanBlocks = 4096;   // => at most 2^20 resampled weights
+constexpr int kMaxLdsBytes = 64 * 1024;  // per-workgroup LDS budget
 
 enum StepMode { kModeRef = 0, kModeCand = 1 };
 enum AncMode { kAncArray = 0, kAncFused = 1 };
@@ -119,9 +120,10 @@ __device__ __forceinline__ T wave_sum(T v)
 // The prefix sum is held as block-local scans (cloc) plus block sums (bsum);
 // resolve_ancestor() finds a_i for one block of outputs.
 
-// LDS workspace of resolve_ancestor: nb + 1 prefix words, 2 staged input
-// blocks, wave totals and a few broadcast slots.
-__host__ __device__ constexpr int resolve_lds_words(int nb) { return nb + 1 + 2 * kWeightsBlock + 8; }
+// LDS workspace of resolve_ancestor: nb + 1 prefix words, kStageBlocks
+// staged input blocks, wave totals and a few broadcast slots.
+constexpr int kStageBlocks = 12;
+__host__ __device__ constexpr int resolve_lds_words(int nb) { return nb + 1 + kStageBlocks * kWeightsBlock + 8; }
 
 // a_j for outputs j in [j0, j0 + blockDim) (j = j0 + threadIdx.x); every
 // thread of the block must call it.  w (u64 LDS words) is the workspace.
@@ -131,8 +133,8 @@ __device__ int resolve_ancestor(const Pending &pd, int N, uint32_t U, int j0, un
     const int tid = threadIdx.x, nt = blockDim.x, nb = pd.nb;
     const int lane = tid & 63, wid = tid >> 6, nwaves = (nt + 63) >> 6;
     unsigned long long *pre = w;                      // nb + 1
-    unsigned long long *stg = w + nb + 1;             // 2 * kWeightsBlock
-    unsigned long long *misc = stg + 2 * kWeightsBlock;  // 8
+    unsigned long long *stg = w + nb + 1;             // kStageBlocks * kWeightsBlock
+    unsigned long long *misc = stg + kStageBlocks * kWeightsBlock;  // 8
     // 1. exclusive prefix of the block sums (chunked scan across the block)
     const int per = (nb + nt - 1) / nt;
     const int c0 = min(nb, tid * per), c1 = min(nb, c0 + per);
@@ -185,9 +187,12 @@ __device__ int resolve_ancestor(const Pending &pd, int N, uint32_t U, int j0, un
     if (j == jlast) misc[5] = (unsigned long long)bj;
     __syncthreads();
     const int blo = (int)misc[4], bhi = (int)misc[5];
+    // 3. a span of at most kStageBlocks input blocks is staged in LDS
+    //    (C = pre + cloc, one memory latency) and searched there; wider spans
+    //    (strongly concentrated weights) search the input block in global
+    //    memory, one dependent probe per level.
     int a;
-    if (bhi - blo <= 1) {
-        // 3a. stage C = pre + cloc of the (at most two) input blocks in LDS
+    if (bhi - blo < kStageBlocks) {
         const int base = blo * kWeightsBlock;
         const int n = min(pd.L, (bhi + 1) * kWeightsBlock) - base;
         for (int k = tid; k < n; k += nt) stg[k] = pre[blo + k / kWeightsBlock] + pd.cloc[base + k];
@@ -200,7 +205,6 @@ __device__ int resolve_ancestor(const Pending &pd, int N, uint32_t U, int j0, un
         }
         a = base + l;
     } else {
-        // 3b. sparse weights: search the input block in global memory
         const unsigned long long pb = pre[bj];
         int l = bj * kWeightsBlock, h = min(pd.L, (bj + 1) * kWeightsBlock) - 1;
         while (l < h) {
