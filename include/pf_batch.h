@@ -1,0 +1,76 @@
+/*
+ * pf_batch.h -- many independent Rao-Blackwellised particle filters on one
+ * gfx950 GPU, advanced together.
+ *
+ * No counterpart in the reference: it runs one ParticleFilter per arm,
+ * sequentially (src/pfPose.cpp:325-326).  Every sequence here runs the same
+ * algorithm as pf_step_candidates() of pf_rbpf.h (pf2DRao.cpp:125-158 with
+ * the PDAF joint likelihood), with its own GMM, particle count and length.
+ * One persistent step kernel and one weights kernel per frame serve all
+ * sequences still active at that frame; the frame loop is captured into a
+ * hipGraph.  Tile size and persistence are picked by pf_batch_autotune().
+ */
+#ifndef PF_BATCH_H
+#define PF_BATCH_H
+
+#include <stdint.h>
+
+#include "pf_rbpf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pf_batch pf_batch;
+
+typedef struct pf_batch_schedule {
+    int32_t block;          /* threads per tile: 64, 128 or 256 */
+    int32_t ppl;            /* particles per lane per tile: 1 or 2 (tile = block * ppl) */
+    int32_t blocks_per_cu;  /* persistent grid = blocks_per_cu * CUs */
+} pf_batch_schedule;
+
+/* S sequences; sequence s has n_particles[s] particles and n_frames[s]
+ * frames.  All share K components, state dimension d (12) and m measurement
+ * rows.  cfg supplies the PDAF constants and cand_capacity (block_size and
+ * n_particles are ignored). */
+int pf_batch_create(pf_batch **out, int32_t S, const int32_t *n_particles, const int32_t *n_frames,
+                    int32_t K, int32_t d, int32_t m, const pf_config *cfg);
+int pf_batch_destroy(pf_batch *b);
+
+/* my_gmm::loadGaussian x K for sequence s (arguments as pf_load_model). */
+int pf_batch_load_model(pf_batch *b, int32_t s, int32_t npose, const int32_t *sel, const double *means,
+                        const double *covs, const double *weights, const double *gamma,
+                        const double *pca_proj, const double *pca_mean, const double *R);
+/* PDAF joint kinds shared by all sequences (before pf_batch_reset). */
+int pf_batch_set_joint_clutter(pf_batch *b, const int32_t *clutter);
+/* my_gmm::resetTracker(bins) for sequence s. */
+int pf_batch_reset(pf_batch *b, int32_t s, const int32_t *bins);
+
+/* Runs every sequence to its last frame.  Device pointers, sequences
+ * concatenated in index order with frame offsets F_s = sum_{r<s} n_frames[r]:
+ * uniforms (F x 2), count (F x J), cand (F x J x cap x 2 floats), lml (F,
+ * written: log mean weight of every sequence-frame).  flags bit 0: capture
+ * the frame loop into a hipGraph. */
+int pf_batch_run_dev(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uniforms,
+                     double *lml, int32_t flags);
+/* Tracks and estimate of sequence s after a run (as pf_get_tracks / pf_estimate). */
+int pf_batch_get_tracks(pf_batch *b, int32_t s, double *x, double *P);
+int pf_batch_estimate(pf_batch *b, int32_t s, double *xbar);
+
+/* Schedule in use / set explicitly. */
+int pf_batch_get_schedule(pf_batch *b, pf_batch_schedule *sch);
+int pf_batch_set_schedule(pf_batch *b, const pf_batch_schedule *sch);
+/* Times `frames` frames (from frame 0 of the inputs) for every candidate
+ * schedule, keeps the fastest and reports its time and that of the default
+ * schedule (256 threads, 1 particle per lane, 2 blocks per CU).  The tracks
+ * are consumed: reset the sequences before the next run. */
+int pf_batch_autotune(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uniforms,
+                      double *lml, int32_t frames, pf_batch_schedule *best, double *best_ms,
+                      double *default_ms);
+/* Device time of the last pf_batch_run_dev in ms. */
+int pf_batch_last_run_ms(pf_batch *b, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

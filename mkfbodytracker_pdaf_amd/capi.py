@@ -21,7 +21,7 @@ PF_ERR_STATE = -4
 PF_ERR_UNSUPPORTED = -5
 PF_ERR_IO = -6
 
-# Every symbol declared in include/pf_rbpf.h and include/pf2d.h.
+# Every symbol declared in include/pf_rbpf.h, include/pf2d.h and include/pf_batch.h.
 EXPORTS = [
     "pf_config_default", "pf_last_error", "pf_create", "pf_destroy", "pf_load_model",
     "pf_set_joint_clutter", "pf_reset", "pf_set_tracks", "pf_get_tracks", "pf_step",
@@ -31,7 +31,17 @@ EXPORTS = [
     "pf_yml_read_matrix", "pf_read_camera_info", "pf_load_model_yml", "pf_philox_u32",
     "pf2d_create", "pf2d_destroy", "pf2d_load_gmm", "pf2d_init", "pf2d_set_particles",
     "pf2d_get_particles", "pf2d_update", "pf2d_estimate",
+    "pf_batch_create", "pf_batch_destroy", "pf_batch_load_model", "pf_batch_set_joint_clutter",
+    "pf_batch_reset", "pf_batch_run_dev", "pf_batch_get_tracks", "pf_batch_estimate",
+    "pf_batch_get_schedule", "pf_batch_set_schedule", "pf_batch_autotune", "pf_batch_last_run_ms",
 ]
+
+
+class Schedule(C.Structure):
+    _fields_ = [("block", C.c_int32), ("ppl", C.c_int32), ("blocks_per_cu", C.c_int32)]
+
+    def as_tuple(self):
+        return (self.block, self.ppl, self.blocks_per_cu)
 
 
 class PfError(RuntimeError):
@@ -101,6 +111,19 @@ def lib():
     _sig(L, "pf2d_get_particles", C.c_int, _vp, _vp, _vp)
     _sig(L, "pf2d_update", C.c_int, _vp, _vp, u32, _vp, _vp, _vp, _vp)
     _sig(L, "pf2d_estimate", C.c_int, _vp, _vp)
+    _sig(L, "pf_batch_create", C.c_int, C.POINTER(_vp), i32, _vp, _vp, i32, i32, i32, C.POINTER(Config))
+    _sig(L, "pf_batch_destroy", C.c_int, _vp)
+    _sig(L, "pf_batch_load_model", C.c_int, _vp, i32, i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+    _sig(L, "pf_batch_set_joint_clutter", C.c_int, _vp, _vp)
+    _sig(L, "pf_batch_reset", C.c_int, _vp, i32, _vp)
+    _sig(L, "pf_batch_run_dev", C.c_int, _vp, _vp, _vp, _vp, _vp, i32)
+    _sig(L, "pf_batch_get_tracks", C.c_int, _vp, i32, _vp, _vp)
+    _sig(L, "pf_batch_estimate", C.c_int, _vp, i32, _vp)
+    _sig(L, "pf_batch_get_schedule", C.c_int, _vp, C.POINTER(Schedule))
+    _sig(L, "pf_batch_set_schedule", C.c_int, _vp, C.POINTER(Schedule))
+    _sig(L, "pf_batch_autotune", C.c_int, _vp, _vp, _vp, _vp, _vp, i32, C.POINTER(Schedule), C.POINTER(dbl),
+         C.POINTER(dbl))
+    _sig(L, "pf_batch_last_run_ms", C.c_int, _vp, C.POINTER(dbl))
     _lib = L
     return L
 

@@ -1,0 +1,856 @@
+// pf_batch.hip -- many independent RBPF sequences advanced together on one
+// gfx950 GPU (C ABI in include/pf_batch.h).
+//
+// Layout: all sequences share one structure-of-arrays state (x[D][ld],
+// P[D(D+1)/2][ld], double buffered), sequence s owning particles
+// [pofs_s, pofs_s + N_s) with pofs_s a multiple of 512.  Sequences are
+// ordered by length (longest first), so at frame t the active sequences are
+// a prefix and so are their tiles and resampler blocks; the persistent
+// kernels read the active counts for frame t from device memory, which makes
+// one captured frame pair replayable for the whole run.
+//
+// Per frame: k_batch_step (ancestors from the previous frame's weights, the
+// same per-particle update as k_step, per-sequence block maxima) and
+// k_batch_weights (per-sequence fixed-point weights and block scans).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../include/pf_batch.h"
+#include "pf_internal.hpp"
+#include "pf_kernels.hpp"
+#include "pf_model.hpp"
+
+using namespace mkfbt;
+
+#define HIP_TRYB(expr)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_error(PF_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+    } while (0)
+
+namespace {
+
+constexpr int kBD = 12;           // state dimension of the batched engine
+constexpr int kSeqAlign = 512;    // particle offset alignment (largest tile)
+
+struct SeqDesc {
+    int pofs, N, T, tofs, bofs, nb, table_ofs, pad;
+    unsigned long long pi_total;
+};
+
+struct BatchArgs {
+    float *buf_x[2], *buf_P[2];
+    int *anc;
+    float *lw;
+    unsigned long long *maxkey;   // 2 x S (parity-major)
+    unsigned int *status;         // S
+    uint32_t *Upend;              // S
+    unsigned long long *cloc, *bsum;
+    double *bexp;
+    const float *tables;
+    const SeqDesc *seqs;
+    const int *tile_seq, *tile_j0, *n_tiles;   // tiles of active sequences first
+    const int *wb_seq, *wb_b, *n_wblocks;      // resampler blocks likewise
+    const int *frame;
+    int *frame_next;
+    const uint32_t *uni;
+    const int *count;
+    const float *cand;
+    double *lml;
+    int S, ld, K, m, J, cap, table_floats, parity, fused, nb_max, all_tiles;
+    unsigned int order_packed, clutter_mask;
+    float pdl, b0, gate;
+};
+
+// Per-sequence view of the arguments in the single-filter layout, so the
+// per-particle update is shared with k_step.
+__device__ __forceinline__ StepArgs seq_view(const BatchArgs &a, const SeqDesc &sd)
+{
+    StepArgs v{};
+    v.count = a.count + (size_t)sd.tofs * a.J;
+    v.cand = a.cand + (size_t)sd.tofs * a.J * a.cap * 2;
+    v.pi_total = sd.pi_total;
+    v.K = a.K;
+    v.N = sd.N;
+    v.m = a.m;
+    v.J = a.J;
+    v.cap = a.cap;
+    v.order_packed = a.order_packed;
+    v.clutter_mask = a.clutter_mask;
+    v.pdl = a.pdl;
+    v.b0 = a.b0;
+    v.gate = a.gate;
+    v.comp_out = nullptr;
+    return v;
+}
+
+template <int PPL>
+__device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc &sd, int s, int t, int jt0,
+                                               unsigned long long *wsp, int (&src)[PPL], bool write_anc,
+                                               int stats_parity, double *lml_slot)
+{
+    const int bd = blockDim.x;
+    Pending pd;
+    pd.cloc = a.cloc + sd.pofs;
+    pd.bsum = a.bsum + sd.bofs;
+    pd.bexp = a.bexp + sd.bofs;
+    pd.L = sd.N;
+    pd.nb = sd.nb;
+    double es = 0.0;
+    const bool first = (jt0 == 0);
+    const unsigned long long total = resolve_prefix(pd, wsp, first, es);
+    const uint32_t U = a.Upend[s];
+#pragma unroll
+    for (int r = 0; r < PPL; ++r) {
+        src[r] = 0;
+        if (jt0 + r * bd >= sd.N) continue;        // block-uniform: no barrier is skipped by part of the block
+        src[r] = resolve_search(pd, sd.N, U, total, jt0 + r * bd, wsp);
+        const int j = jt0 + r * bd + (int)threadIdx.x;
+        if (write_anc && j < sd.N) a.anc[sd.pofs + j] = src[r];
+    }
+    if (first && threadIdx.x == 0) {
+        unsigned long long *mk = a.maxkey + (size_t)stats_parity * a.S + s;
+        const unsigned long long key = *mk;
+        if (key == 0ull) a.status[s] |= 1u;
+        if (lml_slot) *lml_slot = (key == 0ull) ? -INFINITY : key_value(key) + log(es) - log((double)sd.N);
+        *mk = 0ull;
+    }
+}
+
+template <int SELJ, int PPL>
+__global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
+{
+    constexpr int D = kBD, NP = npacked(D);
+    extern __shared__ float4 sm4[];
+    const int t = *a.frame;
+    const int ntiles = a.n_tiles[t];
+    const int bd = blockDim.x;
+    const bool odd = (t & 1) != 0;      // selects, not indexes: a dynamic index would copy the array to scratch
+    const float *xin = odd ? a.buf_x[1] : a.buf_x[0], *Pin = odd ? a.buf_P[1] : a.buf_P[0];
+    float *xout = odd ? a.buf_x[0] : a.buf_x[1], *Pout = odd ? a.buf_P[0] : a.buf_P[1];
+    const unsigned rowb = (unsigned)a.ld * 4u;
+    unsigned long long *wsp = reinterpret_cast<unsigned long long *>(reinterpret_cast<float *>(sm4) + a.table_floats);
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int s = a.tile_seq[tile];
+        const int jt0 = a.tile_j0[tile];
+        const SeqDesc sd = a.seqs[s];
+        int src[PPL];
+        if (a.fused) {
+            tile_ancestors<PPL>(a, sd, s, t, jt0, wsp, src, false, a.parity ^ 1, a.lml + sd.tofs + (t - 1));
+        } else {
+#pragma unroll
+            for (int r = 0; r < PPL; ++r) {
+                const int j = jt0 + r * bd + (int)threadIdx.x;
+                src[r] = j < sd.N ? a.anc[sd.pofs + j] : 0;
+            }
+        }
+        // state loads of all PPL particles, then the model staging
+        float x[PPL][D], P[PPL][NP];
+        const __amdgpu_buffer_rsrc_t rx = soa_rsrc(xin, rowb * D), rP = soa_rsrc(Pin, rowb * NP);
+#pragma unroll
+        for (int r = 0; r < PPL; ++r) {
+            const int j = jt0 + r * bd + (int)threadIdx.x;
+            if (j < sd.N) {
+                const unsigned voff = (unsigned)(sd.pofs + src[r]) * 4u;
+#pragma unroll
+                for (int i = 0; i < D; ++i) x[r][i] = soa_ld(rx, voff, rowb * i);
+#pragma unroll
+                for (int p = 0; p < NP; ++p) P[r][p] = soa_ld(rP, voff, rowb * p);
+            }
+        }
+        __syncthreads();          // the previous tile's LDS readers are done
+        stage_table(sm4, reinterpret_cast<const float4 *>(a.tables + (size_t)sd.table_ofs), a.table_floats >> 2);
+        __syncthreads();
+        const StepArgs v = seq_view(a, sd);
+        const uint32_t U1 = a.uni[(size_t)(sd.tofs + t) * 2];
+        float lmax = -INFINITY;
+        const __amdgpu_buffer_rsrc_t wx = soa_rsrc(xout, rowb * D), wP = soa_rsrc(Pout, rowb * NP);
+#pragma unroll
+        for (int r = 0; r < PPL; ++r) {
+            const int j = jt0 + r * bd + (int)threadIdx.x;
+            if (j < sd.N) {
+                const float lwj = particle_update<D, kModeCand, SELJ>(v, t, j, U1,
+                                                                      reinterpret_cast<const float *>(sm4), x[r], P[r]);
+                const unsigned doff = (unsigned)(sd.pofs + j) * 4u;
+#pragma unroll
+                for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[r][i]);
+#pragma unroll
+                for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[r][p]);
+                a.lw[sd.pofs + j] = lwj;
+                if (isfinite(lwj)) lmax = fmaxf(lmax, lwj);
+            }
+        }
+        double mv = (double)lmax;
+        mv = wave_max(mv);
+        if ((threadIdx.x & 63) == 0 && isfinite(mv))
+            atomicMax(a.maxkey + (size_t)a.parity * a.S + s, order_key(mv));
+    }
+}
+
+// Fixed-point weights and block scans, per sequence (kWeightsBlock blocks).
+__global__ void __launch_bounds__(kWeightsBlock) k_batch_weights(BatchArgs a)
+{
+    __shared__ unsigned long long s_wave[4];
+    __shared__ double s_dwave[4];
+    const int t = *a.frame;
+    const int nwb = a.n_wblocks[t];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.frame_next = t + 1;
+    for (int wb = blockIdx.x; wb < nwb; wb += gridDim.x) {
+        const int s = a.wb_seq[wb], b = a.wb_b[wb];
+        const SeqDesc sd = a.seqs[s];
+        const unsigned long long key = a.maxkey[(size_t)a.parity * a.S + s];
+        const bool degenerate = (key == 0ull);
+        const double ref = degenerate ? 0.0 : key_value(key);
+        const int i = b * kWeightsBlock + (int)threadIdx.x;
+        unsigned long long q = 0;
+        double v = 0.0;
+        if (i < sd.N) {
+            if (degenerate) {
+                q = 1;
+            } else {
+                const double x = (double)a.lw[sd.pofs + i];
+                v = isfinite(x) ? det_exp(x - ref) : 0.0;
+                q = qlog(x, ref);
+            }
+        }
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        unsigned long long sc = q;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned long long n = __shfl_up(sc, o, 64);
+            if (lane >= o) sc += n;
+        }
+        const double vs = wave_sum(v);
+        __syncthreads();
+        if (lane == 63) s_wave[wid] = sc;
+        if (lane == 0) s_dwave[wid] = vs;
+        __syncthreads();
+        unsigned long long off = 0;
+        for (int w = 0; w < wid; ++w) off += s_wave[w];
+        sc += off;
+        if (i < sd.N) a.cloc[sd.pofs + i] = sc;
+        if (threadIdx.x == kWeightsBlock - 1) {
+            a.bsum[sd.bofs + b] = sc;
+            a.bexp[sd.bofs + b] = (s_dwave[0] + s_dwave[1]) + (s_dwave[2] + s_dwave[3]);
+        }
+        if (b == 0 && threadIdx.x == 0) a.Upend[s] = a.uni[(size_t)(sd.tofs + t) * 2 + 1];
+    }
+}
+
+// Materialises every sequence's last resampling: ancestors and the log mean
+// weight of its last frame.
+template <int PPL>
+__global__ void __launch_bounds__(256) k_batch_finish(BatchArgs a)
+{
+    extern __shared__ float4 sm4[];
+    unsigned long long *wsp = reinterpret_cast<unsigned long long *>(sm4);
+    for (int tile = blockIdx.x; tile < a.all_tiles; tile += gridDim.x) {
+        const int s = a.tile_seq[tile];
+        const int jt0 = a.tile_j0[tile];
+        const SeqDesc sd = a.seqs[s];
+        int src[PPL];
+        tile_ancestors<PPL>(a, sd, s, sd.T - 1, jt0, wsp, src, true, (sd.T - 1) & 1, a.lml + sd.tofs + sd.T - 1);
+    }
+}
+
+// Start of a run: per-sequence maxima and flags cleared (a kernel, so the
+// clear may sit in a captured stream).
+__global__ void k_batch_clear(unsigned long long *maxkey, unsigned int *status, int S)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * S) maxkey[i] = 0ull;
+    if (i < S) status[i] = 0u;
+}
+
+__global__ void k_identity_seq(int *anc, int pofs, int N)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < N) anc[pofs + j] = j;
+}
+
+template <typename T>
+hipError_t dallocb(T **p, size_t n)
+{
+    return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1));
+}
+
+}  // namespace
+
+struct pf_batch {
+    int device = 0, S = 0, K = 0, D = 0, m = 0, J = 0, cap = 16, cus = 256;
+    double pd = 0.95, gate = 9.210340371976184, lambda = 3.0 / (640.0 * 480.0);
+    std::vector<int> N, T, order, rank;      // order[i] = user sequence of internal slot i; rank = inverse
+    std::vector<int> pofs, tofs_user, tofs, bofs, nb;
+    int ld = 0, Tmax = 0, nb_max = 0;
+    std::vector<HostModel> hm;
+    std::vector<char> loaded, sel_ok, fresh;   // fresh[s]: reset since the last run
+    unsigned clutter_mask = 0, order_packed = 0;
+    int table_floats = 0;
+    bool tracks = false;
+    pf_batch_schedule sch{256, 1, 2};
+    hipStream_t stream = nullptr;
+    float *bx[2] = {nullptr, nullptr}, *bP[2] = {nullptr, nullptr};
+    float *lw = nullptr, *tables = nullptr, *mus = nullptr, *sigs = nullptr;
+    double *Tis = nullptr, *Ts = nullptr, *bexp = nullptr, *partial = nullptr, *est = nullptr, *gbuf = nullptr;
+    size_t gbuf_bytes = 0;
+    int *anc = nullptr, *frame = nullptr, *bins = nullptr;
+    unsigned long long *maxkey = nullptr, *cloc = nullptr, *bsum = nullptr;
+    unsigned int *status = nullptr;
+    uint32_t *Upend = nullptr;
+    SeqDesc *seqs = nullptr;
+    std::vector<SeqDesc> h_seqs;
+    int *tile_seq = nullptr, *tile_j0 = nullptr, *n_tiles = nullptr, all_tiles = 0;
+    int *wb_seq = nullptr, *wb_b = nullptr, *n_wblocks = nullptr;
+    FrameScratch *fs = nullptr;   // for the per-sequence estimate reduction
+    float last_ms = 0.f;
+};
+
+namespace {
+
+int bcheck(pf_batch *b)
+{
+    if (!b) return set_error(PF_ERR_ARG, "null batch");
+    HIP_TRYB(hipSetDevice(b->device));
+    return PF_OK;
+}
+
+// Tile tables for the current schedule (tile = block * ppl particles).
+int build_tiles(pf_batch *b)
+{
+    const int tile = b->sch.block * b->sch.ppl;
+    std::vector<int> ts, tj, wbs, wbb;
+    std::vector<int> first_tile(b->S + 1), first_wb(b->S + 1);
+    for (int i = 0; i < b->S; ++i) {
+        first_tile[i] = (int)ts.size();
+        first_wb[i] = (int)wbs.size();
+        const int n = b->N[b->order[i]];
+        for (int j0 = 0; j0 < n; j0 += tile) { ts.push_back(i); tj.push_back(j0); }
+        for (int k = 0; k * kWeightsBlock < n; ++k) { wbs.push_back(i); wbb.push_back(k); }
+    }
+    first_tile[b->S] = (int)ts.size();
+    first_wb[b->S] = (int)wbs.size();
+    std::vector<int> nt((size_t)b->Tmax + 1), nw((size_t)b->Tmax + 1);
+    for (int t = 0; t <= b->Tmax; ++t) {
+        int act = 0;                       // sequences are sorted by length, longest first
+        while (act < b->S && b->T[b->order[act]] > t) ++act;
+        nt[t] = first_tile[act];
+        nw[t] = first_wb[act];
+    }
+    b->all_tiles = (int)ts.size();
+    (void)hipFree(b->tile_seq); (void)hipFree(b->tile_j0); (void)hipFree(b->n_tiles);
+    (void)hipFree(b->wb_seq); (void)hipFree(b->wb_b); (void)hipFree(b->n_wblocks);
+    HIP_TRYB(dallocb(&b->tile_seq, ts.size()));
+    HIP_TRYB(dallocb(&b->tile_j0, tj.size()));
+    HIP_TRYB(dallocb(&b->n_tiles, nt.size()));
+    HIP_TRYB(dallocb(&b->wb_seq, wbs.size()));
+    HIP_TRYB(dallocb(&b->wb_b, wbb.size()));
+    HIP_TRYB(dallocb(&b->n_wblocks, nw.size()));
+    HIP_TRYB(hipMemcpy(b->tile_seq, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice));
+    HIP_TRYB(hipMemcpy(b->tile_j0, tj.data(), sizeof(int) * tj.size(), hipMemcpyHostToDevice));
+    HIP_TRYB(hipMemcpy(b->n_tiles, nt.data(), sizeof(int) * nt.size(), hipMemcpyHostToDevice));
+    HIP_TRYB(hipMemcpy(b->wb_seq, wbs.data(), sizeof(int) * wbs.size(), hipMemcpyHostToDevice));
+    HIP_TRYB(hipMemcpy(b->wb_b, wbb.data(), sizeof(int) * wbb.size(), hipMemcpyHostToDevice));
+    HIP_TRYB(hipMemcpy(b->n_wblocks, nw.data(), sizeof(int) * nw.size(), hipMemcpyHostToDevice));
+    return PF_OK;
+}
+
+// Model images of every sequence (same processing order for all).
+int build_batch_tables(pf_batch *b)
+{
+    for (int i = 0; i < b->S; ++i)
+        if (!b->loaded[b->order[i]]) return PF_OK;    // uploaded once every model is present
+    std::vector<ModelImage> imgs((size_t)b->S);
+    size_t tf = 0;
+    for (int i = 0; i < b->S; ++i) {
+        const int s = b->order[i];
+        int st = make_model_image(b->hm[s], b->order_packed, b->N[s], imgs[i]);
+        if (st) return st;
+        tf = std::max(tf, imgs[i].blob.size());
+        b->sel_ok[s] = imgs[i].sel_mode;
+    }
+    const int d = b->D, NP = npacked(d);
+    b->table_floats = (int)tf;
+    std::vector<float> tabs(tf * b->S, 0.f), mus((size_t)b->S * b->K * d), sgs((size_t)b->S * b->K * NP);
+    std::vector<double> Ts((size_t)b->S * d * d), Tis((size_t)b->S * d * d);
+    for (int i = 0; i < b->S; ++i) {
+        std::copy(imgs[i].blob.begin(), imgs[i].blob.end(), tabs.begin() + (size_t)i * tf);
+        std::copy(imgs[i].mu.begin(), imgs[i].mu.end(), mus.begin() + (size_t)i * b->K * d);
+        std::copy(imgs[i].sg.begin(), imgs[i].sg.end(), sgs.begin() + (size_t)i * b->K * NP);
+        std::copy(imgs[i].T.begin(), imgs[i].T.end(), Ts.begin() + (size_t)i * d * d);
+        std::copy(imgs[i].Ti.begin(), imgs[i].Ti.end(), Tis.begin() + (size_t)i * d * d);
+        b->h_seqs[i].table_ofs = (int)(i * tf);
+        b->h_seqs[i].pi_total = imgs[i].pi_total;
+    }
+    (void)hipFree(b->tables); (void)hipFree(b->mus); (void)hipFree(b->sigs); (void)hipFree(b->Ts); (void)hipFree(b->Tis);
+    HIP_TRYB(dallocb(&b->tables, tabs.size()));
+    HIP_TRYB(dallocb(&b->mus, mus.size()));
+    HIP_TRYB(dallocb(&b->sigs, sgs.size()));
+    HIP_TRYB(dallocb(&b->Ts, Ts.size()));
+    HIP_TRYB(dallocb(&b->Tis, Tis.size()));
+    HIP_TRYB(hipMemcpy(b->tables, tabs.data(), sizeof(float) * tabs.size(), hipMemcpyHostToDevice));
+    HIP_TRYB(hipMemcpy(b->mus, mus.data(), sizeof(float) * mus.size(), hipMemcpyHostToDevice));
+    HIP_TRYB(hipMemcpy(b->sigs, sgs.data(), sizeof(float) * sgs.size(), hipMemcpyHostToDevice));
+    HIP_TRYB(hipMemcpy(b->Ts, Ts.data(), sizeof(double) * Ts.size(), hipMemcpyHostToDevice));
+    HIP_TRYB(hipMemcpy(b->Tis, Tis.data(), sizeof(double) * Tis.size(), hipMemcpyHostToDevice));
+    HIP_TRYB(hipMemcpy(b->seqs, b->h_seqs.data(), sizeof(SeqDesc) * b->S, hipMemcpyHostToDevice));
+    return PF_OK;
+}
+
+BatchArgs batch_args(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uni, double *lml,
+                     int parity, int fused)
+{
+    BatchArgs a{};
+    a.buf_x[0] = b->bx[0]; a.buf_x[1] = b->bx[1];
+    a.buf_P[0] = b->bP[0]; a.buf_P[1] = b->bP[1];
+    a.anc = b->anc;
+    a.lw = b->lw;
+    a.maxkey = b->maxkey;
+    a.status = b->status;
+    a.Upend = b->Upend;
+    a.cloc = b->cloc;
+    a.bsum = b->bsum;
+    a.bexp = b->bexp;
+    a.tables = b->tables;
+    a.seqs = b->seqs;
+    a.tile_seq = b->tile_seq;
+    a.tile_j0 = b->tile_j0;
+    a.n_tiles = b->n_tiles;
+    a.wb_seq = b->wb_seq;
+    a.wb_b = b->wb_b;
+    a.n_wblocks = b->n_wblocks;
+    a.frame = b->frame + parity;
+    a.frame_next = b->frame + (1 - parity);
+    a.uni = uni;
+    a.count = count;
+    a.cand = cand;
+    a.lml = lml;
+    a.S = b->S;
+    a.ld = b->ld;
+    a.K = b->K;
+    a.m = b->m;
+    a.J = b->J;
+    a.cap = b->cap;
+    a.table_floats = b->table_floats;
+    a.parity = parity;
+    a.fused = fused;
+    a.nb_max = b->nb_max;
+    a.all_tiles = b->all_tiles;
+    a.order_packed = b->order_packed;
+    a.clutter_mask = b->clutter_mask;
+    const double pg = 1.0 - std::exp(-0.5 * b->gate);
+    a.pdl = (float)(b->pd / b->lambda);
+    a.b0 = (float)(1.0 - b->pd * pg);
+    a.gate = (float)b->gate;
+    return a;
+}
+
+template <int SELJ, int PPL>
+void launch_step_t(pf_batch *b, const BatchArgs &a, size_t lds)
+{
+    const int grid = b->sch.blocks_per_cu * b->cus;
+    hipLaunchKernelGGL((k_batch_step<SELJ, PPL>), dim3(grid), dim3(b->sch.block), lds, b->stream, a);
+}
+
+int launch_batch_step(pf_batch *b, const BatchArgs &a)
+{
+    const size_t lds = (size_t)step_lds_bytes(b->table_floats, b->nb_max);
+    bool all_sel = true;
+    for (int s = 0; s < b->S; ++s) all_sel = all_sel && b->sel_ok[s];
+    const int selj = all_sel ? b->J : 0;
+    const int ppl = b->sch.ppl;
+#define PF_BATCH_CASE(J_)                                           \
+    case J_:                                                        \
+        if (ppl == 2) launch_step_t<J_, 2>(b, a, lds);             \
+        else launch_step_t<J_, 1>(b, a, lds);                      \
+        break;
+    switch (selj) {
+        PF_BATCH_CASE(0)
+        PF_BATCH_CASE(1)
+        PF_BATCH_CASE(2)
+        PF_BATCH_CASE(3)
+        PF_BATCH_CASE(4)
+        PF_BATCH_CASE(5)
+        PF_BATCH_CASE(6)
+    default: return set_error(PF_ERR_UNSUPPORTED, "batched engine: %d joints", selj);
+    }
+#undef PF_BATCH_CASE
+    HIP_TRYB(hipGetLastError());
+    return PF_OK;
+}
+
+int launch_batch_weights(pf_batch *b, const BatchArgs &a)
+{
+    hipLaunchKernelGGL(k_batch_weights, dim3(4 * b->cus), dim3(kWeightsBlock), 0, b->stream, a);
+    HIP_TRYB(hipGetLastError());
+    return PF_OK;
+}
+
+int launch_batch_finish(pf_batch *b, const BatchArgs &a)
+{
+    const size_t lds = 8 * (size_t)resolve_lds_words(b->nb_max);
+    if (b->sch.ppl == 2)
+        hipLaunchKernelGGL((k_batch_finish<2>), dim3(2 * b->cus), dim3(b->sch.block), lds, b->stream, a);
+    else
+        hipLaunchKernelGGL((k_batch_finish<1>), dim3(2 * b->cus), dim3(b->sch.block), lds, b->stream, a);
+    HIP_TRYB(hipGetLastError());
+    return PF_OK;
+}
+
+int run_all(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uni, double *lml, int frames,
+            bool graph)
+{
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, b->stream, b->frame, 0);
+    hipLaunchKernelGGL(k_batch_clear, dim3((2 * b->S + 255) / 256), dim3(256), 0, b->stream, b->maxkey, b->status,
+                       b->S);
+    auto one = [&](int f) -> int {
+        const int parity = f & 1;
+        BatchArgs a = batch_args(b, count, cand, uni, lml, parity, f > 0);
+        int st = launch_batch_step(b, a);
+        if (st) return st;
+        return launch_batch_weights(b, a);
+    };
+    int st = one(0);
+    if (st) return st;
+    const int pairs = (frames - 1) / 2;
+    if (graph && pairs > 0) {
+        hipGraph_t g;
+        hipGraphExec_t ge;
+        HIP_TRYB(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int s1 = one(1);
+        int s2 = s1 ? s1 : one(2);
+        hipError_t ec = hipStreamEndCapture(b->stream, &g);
+        if (s2) return s2;
+        if (ec != hipSuccess) return set_error(PF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        HIP_TRYB(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        for (int p = 0; p < pairs; ++p) HIP_TRYB(hipGraphLaunch(ge, b->stream));
+        HIP_TRYB(hipStreamSynchronize(b->stream));
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        if ((frames - 1) & 1) {
+            st = one(frames - 1);
+            if (st) return st;
+        }
+    } else {
+        for (int f = 1; f < frames; ++f) {
+            st = one(f);
+            if (st) return st;
+        }
+    }
+    return PF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf_batch_destroy(pf_batch *b)
+{
+    if (!b) return PF_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    void *ptrs[] = {b->bx[0], b->bx[1], b->bP[0], b->bP[1], b->lw, b->tables, b->mus, b->sigs, b->Tis, b->Ts,
+                    b->bexp, b->partial, b->est, b->gbuf, b->anc, b->frame, b->bins, b->maxkey, b->cloc, b->bsum,
+                    b->status, b->Upend, b->seqs, b->tile_seq, b->tile_j0, b->n_tiles, b->wb_seq, b->wb_b,
+                    b->n_wblocks, b->fs};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return PF_OK;
+}
+
+int pf_batch_create(pf_batch **out, int32_t S, const int32_t *n_particles, const int32_t *n_frames, int32_t K,
+                    int32_t d, int32_t m, const pf_config *cfg)
+{
+    if (!out || !n_particles || !n_frames) return set_error(PF_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (S <= 0 || S > (1 << 16)) return set_error(PF_ERR_ARG, "S=%d out of range", S);
+    if (d != kBD) return set_error(PF_ERR_UNSUPPORTED, "the batched engine runs d = %d", kBD);
+    if (K <= 0 || K > 256 || m <= 0 || (m & 1) || m > 12) return set_error(PF_ERR_ARG, "bad K/m");
+    pf_config c;
+    pf_config_default(&c);
+    if (cfg) c = *cfg;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_error(PF_ERR_HIP, "no HIP device available");
+    if (c.device < 0 || c.device >= ndev) return set_error(PF_ERR_ARG, "device out of range");
+    HIP_TRYB(hipSetDevice(c.device));
+    pf_batch *b = new pf_batch();
+    b->device = c.device;
+    b->S = S; b->K = K; b->D = d; b->m = m; b->J = m / 2;
+    b->cap = c.cand_capacity > 0 ? c.cand_capacity : 16;
+    b->pd = c.p_detect; b->gate = c.gate_chi2; b->lambda = c.clutter_density;
+    b->N.assign(n_particles, n_particles + S);
+    b->T.assign(n_frames, n_frames + S);
+    for (int s = 0; s < S; ++s)
+        if (b->N[s] <= 0 || b->N[s] > kMaxScanBlocks * kWeightsBlock || b->T[s] <= 0) {
+            delete b;
+            return set_error(PF_ERR_ARG, "sequence %d: bad size (N=%d, T=%d)", s, n_particles[s], n_frames[s]);
+        }
+    int cus = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device);
+    b->cus = cus > 0 ? cus : 256;
+    b->order.resize(S);
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int p, int q) { return b->T[p] > b->T[q]; });
+    b->rank.resize(S);
+    for (int i = 0; i < S; ++i) b->rank[b->order[i]] = i;
+    b->tofs_user.resize(S);
+    for (int s = 0, acc = 0; s < S; ++s) { b->tofs_user[s] = acc; acc += b->T[s]; }
+    b->h_seqs.resize(S);
+    long long pofs = 0, bofs = 0;
+    for (int i = 0; i < S; ++i) {
+        const int s = b->order[i];
+        SeqDesc &sd = b->h_seqs[i];
+        sd.pofs = (int)pofs;
+        sd.N = b->N[s];
+        sd.T = b->T[s];
+        sd.tofs = b->tofs_user[s];
+        sd.nb = (sd.N + kWeightsBlock - 1) / kWeightsBlock;
+        sd.bofs = (int)bofs;
+        pofs += (sd.N + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+        bofs += sd.nb;
+        b->nb_max = std::max(b->nb_max, sd.nb);
+        b->Tmax = std::max(b->Tmax, sd.T);
+    }
+    if (pofs > (1LL << 28)) { delete b; return set_error(PF_ERR_ARG, "batch too large (%lld particles)", pofs); }
+    b->ld = (int)pofs;
+    b->hm.resize(S);
+    b->loaded.assign(S, 0);
+    b->sel_ok.assign(S, 0);
+    b->fresh.assign(S, 0);
+    const size_t NP = npacked(d);
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = dallocb(&b->bx[k], (size_t)d * b->ld);
+        if (e == hipSuccess) e = dallocb(&b->bP[k], NP * b->ld);
+    }
+    e = e ? e : dallocb(&b->lw, (size_t)b->ld);
+    e = e ? e : dallocb(&b->anc, (size_t)b->ld);
+    e = e ? e : dallocb(&b->cloc, (size_t)b->ld);
+    e = e ? e : dallocb(&b->bsum, (size_t)bofs);
+    e = e ? e : dallocb(&b->bexp, (size_t)bofs);
+    e = e ? e : dallocb(&b->maxkey, 2 * (size_t)S);
+    e = e ? e : dallocb(&b->status, (size_t)S);
+    e = e ? e : dallocb(&b->Upend, (size_t)S);
+    e = e ? e : dallocb(&b->seqs, (size_t)S);
+    e = e ? e : dallocb(&b->frame, 2);
+    e = e ? e : dallocb(&b->bins, (size_t)kMaxScanBlocks * kWeightsBlock);
+    e = e ? e : dallocb(&b->partial, (size_t)kMaxScanBlocks * 64);
+    e = e ? e : dallocb(&b->est, 64);
+    e = e ? e : dallocb(&b->fs, 1);
+    if (e == hipSuccess) e = hipMemset(b->maxkey, 0, sizeof(unsigned long long) * 2 * S);
+    if (e == hipSuccess) e = hipMemset(b->status, 0, sizeof(unsigned) * S);
+    if (e == hipSuccess) e = hipMemset(b->fs, 0, sizeof(FrameScratch));
+    if (e == hipSuccess) e = hipMemcpy(b->seqs, b->h_seqs.data(), sizeof(SeqDesc) * S, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        pf_batch_destroy(b);
+        return set_error(PF_ERR_HIP, "batch allocation failed: %s", hipGetErrorString(e));
+    }
+    b->order_packed = joint_order(0, b->J);
+    int st = build_tiles(b);
+    if (st) { pf_batch_destroy(b); return st; }
+    *out = b;
+    return PF_OK;
+}
+
+int pf_batch_load_model(pf_batch *b, int32_t s, int32_t npose, const int32_t *sel, const double *means,
+                        const double *covs, const double *weights, const double *gamma, const double *pca_proj,
+                        const double *pca_mean, const double *R)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (s < 0 || s >= b->S) return set_error(PF_ERR_ARG, "sequence %d out of range", s);
+    HostModel hm;
+    st = host_model_from_args(hm, b->K, b->D, npose, b->m, sel, means, covs, weights, gamma, pca_proj, pca_mean, R);
+    if (st) return st;
+    b->hm[s] = hm;
+    b->loaded[s] = 1;
+    b->fresh[s] = 0;                          // tracks live in the basis of the model image
+    b->tracks = false;
+    return build_batch_tables(b);
+}
+
+int pf_batch_set_joint_clutter(pf_batch *b, const int32_t *clutter)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (std::any_of(b->fresh.begin(), b->fresh.end(), [](char c) { return c != 0; }))
+        return set_error(PF_ERR_STATE, "joint kinds must be set before the tracks are initialised");
+    unsigned mask = 0;
+    for (int j = 0; j < b->J; ++j)
+        if (clutter && clutter[j]) mask |= 1u << j;
+    b->clutter_mask = mask;
+    b->order_packed = joint_order(mask, b->J);
+    return build_batch_tables(b);
+}
+
+int pf_batch_reset(pf_batch *b, int32_t s, const int32_t *bins)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (s < 0 || s >= b->S || !bins) return set_error(PF_ERR_ARG, "bad reset arguments");
+    for (int i = 0; i < b->S; ++i)
+        if (!b->loaded[i]) return set_error(PF_ERR_STATE, "load every sequence's model first");
+    const int i = b->rank[s];
+    const SeqDesc &sd = b->h_seqs[i];
+    for (int j = 0; j < sd.N; ++j)
+        if (bins[j] < 0 || bins[j] >= b->K) return set_error(PF_ERR_ARG, "bins[%d] out of range", j);
+    HIP_TRYB(hipMemcpyAsync(b->bins, bins, sizeof(int) * sd.N, hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL((k_reset<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->bins,
+                       b->mus + (size_t)i * b->K * b->D, b->sigs + (size_t)i * b->K * npacked(b->D),
+                       b->bx[0] + sd.pofs, b->bP[0] + sd.pofs, sd.N, b->ld);
+    hipLaunchKernelGGL(k_identity_seq, dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->anc, sd.pofs, sd.N);
+    HIP_TRYB(hipGetLastError());
+    HIP_TRYB(hipStreamSynchronize(b->stream));
+    b->fresh[s] = 1;
+    b->tracks = std::all_of(b->fresh.begin(), b->fresh.end(), [](char c) { return c != 0; });
+    return PF_OK;
+}
+
+int pf_batch_run_dev(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uniforms, double *lml,
+                     int32_t flags)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (!b->tracks) return set_error(PF_ERR_STATE, "reset every sequence before a run");
+    if (!count || !cand || !uniforms || !lml) return set_error(PF_ERR_ARG, "null run argument");
+    b->tracks = false;                        // a run consumes the initial tracks
+    std::fill(b->fresh.begin(), b->fresh.end(), 0);
+    if ((size_t)step_lds_bytes(b->table_floats, b->nb_max) > (size_t)kMaxLdsBytes)
+        return set_error(PF_ERR_UNSUPPORTED, "model image and resampler workspace exceed the LDS budget");
+    hipEvent_t e0, e1;
+    HIP_TRYB(hipEventCreate(&e0));
+    HIP_TRYB(hipEventCreate(&e1));
+    HIP_TRYB(hipEventRecord(e0, b->stream));
+    st = run_all(b, count, cand, uniforms, lml, b->Tmax, (flags & 1) != 0);
+    if (st) return st;
+    BatchArgs a = batch_args(b, count, cand, uniforms, lml, 0, 1);
+    st = launch_batch_finish(b, a);
+    if (st) return st;
+    HIP_TRYB(hipEventRecord(e1, b->stream));
+    HIP_TRYB(hipEventSynchronize(e1));
+    HIP_TRYB(hipEventElapsedTime(&b->last_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return PF_OK;
+}
+
+int pf_batch_last_run_ms(pf_batch *b, double *ms)
+{
+    if (!b || !ms) return set_error(PF_ERR_ARG, "null argument");
+    *ms = b->last_ms;
+    return PF_OK;
+}
+
+int pf_batch_get_tracks(pf_batch *b, int32_t s, double *x, double *P)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (s < 0 || s >= b->S) return set_error(PF_ERR_ARG, "sequence out of range");
+    const int i = b->rank[s], d = b->D;
+    const SeqDesc &sd = b->h_seqs[i];
+    const size_t nx = (size_t)sd.N * d, np = (size_t)sd.N * d * d;
+    if (b->gbuf_bytes < sizeof(double) * (nx + np)) {
+        (void)hipFree(b->gbuf);
+        b->gbuf = nullptr;
+        HIP_TRYB(hipMalloc((void **)&b->gbuf, sizeof(double) * (nx + np)));
+        b->gbuf_bytes = sizeof(double) * (nx + np);
+    }
+    const int buf = sd.T & 1;   // frame f writes buffer (f + 1) & 1; the last frame is T - 1
+    hipLaunchKernelGGL((k_export<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->bx[buf] + sd.pofs,
+                       b->bP[buf] + sd.pofs, b->anc + sd.pofs, b->Tis + (size_t)i * d * d, b->gbuf, b->gbuf + nx,
+                       sd.N, b->ld);
+    HIP_TRYB(hipGetLastError());
+    if (x) HIP_TRYB(hipMemcpyAsync(x, b->gbuf, sizeof(double) * nx, hipMemcpyDeviceToHost, b->stream));
+    if (P) HIP_TRYB(hipMemcpyAsync(P, b->gbuf + nx, sizeof(double) * np, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRYB(hipStreamSynchronize(b->stream));
+    return PF_OK;
+}
+
+int pf_batch_estimate(pf_batch *b, int32_t s, double *xbar)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (s < 0 || s >= b->S || !xbar) return set_error(PF_ERR_ARG, "bad estimate arguments");
+    const int i = b->rank[s];
+    const SeqDesc &sd = b->h_seqs[i];
+    const int buf = sd.T & 1;
+    hipLaunchKernelGGL((k_estimate<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->bx[buf] + sd.pofs,
+                       b->anc + sd.pofs, sd.N, b->ld, b->partial, b->fs, b->Tis + (size_t)i * b->D * b->D, b->est);
+    HIP_TRYB(hipGetLastError());
+    HIP_TRYB(hipMemcpyAsync(xbar, b->est, sizeof(double) * b->D, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRYB(hipStreamSynchronize(b->stream));
+    return PF_OK;
+}
+
+int pf_batch_get_schedule(pf_batch *b, pf_batch_schedule *sch)
+{
+    if (!b || !sch) return set_error(PF_ERR_ARG, "null argument");
+    *sch = b->sch;
+    return PF_OK;
+}
+
+int pf_batch_set_schedule(pf_batch *b, const pf_batch_schedule *sch)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (!sch || !(sch->block == 64 || sch->block == 128 || sch->block == 256) || !(sch->ppl == 1 || sch->ppl == 2) ||
+        sch->blocks_per_cu < 1 || sch->blocks_per_cu > 8)
+        return set_error(PF_ERR_ARG, "bad schedule");
+    b->sch = *sch;
+    return build_tiles(b);
+}
+
+int pf_batch_autotune(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uniforms, double *lml,
+                      int32_t frames, pf_batch_schedule *best, double *best_ms, double *default_ms)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (!b->tracks) return set_error(PF_ERR_STATE, "reset every sequence before tuning");
+    if (!count || !cand || !uniforms || !lml) return set_error(PF_ERR_ARG, "null tuning argument");
+    b->tracks = false;
+    std::fill(b->fresh.begin(), b->fresh.end(), 0);
+    frames = std::max(1, std::min(frames, b->Tmax));
+    const pf_batch_schedule def{256, 1, 2};
+    std::vector<pf_batch_schedule> cands;
+    for (int blk : {64, 128, 256})
+        for (int ppl : {1, 2})
+            for (int bpc : {1, 2, 3, 4}) cands.push_back({blk, ppl, bpc});
+    hipEvent_t e0, e1;
+    HIP_TRYB(hipEventCreate(&e0));
+    HIP_TRYB(hipEventCreate(&e1));
+    double best_t = 1e30, def_t = -1.0;
+    pf_batch_schedule best_s = def;
+    for (const auto &c : cands) {
+        st = pf_batch_set_schedule(b, &c);
+        if (st) return st;
+        // one untimed warm-up frame pair, then `frames` timed frames
+        st = run_all(b, count, cand, uniforms, lml, 3, false);
+        if (st) return st;
+        HIP_TRYB(hipEventRecord(e0, b->stream));
+        st = run_all(b, count, cand, uniforms, lml, frames, false);
+        if (st) return st;
+        HIP_TRYB(hipEventRecord(e1, b->stream));
+        HIP_TRYB(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRYB(hipEventElapsedTime(&ms, e0, e1));
+        if (c.block == def.block && c.ppl == def.ppl && c.blocks_per_cu == def.blocks_per_cu) def_t = ms;
+        if (ms < best_t) { best_t = ms; best_s = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st = pf_batch_set_schedule(b, &best_s);
+    if (st) return st;
+    if (best) *best = best_s;
+    if (best_ms) *best_ms = best_t;
+    if (default_ms) *default_ms = def_t;
+    return PF_OK;
+}
+
+}  // extern "C"

@@ -125,51 +125,62 @@ __device__ __forceinline__ T wave_sum(T v)
 constexpr int kStageBlocks = 12;
 __host__ __device__ constexpr int resolve_lds_words(int nb) { return nb + 1 + kStageBlocks * kWeightsBlock + 8; }
 
-// a_j for outputs j in [j0, j0 + blockDim) (j = j0 + threadIdx.x); every
-// thread of the block must call it.  w (u64 LDS words) is the workspace.
-__device__ int resolve_ancestor(const Pending &pd, int N, uint32_t U, int j0, unsigned long long *w,
-                                unsigned long long *total_out, double *bexp_sum_out)
+// Phase 1 of the ancestor resolution: exclusive prefix of the block sums in
+// LDS (pre[0..nb], pre[nb] = total) and, if want_es, the FP64 sum of the
+// block weight sums in es.  Every thread of the block must call it (want_es
+// block-uniform).
+__device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, unsigned long long *w, bool want_es,
+                                                             double &es_out)
 {
     const int tid = threadIdx.x, nt = blockDim.x, nb = pd.nb;
     const int lane = tid & 63, wid = tid >> 6, nwaves = (nt + 63) >> 6;
-    unsigned long long *pre = w;                      // nb + 1
-    unsigned long long *stg = w + nb + 1;             // kStageBlocks * kWeightsBlock
-    unsigned long long *misc = stg + kStageBlocks * kWeightsBlock;  // 8
-    // 1. exclusive prefix of the block sums (chunked scan across the block)
+    unsigned long long *pre = w;
+    unsigned long long *stg = w + nb + 1;
+    unsigned long long *misc = stg + kStageBlocks * kWeightsBlock;
     const int per = (nb + nt - 1) / nt;
     const int c0 = min(nb, tid * per), c1 = min(nb, c0 + per);
     unsigned long long loc = 0;
-    double eloc = 0.0;
-    for (int k = c0; k < c1; ++k) {
-        loc += pd.bsum[k];
-        if (bexp_sum_out) eloc += pd.bexp[k];
-    }
+    for (int k = c0; k < c1; ++k) loc += pd.bsum[k];
     unsigned long long sc = loc;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const unsigned long long n = __shfl_up(sc, o, 64);
         if (lane >= o) sc += n;
     }
-    if (bexp_sum_out) eloc = wave_sum(eloc);
+    // FP64 weight sum in an order fixed by nb alone (lane l of wave 0 takes
+    // blocks l, l + 64, ...), so it does not depend on the block size.
+    double es = 0.0;
+    if (want_es && wid == 0) {
+        for (int k = lane; k < nb; k += 64) es += pd.bexp[k];
+        es = wave_sum(es);
+    }
+    __syncthreads();                     // LDS of a previous use is free
     if (lane == 63) misc[wid] = sc;
-    if (bexp_sum_out && lane == 0) reinterpret_cast<double *>(stg)[wid] = eloc;
+    if (want_es && tid == 0) reinterpret_cast<double *>(stg)[0] = es;
     __syncthreads();
     unsigned long long run = sc - loc;
     for (int k = 0; k < wid; ++k) run += misc[k];
     unsigned long long total = 0;
     for (int k = 0; k < nwaves; ++k) total += misc[k];
-    if (bexp_sum_out) {
-        double es = 0.0;
-        for (int k = 0; k < nwaves; ++k) es += reinterpret_cast<double *>(stg)[k];
-        *bexp_sum_out = es;
-    }
+    if (want_es) es_out = reinterpret_cast<double *>(stg)[0];
     for (int k = c0; k < c1; ++k) {
         pre[k] = run;
         run += pd.bsum[k];
     }
     if (tid == 0) pre[nb] = total;
-    *total_out = total;
     __syncthreads();
+    return total;
+}
+
+// Phase 2: a_j for outputs j in [j0, j0 + blockDim) (j = j0 + threadIdx.x),
+// given the prefix of phase 1.  Every thread of the block must call it.
+__device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned long long total, int j0,
+                              unsigned long long *w)
+{
+    const int tid = threadIdx.x, nt = blockDim.x, nb = pd.nb;
+    unsigned long long *pre = w;
+    unsigned long long *stg = w + nb + 1;
+    unsigned long long *misc = stg + kStageBlocks * kWeightsBlock;
     // 2. the input block of each output: largest b with pre[b] * S <= t_j
     const int j = j0 + tid;
     const int jc = min(j, N - 1);
@@ -183,6 +194,7 @@ __device__ int resolve_ancestor(const Pending &pd, int N, uint32_t U, int j0, un
     }
     const int bj = lo;
     const int jlast = min(j0 + nt, N) - 1;
+    __syncthreads();                     // stg / misc of a previous search are free
     if (j == j0) misc[4] = (unsigned long long)bj;
     if (j == jlast) misc[5] = (unsigned long long)bj;
     __syncthreads();
@@ -215,6 +227,14 @@ __device__ int resolve_ancestor(const Pending &pd, int N, uint32_t U, int j0, un
         a = l;
     }
     return a;
+}
+
+// Both phases for one block of outputs.
+__device__ __forceinline__ int resolve_ancestor(const Pending &pd, int N, uint32_t U, int j0, unsigned long long *w,
+                                                bool want_es, double &es_out)
+{
+    const unsigned long long total = resolve_prefix(pd, w, want_es, es_out);
+    return resolve_search(pd, N, U, total, j0, w);
 }
 
 // Frame statistics of a consumed resampling (block 0, thread 0): the log
@@ -339,20 +359,26 @@ struct Joint {
 
 // Copies n4 float4 of the model table into LDS: all loads of a chunk are
 // issued before the first LDS write so the copy costs one memory latency.
+// The loads are unconditional buffer loads (out-of-range lanes read 0), so
+// the chunk stays in registers.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void stage_table(float4 *dst, const float4 *__restrict__ src, int n4)
 {
     constexpr int kChunk = 4;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(src), (short)0, n4 * 16, 0x00020000);
+    u32x4 *d4 = reinterpret_cast<u32x4 *>(dst);
     for (int base = 0; base < n4; base += kChunk * (int)blockDim.x) {
-        float4 r[kChunk];
+        u32x4 r[kChunk];
 #pragma unroll
         for (int it = 0; it < kChunk; ++it) {
             const int i = base + it * (int)blockDim.x + (int)threadIdx.x;
-            if (i < n4) r[it] = src[i];
+            r[it] = __builtin_amdgcn_raw_buffer_load_b128(rs, i * 16, 0, 0);
         }
 #pragma unroll
         for (int it = 0; it < kChunk; ++it) {
             const int i = base + it * (int)blockDim.x + (int)threadIdx.x;
-            if (i < n4) dst[i] = r[it];
+            if (i < n4) d4[i] = r[it];
         }
     }
 }
@@ -470,11 +496,70 @@ __host__ __device__ constexpr int step_lds_bytes(int table_floats, int nb_pendin
     return 4 * table_floats + (nb_pending > 0 ? 8 * resolve_lds_words(nb_pending) : 0);
 }
 
+// One particle's frame: component draw (pf2DRao.cpp:128), mean-reverting
+// predict (KF_model.cpp:11-15) and the joint updates (pf2DRao.cpp:138-140);
+// returns the particle's log-weight.  sm is the staged LDS model image.
+template <int D, int MODE, int SELJ>
+__device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j, uint32_t U1, const float *sm,
+                                                 float (&x)[D], float (&P)[npacked(D)])
+{
+    constexpr int NP = npacked(D);
+    constexpr int CS = 2 + D + NP;
+    const unsigned long long *sthr = reinterpret_cast<const unsigned long long *>(sm);
+    const float *stab = sm + 4 * a.K;
+    const int k = draw_component(sthr, a.K, (uint32_t)j, U1, a.pi_total);
+    if (a.comp_out) a.comp_out[j] = k;
+    const float *ck = stab + k * CS;
+    const float g = ck[0], g2 = ck[1];
+#pragma unroll
+    for (int i = 0; i < D; ++i) x[i] = fmaf(g, x[i], ck[2 + i]);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) P[p] = fmaf(g2, P[p], ck[2 + D + p]);
+
+    const float *Hs = stab + a.K * CS;
+    const float *BHs = Hs + a.m * D;
+    const float *Rs = BHs + a.m;
+    float acc = 0.f;
+    if constexpr (SELJ > 0) {
+#pragma unroll
+        for (int oi = 0; oi < SELJ; ++oi) {
+            const int b = (int)((a.order_packed >> (4 * oi)) & 15u);
+            int cnt;
+            const float *cz;
+            bool clutter;
+            if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter)) continue;
+            float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
+            Joint<D>::prepare_sel(x, P, 2 * oi, BHs[2 * b], BHs[2 * b + 1], Rs[3 * b], Rs[3 * b + 1],
+                                  Rs[3 * b + 2], w0, w1, y0, y1, il00, l10, il11, logl);
+            float v0, v1, m00, m01, m11;
+            acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
+                                       v1, m00, m01, m11);
+            Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
+        }
+    } else {
+        for (int oi = 0; oi < a.J; ++oi) {
+            const int b = (MODE == kModeRef) ? oi : (int)((a.order_packed >> (4 * oi)) & 15u);
+            int cnt;
+            const float *cz;
+            bool clutter;
+            if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter)) continue;
+            float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
+            Joint<D>::prepare(x, P, Hs + (2 * b) * D, Hs + (2 * b + 1) * D, BHs[2 * b], BHs[2 * b + 1],
+                              Rs[3 * b], Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0, y1, il00, l10,
+                              il11, logl);
+            float v0, v1, m00, m01, m11;
+            acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
+                                       v1, m00, m01, m11);
+            Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
+        }
+    }
+    return acc;
+}
+
 template <int D, int MODE, int SELJ>
 __global__ void __launch_bounds__(256) k_step(StepArgs a)
 {
     constexpr int NP = npacked(D);
-    constexpr int CS = 2 + D + NP;
     extern __shared__ float4 sm4[];
     const int t = *a.frame;
     const uint32_t U1 = a.uni[2 * t];
@@ -487,10 +572,8 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
         // statistics (block 0).
         unsigned long long *wsp = reinterpret_cast<unsigned long long *>(
             reinterpret_cast<float *>(sm4) + a.table_floats);
-        unsigned long long total;
         double es = 0.0;
-        src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, &total,
-                               blockIdx.x == 0 ? &es : nullptr);
+        src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0, es);
         if (live) a.anc[j] = src;
         if (blockIdx.x == 0 && threadIdx.x == 0)
             finish_resampling(a.fs, a.parity ^ 1, es, a.N, 0, a.lml ? a.lml + (t - 1) : nullptr);
@@ -511,58 +594,10 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
     }
     stage_table(sm4, reinterpret_cast<const float4 *>(a.table), a.table_floats >> 2);
     __syncthreads();
-    const unsigned long long *sthr = reinterpret_cast<const unsigned long long *>(sm4);
-    const float *stab = reinterpret_cast<const float *>(sm4) + 4 * a.K;
 
     float lwj = -INFINITY;
     if (live) {
-        const int k = draw_component(sthr, a.K, (uint32_t)j, U1, a.pi_total);
-        if (a.comp_out) a.comp_out[j] = k;
-        const float *ck = stab + k * CS;
-        const float g = ck[0], g2 = ck[1];
-#pragma unroll
-        for (int i = 0; i < D; ++i) x[i] = fmaf(g, x[i], ck[2 + i]);
-#pragma unroll
-        for (int p = 0; p < NP; ++p) P[p] = fmaf(g2, P[p], ck[2 + D + p]);
-
-        const float *Hs = stab + a.K * CS;
-        const float *BHs = Hs + a.m * D;
-        const float *Rs = BHs + a.m;
-        float acc = 0.f;
-        if constexpr (SELJ > 0) {
-#pragma unroll
-            for (int oi = 0; oi < SELJ; ++oi) {
-                const int b = (int)((a.order_packed >> (4 * oi)) & 15u);
-                int cnt;
-                const float *cz;
-                bool clutter;
-                if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter)) continue;
-                float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
-                Joint<D>::prepare_sel(x, P, 2 * oi, BHs[2 * b], BHs[2 * b + 1], Rs[3 * b], Rs[3 * b + 1],
-                                      Rs[3 * b + 2], w0, w1, y0, y1, il00, l10, il11, logl);
-                float v0, v1, m00, m01, m11;
-                acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
-                                           v1, m00, m01, m11);
-                Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
-            }
-        } else {
-            for (int oi = 0; oi < a.J; ++oi) {
-                const int b = (MODE == kModeRef) ? oi : (int)((a.order_packed >> (4 * oi)) & 15u);
-                int cnt;
-                const float *cz;
-                bool clutter;
-                if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter)) continue;
-                float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
-                Joint<D>::prepare(x, P, Hs + (2 * b) * D, Hs + (2 * b + 1) * D, BHs[2 * b], BHs[2 * b + 1],
-                                  Rs[3 * b], Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0, y1, il00, l10,
-                                  il11, logl);
-                float v0, v1, m00, m01, m11;
-                acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
-                                           v1, m00, m01, m11);
-                Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
-            }
-        }
-        lwj = acc;
+        lwj = particle_update<D, MODE, SELJ>(a, t, j, U1, reinterpret_cast<const float *>(sm4), x, P);
         const unsigned doff = (unsigned)j * 4u;
         const __amdgpu_buffer_rsrc_t wx = soa_rsrc(a.xout, rowb * D), wP = soa_rsrc(a.Pout, rowb * NP);
 #pragma unroll
@@ -667,10 +702,8 @@ static __global__ void __launch_bounds__(256) k_ancestors(AncArgs a)
     extern __shared__ unsigned long long wsp[];
     const int j0 = blockIdx.x * blockDim.x;
     const int j = j0 + threadIdx.x;
-    unsigned long long total;
     double es = 0.0;
-    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, &total,
-                                     blockIdx.x == 0 ? &es : nullptr);
+    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0, es);
     if (j < a.N) a.anc[j] = src;
     if (blockIdx.x == 0 && threadIdx.x == 0) finish_resampling(a.fs, a.parity, es, a.N, a.kind_linear, a.lml);
 }

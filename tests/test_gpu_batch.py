@@ -1,0 +1,140 @@
+"""Batched multi-sequence engine (include/pf_batch.h) against independent
+single-filter runs of the same sequences.
+
+Every sequence of a batch must reproduce, bit for bit, the tracks, the final
+ancestors and the per-frame log mean weights of its own pf_run_candidates_dev
+run (same model, uniforms and detections), for every schedule: the batch
+shares kernels' per-particle arithmetic with the single filter and its
+resampler sums are in an order fixed by the particle count alone.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from mkfbodytracker_pdaf_amd import synthetic
+from mkfbodytracker_pdaf_amd.batch import RbpfBatch
+from mkfbodytracker_pdaf_amd.rbpf import Rbpf
+
+pytestmark = pytest.mark.gpu
+
+CAP = 8
+LAM = 3.0 / (640 * 480)
+
+
+def _problem(Ns, Ts, seed0=11):
+    models, seqs, bins = [], [], []
+    for s, (N, T) in enumerate(zip(Ns, Ts)):
+        mdl = synthetic.make_model(seed=seed0 + s)
+        models.append(mdl)
+        seqs.append(synthetic.simulate(mdl, T, seed=100 + s, cand_per_joint=4, cap=CAP))
+        bins.append(oracle.resample_linear(mdl.weights, N, 1000 + s)[0])
+    return models, seqs, bins
+
+
+def _single(mdl, seq, bins, N, T):
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    rb = Rbpf(N, cand_capacity=CAP, clutter_density=LAM)
+    rb.load_model(mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, sel=mdl.sel, R=mdl.R)
+    rb.set_joint_clutter(mdl.clutter)
+    rb.reset(bins)
+    d_uni = DeviceArray(seq.uniforms.astype(np.uint32))
+    d_lml = DeviceArray.empty(T, np.float64)
+    d_cnt = DeviceArray(seq.count.astype(np.int32))
+    d_cand = DeviceArray(seq.cand.astype(np.float32))
+    rb.run_candidates_dev(T, d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, 1)
+    rb.synchronize()
+    x, P = rb.get_tracks()
+    est = rb.estimate()
+    lml = d_lml.get()
+    rb.close()
+    return x, P, est, lml
+
+
+def _batch_inputs(seqs):
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    cnt = np.concatenate([q.count for q in seqs]).astype(np.int32)
+    cand = np.concatenate([q.cand for q in seqs]).astype(np.float32)
+    uni = np.concatenate([q.uniforms for q in seqs]).astype(np.uint32)
+    F = cnt.shape[0]
+    return DeviceArray(cnt), DeviceArray(cand), DeviceArray(uni), DeviceArray.empty(F, np.float64)
+
+
+def _make_batch(models, bins, Ns, Ts):
+    b = RbpfBatch(Ns, Ts, K=models[0].K, m=models[0].m, cand_capacity=CAP, clutter_density=LAM)
+    for s, mdl in enumerate(models):
+        b.load_model(s, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
+    b.set_joint_clutter(models[0].clutter)
+    for s in range(len(Ns)):
+        b.reset(s, bins[s])
+    return b
+
+
+NS = [300, 1000, 2500, 777, 2049, 64]
+TS = [5, 17, 12, 1, 9, 17]
+
+
+@pytest.fixture(scope="module")
+def problem():
+    models, seqs, bins = _problem(NS, TS)
+    ref = [_single(models[s], seqs[s], bins[s], NS[s], TS[s]) for s in range(len(NS))]
+    return models, seqs, bins, ref
+
+
+def _check(b, ref, lml_all):
+    offs = b.frame_offsets
+    for s in range(b.S):
+        x, P = b.get_tracks(s)
+        xr, Pr, er, lr = ref[s]
+        assert np.array_equal(x, xr), s
+        assert np.array_equal(P, Pr), s
+        assert np.array_equal(b.estimate(s), er), s
+        lb = lml_all[offs[s]:offs[s] + TS[s]]
+        assert np.array_equal(lb, lr), (s, lb, lr)
+
+
+@pytest.mark.parametrize("schedule,graph", [((256, 1, 2), True), ((256, 1, 2), False), ((64, 2, 1), True),
+                                            ((128, 2, 3), True), ((128, 1, 4), False)])
+def test_batch_matches_single_runs(gpu, problem, schedule, graph):
+    models, seqs, bins, ref = problem
+    b = _make_batch(models, bins, NS, TS)
+    b.schedule = schedule
+    assert b.schedule == schedule
+    d_cnt, d_cand, d_uni, d_lml = _batch_inputs(seqs)
+    b.run_dev(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, graph=graph)
+    assert b.last_run_ms() > 0
+    _check(b, ref, d_lml.get())
+    b.close()
+
+
+def test_batch_autotune_then_run(gpu, problem):
+    models, seqs, bins, ref = problem
+    b = _make_batch(models, bins, NS, TS)
+    d_cnt, d_cand, d_uni, d_lml = _batch_inputs(seqs)
+    best, best_ms, def_ms = b.autotune(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, frames=8)
+    assert best == b.schedule
+    assert best[0] in (64, 128, 256) and best[1] in (1, 2)
+    assert 0 < best_ms <= def_ms
+    for s in range(len(NS)):                  # tracks were consumed by the timing runs
+        b.reset(s, bins[s])
+    b.run_dev(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr)
+    _check(b, ref, d_lml.get())
+    b.close()
+
+
+def test_batch_errors(gpu):
+    from mkfbodytracker_pdaf_amd.capi import PfError
+    with pytest.raises(PfError):
+        RbpfBatch([100], [5], K=15, m=6, d=8)           # the batched engine runs d = 12
+    with pytest.raises(PfError):
+        RbpfBatch([0], [5], K=15, m=6)
+    b = RbpfBatch([100, 50], [5, 3], K=15, m=6)
+    mdl = synthetic.make_model(seed=1)
+    with pytest.raises(PfError):                        # reset before every model is loaded
+        b.reset(0, np.zeros(100, np.int32))
+    with pytest.raises(PfError):
+        b.schedule = (96, 1, 2)
+    b.load_model(0, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
+    b.load_model(1, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
+    with pytest.raises(PfError):
+        b.reset(1, np.full(50, 15, np.int32))           # component index out of range
+    b.close()
