@@ -10,6 +10,10 @@ all N particles; the timed region is device-resident (inputs in HBM).
 Prints one JSON line with the metric, the roofline of the dominant kernel
 (the fused step kernel, HBM-bound) and the CPU baseline (the FP64 oracle on a
 bounded sample of the same workload, on this host).
+
+``--workload batched`` instead runs many ragged sequences (lengths
+U{100..2000} frames, N log-uniform in [1k, 64k], own GMM each) through the
+batched engine and reports the default and the autotuned schedule.
 """
 from __future__ import annotations
 
@@ -47,10 +51,17 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="step kernel block size (0 = default)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this size (PMC calibration)")
+    ap.add_argument("--workload", choices=("single", "batched"), default="single",
+                    help="single: one sequence of N particles (the headline line); batched: many ragged "
+                         "sequences in one persistent step per frame, tuned vs default schedule")
+    ap.add_argument("--sequences", type=int, default=256, help="batched: number of sequences")
+    ap.add_argument("--tune-frames", type=int, default=32, help="batched: frames timed per candidate schedule")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "batched":
+        return run_batched(args)
 
     from mkfbodytracker_pdaf_amd import synthetic
     from mkfbodytracker_pdaf_amd.devmem import DeviceArray, synchronize
@@ -113,6 +124,7 @@ def main():
         b.copy_from(a)
         synchronize()
 
+    traffic = measured_traffic(N, mdl.d, mdl.K)
     evals = N * K
     out = {
         "metric": "likelihood evals/s at N=100k particles (single sequence, PDAF candidates)",
@@ -134,7 +146,12 @@ def main():
         "frames_per_s": K / dt,
         "lml_total": float(np.sum(lml)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if traffic is None else traffic["bytes_per_launch"],
+                     "traffic_source": None if traffic is None else
+                     "profiles/k_step_cand_n100k.md (rocprofv3 FETCH_SIZE x %.2f + WRITE_SIZE x %.2f, "
+                     "copy-kernel calibration)" % (traffic["calib_fetch"], traffic["calib_write"]),
+                     "algorithmic_bytes": bpp * N,
                      "kernel": "k_step<12,cand>", "bytes_per_particle": bpp,
                      "avg_launch_us": step_avg_s * 1e6, "launches": launches,
                      "frame_us_no_graph": frame_ms_total / launches * 1e3},
@@ -146,7 +163,117 @@ def main():
     print(json.dumps(out))
 
 
-def cpu_baseline(mdl, seq, N, cap, lam, budget_s):
+def measured_traffic(N, d, K):
+    """HBM bytes per step-kernel launch measured with rocprofv3 counters on
+    this workload (the committed summary next to the profile), or None for
+    other sizes."""
+    path = os.path.join(ROOT, "profiles", "k_step_cand_n100k_traffic.json")
+    if (N, d, K) != (100_000, 12, 15) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def run_batched(args):
+    """Many independent sequences (lengths U{100..2000} frames, particle counts
+    log-uniform in [1k, 64k], each with its own seeded GMM) advanced together,
+    one persistent step kernel and one weights kernel per frame, the frame
+    loop replayed from a hipGraph.  Times the whole run with the default
+    schedule, autotunes, then times the whole run again with the tuned one."""
+    from mkfbodytracker_pdaf_amd import synthetic
+    from mkfbodytracker_pdaf_amd.batch import RbpfBatch
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray, synchronize
+    from mkfbodytracker_pdaf_amd.rbpf import Rbpf
+
+    seed, S, cap, cpj = 7, args.sequences, 16, 8
+    lam = (cpj - 1) / (640.0 * 480.0)
+    rng = np.random.default_rng(seed)
+    Ts = rng.integers(100, 2001, S).astype(np.int32)
+    Ns = np.exp(rng.uniform(np.log(1000), np.log(64000), S)).astype(np.int32)
+    models = [synthetic.make_model(seed=seed * 1000 + s) for s in range(S)]
+    seqs = [synthetic.simulate_fast(models[s], int(Ts[s]), seed=seed * 1000 + s, cand_per_joint=cpj, cap=cap)
+            for s in range(S)]
+    b = RbpfBatch(Ns, Ts, K=models[0].K, m=models[0].m, cand_capacity=cap, clutter_density=lam)
+    for s, mdl in enumerate(models):
+        b.load_model(s, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
+    b.set_joint_clutter(models[0].clutter)
+    helper = Rbpf(int(Ns.max()))
+    bins = [synthetic.init_bins(models[s], int(Ns[s]), seed * 1000 + s,
+                                lambda w, n, U: helper.resample(w, n, U)[0]) for s in range(S)]
+    helper.close()
+
+    def reset_all():
+        for s in range(S):
+            b.reset(s, bins[s])
+
+    d_cnt = DeviceArray(np.concatenate([q.count for q in seqs]).astype(np.int32))
+    d_cand = DeviceArray(np.concatenate([q.cand for q in seqs]).astype(np.float32))
+    d_uni = DeviceArray(np.concatenate([q.uniforms for q in seqs]).astype(np.uint32))
+    F = int(Ts.sum())
+    d_lml = DeviceArray.empty(F, np.float64)
+
+    def timed_run():
+        reset_all()
+        synchronize()
+        t0 = time.perf_counter()
+        b.run_dev(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, graph=not args.no_graph)
+        synchronize()
+        return time.perf_counter() - t0, b.last_run_ms() / 1e3
+
+    default = (256, 1, 2)
+    b.schedule = default
+    timed_run()                                   # warm-up run
+    t_def, dev_def = timed_run()
+    lml_def = d_lml.get()
+    reset_all()
+    best, tune_best_ms, tune_def_ms = b.autotune(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr,
+                                                 frames=args.tune_frames)
+    t_tuned, dev_tuned = timed_run()
+    lml_tuned = d_lml.get()
+
+    evals = float(np.sum(Ns.astype(np.int64) * Ts))
+    bpp = step_bytes_per_particle(models[0].d)
+    achieved = bpp * evals / t_tuned / 1e9
+    out = {
+        "metric": "likelihood evals/s, batched ragged sequences (PDAF candidates)",
+        "value": evals / t_tuned,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": int(Ts.max()),
+        "warmup": 1,
+        "ms_per_step": t_tuned / int(Ts.max()) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (per-sequence seeded GMM K=15 d=12, pinhole-projected 3-joint pose, Philox uniforms)",
+        "config": {"workload": "batched: %d sequences, lengths U{100..2000} frames, N log-uniform 1k..64k, "
+                               "K=15, d=12, m=6, 8 candidates per hand" % S,
+                   "sequences": S, "particle_frames": evals, "sequence_frames": F,
+                   "particles_total": int(Ns.sum()), "frames_max": int(Ts.max())},
+        "sequence_frames_per_s": F / t_tuned,
+        "schedule_default": {"block": default[0], "ppl": default[1], "blocks_per_cu": default[2],
+                             "wall_s": t_def, "device_s": dev_def, "evals_per_s": evals / t_def},
+        "schedule_tuned": {"block": best[0], "ppl": best[1], "blocks_per_cu": best[2],
+                           "wall_s": t_tuned, "device_s": dev_tuned, "evals_per_s": evals / t_tuned,
+                           "tuning_frames": args.tune_frames, "tuning_ms_best": tune_best_ms,
+                           "tuning_ms_default": tune_def_ms},
+        "tuned_over_default": t_def / t_tuned,
+        "lml_total": float(np.sum(lml_tuned)),
+        "lml_identical_across_schedules": bool(np.array_equal(lml_def, lml_tuned)),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_batch_step + k_batch_weights (whole run)", "bytes_per_particle": bpp},
+    }
+    if not args.no_cpu:
+        s0 = int(np.argmax(Ns))
+        out["cpu_baseline"] = cpu_baseline(models[s0], seqs[s0], int(Ns[s0]), cap, lam, args.cpu_seconds,
+                                           seed=seed * 1000 + s0)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+def cpu_baseline(mdl, seq, N, cap, lam, budget_s, seed=2024):
     """FP64 oracle (1 thread) on the same workload: same model, the same
     frames' detections and uniforms, N particles, as many frames as fit in the
     budget (at least 2)."""
@@ -154,7 +281,7 @@ def cpu_baseline(mdl, seq, N, cap, lam, budget_s):
     oracle.set_threads(1)
     om = oracle.Model(mdl.weights, mdl.gamma, mdl.means, mdl.Sigma, mdl.H(), mdl.BH(), mdl.R)
     from mkfbodytracker_pdaf_amd import synthetic
-    bins = synthetic.init_bins(mdl, N, 2024, lambda w, n, U: oracle.resample_linear(w, n, U)[0])
+    bins = synthetic.init_bins(mdl, N, seed, lambda w, n, U: oracle.resample_linear(w, n, U)[0])
     x, P = om.reset(bins)
     frames = 0
     t0 = time.perf_counter()

@@ -165,6 +165,46 @@ def simulate(model: Model, T: int, seed: int, cand_per_joint: int = 4, cap: int 
     return Sequence(xs, comps, zs, count, cand, uni)
 
 
+def simulate_fast(model: Model, T: int, seed: int, cand_per_joint: int = 4, cap: int = 8,
+                  noise_px: float = 5.0) -> Sequence:
+    """Same generative model as ``simulate`` with all random draws made in
+    bulk (a different random stream): for long or many sequences."""
+    rng = np.random.default_rng(seed + 2000)
+    K, d, m = model.K, model.d, model.m
+    J = m // 2
+    L = np.linalg.cholesky(model.Sigma)
+    comps = rng.choice(K, size=T, p=model.weights).astype(np.int32)
+    k0 = rng.choice(K, p=model.weights)
+    x = model.means[k0] + L[k0] @ rng.standard_normal(d)
+    g = model.gamma[comps]
+    eps = rng.standard_normal((T, d))
+    drift = (1 - g)[:, None] * model.means[comps] + np.sqrt(1 - g * g)[:, None] * np.einsum("tij,tj->ti", L[comps], eps)
+    xs = np.empty((T, d))
+    for t in range(T):
+        x = g[t] * x + drift[t]
+        xs[t] = x
+    H, BH = model.H(), model.BH()
+    zs = xs @ H.T + BH + noise_px * rng.standard_normal((T, m))
+    count = np.zeros((T, J), np.int32)
+    cand = np.zeros((T, J, cap, 2))
+    for b in range(J):
+        zb = zs[:, 2 * b:2 * b + 2]
+        if model.clutter[b]:
+            n = np.minimum(1 + rng.poisson(cand_per_joint - 1, T), cap)
+            pts = np.stack([rng.uniform(0, WIDTH, (T, cap)), rng.uniform(0, HEIGHT, (T, cap))], -1)
+            slot = (rng.random(T) * n).astype(np.int64)
+            pts[np.arange(T), slot] = zb
+            pts[np.arange(cap)[None, :] >= n[:, None]] = 0.0
+            count[:, b] = n
+            cand[:, b] = pts
+        else:
+            count[:, b] = 1
+            cand[:, b, 0] = zb
+    from .capi import philox_u32
+    uni = philox_u32(seed, 1, 0, 2 * T).reshape(T, 2)
+    return Sequence(xs, comps, zs, count, cand, uni)
+
+
 def init_bins(model: Model, N: int, seed: int, resample):
     """Initial components: resample(gmm.weight, N) then resetTracker
     (src/pfPose.cpp:68-71).  ``resample(w, N, U)`` is the resampler to use."""
