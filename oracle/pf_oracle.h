@@ -9,8 +9,8 @@
  * Every function cites the reference file:line whose behaviour it restates
  * (paths relative to the reference repository root).  Where the reference's
  * literal arithmetic is version-dependent or unintended, the oracle implements
- * the specified algorithm and names the deviation (quirk IDs Q1..Q15 match the
- * table in the design notes).
+ * the specified algorithm and names the deviation as a quirk ID (Q1..Q15,
+ * each stated where it is used).
  *
  * Layouts: all matrices row-major double.  Particle states are AoS:
  * x[N*d], P[N*d*d].  A measurement matrix Z (m x N) is row-major, i.e. the
