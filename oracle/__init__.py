@@ -54,6 +54,9 @@ def lib():
             build()
         L = C.CDLL(_LIB_PATH)
         L.orc_philox4x32_10.argtypes = [_up, _up, _up]
+        L.orc_adj_n_theta.argtypes = [C.c_void_p]
+        L.orc_adj_value.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_adj_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_philox_u32.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_int64, _up]
         L.orc_det_exp.argtypes = [C.c_double]
         L.orc_det_exp.restype = C.c_double
@@ -272,3 +275,68 @@ def pf2d_estimate(particles, weights):
 
 def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
+
+
+# ------------------------------------------------------------- adjoint reference
+class _AdjProblem(C.Structure):
+    _fields_ = [("K", C.c_int), ("d", C.c_int), ("m", C.c_int), ("N", C.c_int), ("T", C.c_int),
+                ("cap", C.c_int), ("H", C.c_void_p), ("BH", C.c_void_p), ("R", C.c_void_p),
+                ("clutter", C.c_void_p), ("count", C.c_void_p), ("cand", C.c_void_p),
+                ("pd", C.c_double), ("gate", C.c_double), ("lam", C.c_double),
+                ("bins", C.c_void_p), ("comp", C.c_void_p), ("anc", C.c_void_p)]
+
+
+class AdjProblem:
+    """A recorded candidate-mode run (frozen component draws comp[T][N] and
+    ancestors anc[T][N], initial bins[N], detections) whose log-marginal-
+    likelihood is differentiated w.r.t. theta = [means | logits | rates | chol]
+    (``orc_adj_*`` of ``pf_oracle.h``)."""
+
+    def __init__(self, H, BH, R, clutter, count, cand, bins, comp, anc, pd=0.95,
+                 gate=9.210340371976184, lam=3.0 / (640 * 480), K=15):
+        self._keep = [_f64(H), _f64(BH), _f64(R), _i32(clutter), _i32(count), _f64(cand),
+                      _i32(bins), _i32(comp), _i32(anc)]
+        H_, BH_, R_, cl_, cnt_, cand_, bins_, comp_, anc_ = self._keep
+        T, N = comp_.shape
+        m, d = H_.shape
+        cap = cand_.shape[2]
+        self.K, self.d, self.m, self.N, self.T = K, d, m, N, T
+        self.s = _AdjProblem(K, d, m, N, T, cap, H_.ctypes.data, BH_.ctypes.data, R_.ctypes.data,
+                             cl_.ctypes.data, cnt_.ctypes.data, cand_.ctypes.data, pd, gate, lam,
+                             bins_.ctypes.data, comp_.ctypes.data, anc_.ctypes.data)
+        self.n_theta = lib().orc_adj_n_theta(C.byref(self.s))
+
+    @staticmethod
+    def theta(means, weights, gamma, covs):
+        """theta from the my_gmm parameters: logits = log(weights), rates = 1 - gamma,
+        chol = lower Cholesky factors of the covariances."""
+        means = _f64(means)
+        K, d = means.shape
+        Ls = np.linalg.cholesky(_f64(covs).reshape(K, d, d))
+        return np.concatenate([means.ravel(), np.log(_f64(weights)), 1.0 - _f64(gamma), Ls.ravel()])
+
+    def split(self, theta):
+        K, d = self.K, self.d
+        a = K * d
+        return (theta[:a].reshape(K, d), theta[a:a + K], theta[a + K:a + 2 * K],
+                theta[a + 2 * K:].reshape(K, d, d))
+
+    def value(self, theta, logpi0=None):
+        theta = _f64(theta)
+        if logpi0 is None:
+            lg = theta[self.K * self.d:self.K * self.d + self.K]
+            logpi0 = lg - np.logaddexp.reduce(lg)
+        F = C.c_double(0.0)
+        lml = np.zeros(self.T)
+        rc = lib().orc_adj_value(C.byref(self.s), theta.ctypes.data, _f64(logpi0).ctypes.data, C.byref(F),
+                                 lml.ctypes.data)
+        assert rc == 0
+        return F.value, lml
+
+    def grad(self, theta):
+        theta = _f64(theta)
+        F = C.c_double(0.0)
+        g = np.zeros(self.n_theta)
+        rc = lib().orc_adj_grad(C.byref(self.s), theta.ctypes.data, C.byref(F), g.ctypes.data)
+        assert rc == 0
+        return F.value, g

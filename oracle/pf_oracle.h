@@ -179,6 +179,33 @@ int orc_pf2d_update(const orc_pf2d_gmm *g, int N, double *particles, double *wei
 /* ParticleFilter::getEstimator (pf2D.cpp:79-87): sum_i weights_i particles_i. */
 void orc_pf2d_estimate(int N, int d, const double *particles, const double *weights, double *out);
 
+/* ------------------------------------------------- adjoint reference */
+/* The log-marginal-likelihood of a recorded candidate-mode run as a function
+ * of theta = [means K*d | logits K | rates K | chol K*d*d (lower used)], with
+ * the run's component draws comp[T][N] and ancestors anc[T][N] frozen
+ * (resampling as a stop-gradient; anc[t] is the resampling at the end of
+ * frame t), initial components bins[N] (my_gmm.cpp:30-42) and the frames'
+ * detections.  Each particle's log-weight carries log pi_k(theta) - log pi0_k
+ * (value 0 at pi0), so d/d logits is the score of the component draws.
+ * Model basis, FP64 (pf_adjoint_ref.cpp). */
+typedef struct orc_adj_problem {
+    int K, d, m, N, T, cap;
+    const double *H, *BH, *R;      /* m*d, m, m*m */
+    const int *clutter;            /* J = m/2 */
+    const int *count;              /* T*J */
+    const double *cand;            /* T*J*cap*2 */
+    double pd, gate, lambda;
+    const int *bins;               /* N */
+    const int *comp, *anc;         /* T*N each */
+} orc_adj_problem;
+int orc_adj_n_theta(const orc_adj_problem *p);
+/* F(theta) with the score term taken against logpi0 (K); lml (T, optional)
+ * receives the per-frame log mean weights. */
+int orc_adj_value(const orc_adj_problem *p, const double *theta, const double *logpi0, double *F, double *lml);
+/* F and dF/dtheta at theta (pi0 = softmax(logits of theta)), by forward-mode
+ * dual numbers, 8 directions per pass. */
+int orc_adj_grad(const orc_adj_problem *p, const double *theta, double *F, double *grad);
+
 /* ------------------------------------------------------------- misc */
 /* Number of OpenMP threads used by the particle loops (1 = sequential). */
 void orc_set_threads(int n);
