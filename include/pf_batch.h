@@ -70,6 +70,33 @@ int pf_batch_autotune(pf_batch *b, const int32_t *count, const float *cand, cons
 /* Device time of the last pf_batch_run_dev in ms. */
 int pf_batch_last_run_ms(pf_batch *b, double *ms);
 
+/* Log-marginal-likelihood and its gradient, per sequence.  No counterpart in
+ * the reference (it has no learning); the objective is the sum over frames
+ * of the log mean weight that ParticleFilter::update's weights define
+ * (pf2DRao.cpp:138-148), with resampling (pf2DRao.cpp:152) a stop-gradient:
+ * the run's ancestors and component draws are held fixed, and each
+ * log-weight carries log pi_k - stopgrad(log pi_k), whose gradient is the
+ * score of the component draws.  Runs the sequences forward as
+ * pf_batch_run_dev (same inputs; lml written the same way) recording the
+ * choices and checkpointing the state every checkpoint_every frames, then the
+ * adjoint backward.  Gradients (host, FP64, model basis, NULL to skip), for
+ * sequence s at offset s * (size of one sequence's block):
+ *   grad_means  (S x K x d)      d F / d means (my_gmm.cpp:50 u)
+ *   grad_logits (S x K)          d F / d logits, weights = softmax(logits)
+ *   grad_rates  (S x K)          d F / d r, gamma = 1 - r (my_gmm.cpp:56)
+ *   grad_chol   (S x K x d x d)  d F / d L, covs = L L^T, lower triangle (upper 0)
+ * The tracks are consumed (reset before the next run). */
+int pf_batch_lml_grad(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uniforms,
+                      double *lml, int32_t checkpoint_every, double *grad_means, double *grad_logits,
+                      double *grad_rates, double *grad_chol);
+/* Component draws and ancestors of sequence s recorded by the last
+ * pf_batch_lml_grad (host, n_frames[s] x n_particles[s] each, NULL to skip):
+ * comp[t][j] is particle j's component at frame t (pf2DRao.cpp:128), anc[t][j]
+ * its ancestor in frame t's resampling (pf2DRao.cpp:152). */
+int pf_batch_get_choices(pf_batch *b, int32_t s, int32_t *comp, int32_t *anc);
+/* Device times of the last pf_batch_lml_grad: recorded forward, backward (ms). */
+int pf_batch_last_adjoint_ms(pf_batch *b, double *forward_ms, double *backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,3 +104,27 @@ class RbpfBatch:
         check(self.lib.pf_batch_autotune(self.h, count_ptr, cand_ptr, uni_ptr, lml_ptr, int(frames), C.byref(sc),
                                          C.byref(best), C.byref(dflt)))
         return sc.as_tuple(), best.value, dflt.value
+
+    # ------------------------------------------------------------- adjoint
+    def lml_grad(self, count_ptr, cand_ptr, uni_ptr, lml_ptr, checkpoint_every: int = 50):
+        """Log-marginal-likelihood gradient of every sequence (the adjoint of
+        the run, ``pf_batch_lml_grad``).  Returns a dict of FP64 arrays:
+        means (S, K, d), logits (S, K), rates (S, K), chol (S, K, d, d)."""
+        S, K, d = self.S, self.K, self.d
+        g = dict(means=np.zeros((S, K, d)), logits=np.zeros((S, K)), rates=np.zeros((S, K)),
+                 chol=np.zeros((S, K, d, d)))
+        check(self.lib.pf_batch_lml_grad(self.h, count_ptr, cand_ptr, uni_ptr, lml_ptr, int(checkpoint_every),
+                                         ptr(g["means"]), ptr(g["logits"]), ptr(g["rates"]), ptr(g["chol"])))
+        return g
+
+    def get_choices(self, s):
+        """(comp, anc), each n_frames[s] x n_particles[s], of the last lml_grad run."""
+        T, N = int(self.T[s]), int(self.N[s])
+        comp = np.empty((T, N), np.int32); anc = np.empty((T, N), np.int32)
+        check(self.lib.pf_batch_get_choices(self.h, int(s), ptr(comp), ptr(anc)))
+        return comp, anc
+
+    def last_adjoint_ms(self):
+        f, b = C.c_double(0.0), C.c_double(0.0)
+        check(self.lib.pf_batch_last_adjoint_ms(self.h, C.byref(f), C.byref(b)))
+        return f.value, b.value

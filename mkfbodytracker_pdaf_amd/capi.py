@@ -34,6 +34,7 @@ EXPORTS = [
     "pf_batch_create", "pf_batch_destroy", "pf_batch_load_model", "pf_batch_set_joint_clutter",
     "pf_batch_reset", "pf_batch_run_dev", "pf_batch_get_tracks", "pf_batch_estimate",
     "pf_batch_get_schedule", "pf_batch_set_schedule", "pf_batch_autotune", "pf_batch_last_run_ms",
+    "pf_batch_lml_grad", "pf_batch_get_choices", "pf_batch_last_adjoint_ms",
 ]
 
 
@@ -124,6 +125,9 @@ def lib():
     _sig(L, "pf_batch_autotune", C.c_int, _vp, _vp, _vp, _vp, _vp, i32, C.POINTER(Schedule), C.POINTER(dbl),
          C.POINTER(dbl))
     _sig(L, "pf_batch_last_run_ms", C.c_int, _vp, C.POINTER(dbl))
+    _sig(L, "pf_batch_lml_grad", C.c_int, _vp, _vp, _vp, _vp, _vp, i32, _vp, _vp, _vp, _vp)
+    _sig(L, "pf_batch_get_choices", C.c_int, _vp, i32, _vp, _vp)
+    _sig(L, "pf_batch_last_adjoint_ms", C.c_int, _vp, C.POINTER(dbl), C.POINTER(dbl))
     _lib = L
     return L
 

@@ -24,76 +24,17 @@
 #include "pf_internal.hpp"
 #include "pf_kernels.hpp"
 #include "pf_model.hpp"
+#include "pf_batch_impl.hpp"
 
 using namespace mkfbt;
-
-#define HIP_TRYB(expr)                                                                         \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            return set_error(PF_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
-    } while (0)
+using namespace mkfbt::batch;
 
 namespace {
 
-constexpr int kBD = 12;           // state dimension of the batched engine
-constexpr int kSeqAlign = 512;    // particle offset alignment (largest tile)
-
-struct SeqDesc {
-    int pofs, N, T, tofs, bofs, nb, table_ofs, pad;
-    unsigned long long pi_total;
-};
-
-struct BatchArgs {
-    float *buf_x[2], *buf_P[2];
-    int *anc;
-    float *lw;
-    unsigned long long *maxkey;   // 2 x S (parity-major)
-    unsigned int *status;         // S
-    uint32_t *Upend;              // S
-    unsigned long long *cloc, *bsum;
-    double *bexp;
-    const float *tables;
-    const SeqDesc *seqs;
-    const int *tile_seq, *tile_j0, *n_tiles;   // tiles of active sequences first
-    const int *wb_seq, *wb_b, *n_wblocks;      // resampler blocks likewise
-    const int *frame;
-    int *frame_next;
-    const uint32_t *uni;
-    const int *count;
-    const float *cand;
-    double *lml;
-    int S, ld, K, m, J, cap, table_floats, parity, fused, nb_max, all_tiles;
-    unsigned int order_packed, clutter_mask;
-    float pdl, b0, gate;
-};
-
-// Per-sequence view of the arguments in the single-filter layout, so the
-// per-particle update is shared with k_step.
-__device__ __forceinline__ StepArgs seq_view(const BatchArgs &a, const SeqDesc &sd)
-{
-    StepArgs v{};
-    v.count = a.count + (size_t)sd.tofs * a.J;
-    v.cand = a.cand + (size_t)sd.tofs * a.J * a.cap * 2;
-    v.pi_total = sd.pi_total;
-    v.K = a.K;
-    v.N = sd.N;
-    v.m = a.m;
-    v.J = a.J;
-    v.cap = a.cap;
-    v.order_packed = a.order_packed;
-    v.clutter_mask = a.clutter_mask;
-    v.pdl = a.pdl;
-    v.b0 = a.b0;
-    v.gate = a.gate;
-    v.comp_out = nullptr;
-    return v;
-}
-
 template <int PPL>
 __device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc &sd, int s, int t, int jt0,
-                                               unsigned long long *wsp, int (&src)[PPL], bool write_anc,
-                                               int stats_parity, double *lml_slot)
+                                               unsigned long long *wsp, int (&src)[PPL], int *anc_out,
+                                               int *anc_out2, int stats_parity, double *lml_slot)
 {
     const int bd = blockDim.x;
     Pending pd;
@@ -112,7 +53,10 @@ __device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc
         if (jt0 + r * bd >= sd.N) continue;        // block-uniform: no barrier is skipped by part of the block
         src[r] = resolve_search(pd, sd.N, U, total, jt0 + r * bd, wsp);
         const int j = jt0 + r * bd + (int)threadIdx.x;
-        if (write_anc && j < sd.N) a.anc[sd.pofs + j] = src[r];
+        if (j < sd.N) {
+            if (anc_out) anc_out[sd.pofs + j] = src[r];
+            if (anc_out2) anc_out2[sd.pofs + j] = src[r];
+        }
     }
     if (first && threadIdx.x == 0) {
         unsigned long long *mk = a.maxkey + (size_t)stats_parity * a.S + s;
@@ -142,7 +86,8 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
         const SeqDesc sd = a.seqs[s];
         int src[PPL];
         if (a.fused) {
-            tile_ancestors<PPL>(a, sd, s, t, jt0, wsp, src, false, a.parity ^ 1, a.lml + sd.tofs + (t - 1));
+            tile_ancestors<PPL>(a, sd, s, t, jt0, wsp, src, a.rec_anc ? a.rec_anc + (size_t)(t - 1) * a.ld : nullptr,
+                                nullptr, a.parity ^ 1, a.lml + sd.tofs + (t - 1));
         } else {
 #pragma unroll
             for (int r = 0; r < PPL; ++r) {
@@ -167,7 +112,8 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
         __syncthreads();          // the previous tile's LDS readers are done
         stage_table(sm4, reinterpret_cast<const float4 *>(a.tables + (size_t)sd.table_ofs), a.table_floats >> 2);
         __syncthreads();
-        const StepArgs v = seq_view(a, sd);
+        StepArgs v = seq_view(a, sd);
+        if (a.rec_comp) v.comp_out = a.rec_comp + (size_t)t * a.ld + sd.pofs;
         const uint32_t U1 = a.uni[(size_t)(sd.tofs + t) * 2];
         float lmax = -INFINITY;
         const __amdgpu_buffer_rsrc_t wx = soa_rsrc(xout, rowb * D), wP = soa_rsrc(Pout, rowb * NP);
@@ -255,7 +201,9 @@ __global__ void __launch_bounds__(256) k_batch_finish(BatchArgs a)
         const int jt0 = a.tile_j0[tile];
         const SeqDesc sd = a.seqs[s];
         int src[PPL];
-        tile_ancestors<PPL>(a, sd, s, sd.T - 1, jt0, wsp, src, true, (sd.T - 1) & 1, a.lml + sd.tofs + sd.T - 1);
+        tile_ancestors<PPL>(a, sd, s, sd.T - 1, jt0, wsp, src, a.anc,
+                            a.rec_anc ? a.rec_anc + (size_t)(sd.T - 1) * a.ld : nullptr, (sd.T - 1) & 1,
+                            a.lml + sd.tofs + sd.T - 1);
     }
 }
 
@@ -274,44 +222,11 @@ __global__ void k_identity_seq(int *anc, int pofs, int N)
     if (j < N) anc[pofs + j] = j;
 }
 
-template <typename T>
-hipError_t dallocb(T **p, size_t n)
-{
-    return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1));
-}
 
 }  // namespace
 
-struct pf_batch {
-    int device = 0, S = 0, K = 0, D = 0, m = 0, J = 0, cap = 16, cus = 256;
-    double pd = 0.95, gate = 9.210340371976184, lambda = 3.0 / (640.0 * 480.0);
-    std::vector<int> N, T, order, rank;      // order[i] = user sequence of internal slot i; rank = inverse
-    std::vector<int> pofs, tofs_user, tofs, bofs, nb;
-    int ld = 0, Tmax = 0, nb_max = 0;
-    std::vector<HostModel> hm;
-    std::vector<char> loaded, sel_ok, fresh;   // fresh[s]: reset since the last run
-    unsigned clutter_mask = 0, order_packed = 0;
-    int table_floats = 0;
-    bool tracks = false;
-    pf_batch_schedule sch{256, 1, 2};
-    hipStream_t stream = nullptr;
-    float *bx[2] = {nullptr, nullptr}, *bP[2] = {nullptr, nullptr};
-    float *lw = nullptr, *tables = nullptr, *mus = nullptr, *sigs = nullptr;
-    double *Tis = nullptr, *Ts = nullptr, *bexp = nullptr, *partial = nullptr, *est = nullptr, *gbuf = nullptr;
-    size_t gbuf_bytes = 0;
-    int *anc = nullptr, *frame = nullptr, *bins = nullptr;
-    unsigned long long *maxkey = nullptr, *cloc = nullptr, *bsum = nullptr;
-    unsigned int *status = nullptr;
-    uint32_t *Upend = nullptr;
-    SeqDesc *seqs = nullptr;
-    std::vector<SeqDesc> h_seqs;
-    int *tile_seq = nullptr, *tile_j0 = nullptr, *n_tiles = nullptr, all_tiles = 0;
-    int *wb_seq = nullptr, *wb_b = nullptr, *n_wblocks = nullptr;
-    FrameScratch *fs = nullptr;   // for the per-sequence estimate reduction
-    float last_ms = 0.f;
-};
-
-namespace {
+namespace mkfbt {
+namespace batch {
 
 int bcheck(pf_batch *b)
 {
@@ -343,6 +258,7 @@ int build_tiles(pf_batch *b)
         nw[t] = first_wb[act];
     }
     b->all_tiles = (int)ts.size();
+    b->h_ntiles = nt;
     (void)hipFree(b->tile_seq); (void)hipFree(b->tile_j0); (void)hipFree(b->n_tiles);
     (void)hipFree(b->wb_seq); (void)hipFree(b->wb_b); (void)hipFree(b->n_wblocks);
     HIP_TRYB(dallocb(&b->tile_seq, ts.size()));
@@ -398,6 +314,7 @@ int build_batch_tables(pf_batch *b)
     HIP_TRYB(hipMemcpy(b->sigs, sgs.data(), sizeof(float) * sgs.size(), hipMemcpyHostToDevice));
     HIP_TRYB(hipMemcpy(b->Ts, Ts.data(), sizeof(double) * Ts.size(), hipMemcpyHostToDevice));
     HIP_TRYB(hipMemcpy(b->Tis, Tis.data(), sizeof(double) * Tis.size(), hipMemcpyHostToDevice));
+    b->h_T = Ts;
     HIP_TRYB(hipMemcpy(b->seqs, b->h_seqs.data(), sizeof(SeqDesc) * b->S, hipMemcpyHostToDevice));
     return PF_OK;
 }
@@ -430,6 +347,8 @@ BatchArgs batch_args(pf_batch *b, const int32_t *count, const float *cand, const
     a.count = count;
     a.cand = cand;
     a.lml = lml;
+    a.rec_comp = nullptr;
+    a.rec_anc = nullptr;
     a.S = b->S;
     a.ld = b->ld;
     a.K = b->K;
@@ -503,14 +422,27 @@ int launch_batch_finish(pf_batch *b, const BatchArgs &a)
 }
 
 int run_all(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uni, double *lml, int frames,
-            bool graph)
+            bool graph, bool record)
 {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, b->stream, b->frame, 0);
     hipLaunchKernelGGL(k_batch_clear, dim3((2 * b->S + 255) / 256), dim3(256), 0, b->stream, b->maxkey, b->status,
                        b->S);
+    if (record) graph = false;
+    const size_t xw = (size_t)b->D * b->ld, Pw = (size_t)npacked(b->D) * b->ld;
     auto one = [&](int f) -> int {
         const int parity = f & 1;
         BatchArgs a = batch_args(b, count, cand, uni, lml, parity, f > 0);
+        if (record) {
+            a.rec_comp = b->rec_comp;
+            a.rec_anc = b->rec_anc;
+            if (f % b->ckpt_every == 0) {          // the input state of frame f
+                const size_t c = (size_t)(f / b->ckpt_every);
+                HIP_TRYB(hipMemcpyAsync(b->ck_x + c * xw, b->bx[f & 1], sizeof(float) * xw, hipMemcpyDeviceToDevice,
+                                        b->stream));
+                HIP_TRYB(hipMemcpyAsync(b->ck_P + c * Pw, b->bP[f & 1], sizeof(float) * Pw, hipMemcpyDeviceToDevice,
+                                        b->stream));
+            }
+        }
         int st = launch_batch_step(b, a);
         if (st) return st;
         return launch_batch_weights(b, a);
@@ -545,7 +477,10 @@ int run_all(pf_batch *b, const int32_t *count, const float *cand, const uint32_t
     return PF_OK;
 }
 
-}  // namespace
+}  // namespace batch
+}  // namespace mkfbt
+
+using namespace mkfbt::batch;
 
 extern "C" {
 
@@ -555,9 +490,11 @@ int pf_batch_destroy(pf_batch *b)
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     void *ptrs[] = {b->bx[0], b->bx[1], b->bP[0], b->bP[1], b->lw, b->tables, b->mus, b->sigs, b->Tis, b->Ts,
-                    b->bexp, b->partial, b->est, b->gbuf, b->anc, b->frame, b->bins, b->maxkey, b->cloc, b->bsum,
+                    b->bexp, b->partial, b->est, b->gbuf, b->anc, b->frame, b->maxkey, b->cloc, b->bsum,
                     b->status, b->Upend, b->seqs, b->tile_seq, b->tile_j0, b->n_tiles, b->wb_seq, b->wb_b,
-                    b->n_wblocks, b->fs};
+                    b->n_wblocks, b->fs, b->rec_comp, b->rec_anc, b->rec_bins, b->ck_x, b->ck_P, b->seg_x,
+                    b->seg_P, b->adj_x[0], b->adj_x[1], b->adj_P[0], b->adj_P[1], b->first, b->tacc, b->gsum,
+                    b->t256_seq, b->t256_j0, b->t256_first};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -640,7 +577,7 @@ int pf_batch_create(pf_batch **out, int32_t S, const int32_t *n_particles, const
     e = e ? e : dallocb(&b->Upend, (size_t)S);
     e = e ? e : dallocb(&b->seqs, (size_t)S);
     e = e ? e : dallocb(&b->frame, 2);
-    e = e ? e : dallocb(&b->bins, (size_t)kMaxScanBlocks * kWeightsBlock);
+    e = e ? e : dallocb(&b->rec_bins, (size_t)b->ld);
     e = e ? e : dallocb(&b->partial, (size_t)kMaxScanBlocks * 64);
     e = e ? e : dallocb(&b->est, 64);
     e = e ? e : dallocb(&b->fs, 1);
@@ -701,8 +638,8 @@ int pf_batch_reset(pf_batch *b, int32_t s, const int32_t *bins)
     const SeqDesc &sd = b->h_seqs[i];
     for (int j = 0; j < sd.N; ++j)
         if (bins[j] < 0 || bins[j] >= b->K) return set_error(PF_ERR_ARG, "bins[%d] out of range", j);
-    HIP_TRYB(hipMemcpyAsync(b->bins, bins, sizeof(int) * sd.N, hipMemcpyHostToDevice, b->stream));
-    hipLaunchKernelGGL((k_reset<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->bins,
+    HIP_TRYB(hipMemcpyAsync(b->rec_bins + sd.pofs, bins, sizeof(int) * sd.N, hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL((k_reset<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->rec_bins + sd.pofs,
                        b->mus + (size_t)i * b->K * b->D, b->sigs + (size_t)i * b->K * npacked(b->D),
                        b->bx[0] + sd.pofs, b->bP[0] + sd.pofs, sd.N, b->ld);
     hipLaunchKernelGGL(k_identity_seq, dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->anc, sd.pofs, sd.N);

@@ -1,0 +1,152 @@
+// pf_batch_impl.hpp -- internals of the batched engine (pf_batch.hip) shared
+// with its adjoint (pf_adjoint.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/pf_batch.h"
+#include "pf_internal.hpp"
+#include "pf_kernels.hpp"
+#include "pf_model.hpp"
+
+#define HIP_TRYB(expr)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return mkfbt::set_error(PF_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace mkfbt {
+namespace batch {
+
+constexpr int kBD = 12;           // state dimension of the batched engine
+constexpr int kSeqAlign = 512;    // particle offset alignment (largest tile)
+
+struct SeqDesc {
+    int pofs, N, T, tofs, bofs, nb, table_ofs, pad;
+    unsigned long long pi_total;
+};
+
+struct BatchArgs {
+    float *buf_x[2], *buf_P[2];
+    int *anc;
+    float *lw;
+    unsigned long long *maxkey;   // 2 x S (parity-major)
+    unsigned int *status;         // S
+    uint32_t *Upend;              // S
+    unsigned long long *cloc, *bsum;
+    double *bexp;
+    const float *tables;
+    const SeqDesc *seqs;
+    const int *tile_seq, *tile_j0, *n_tiles;   // tiles of active sequences first
+    const int *wb_seq, *wb_b, *n_wblocks;      // resampler blocks likewise
+    const int *frame;
+    int *frame_next;
+    const uint32_t *uni;
+    const int *count;
+    const float *cand;
+    double *lml;
+    int *rec_comp;                // optional record: components of frame t at rec_comp[t * ld + column]
+    int *rec_anc;                 // optional record: ancestors of frame t's resampling, same layout
+    int S, ld, K, m, J, cap, table_floats, parity, fused, nb_max, all_tiles;
+    unsigned int order_packed, clutter_mask;
+    float pdl, b0, gate;
+};
+
+// Per-sequence view of the arguments in the single-filter layout, so the
+// per-particle update is shared with k_step.
+__device__ __forceinline__ StepArgs seq_view(const BatchArgs &a, const SeqDesc &sd)
+{
+    StepArgs v{};
+    v.count = a.count + (size_t)sd.tofs * a.J;
+    v.cand = a.cand + (size_t)sd.tofs * a.J * a.cap * 2;
+    v.pi_total = sd.pi_total;
+    v.K = a.K;
+    v.N = sd.N;
+    v.m = a.m;
+    v.J = a.J;
+    v.cap = a.cap;
+    v.order_packed = a.order_packed;
+    v.clutter_mask = a.clutter_mask;
+    v.pdl = a.pdl;
+    v.b0 = a.b0;
+    v.gate = a.gate;
+    v.comp_out = nullptr;
+    return v;
+}
+
+
+template <typename T>
+hipError_t dallocb(T **p, size_t n)
+{
+    return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1));
+}
+
+}  // namespace batch
+}  // namespace mkfbt
+
+struct pf_batch {
+    int device = 0, S = 0, K = 0, D = 0, m = 0, J = 0, cap = 16, cus = 256;
+    double pd = 0.95, gate = 9.210340371976184, lambda = 3.0 / (640.0 * 480.0);
+    std::vector<int> N, T, order, rank;      // order[i] = user sequence of internal slot i; rank = inverse
+    std::vector<int> pofs, tofs_user, tofs, bofs, nb;
+    int ld = 0, Tmax = 0, nb_max = 0;
+    std::vector<mkfbt::HostModel> hm;
+    std::vector<char> loaded, sel_ok, fresh;   // fresh[s]: reset since the last run
+    unsigned clutter_mask = 0, order_packed = 0;
+    int table_floats = 0;
+    bool tracks = false;
+    pf_batch_schedule sch{256, 1, 2};
+    hipStream_t stream = nullptr;
+    float *bx[2] = {nullptr, nullptr}, *bP[2] = {nullptr, nullptr};
+    float *lw = nullptr, *tables = nullptr, *mus = nullptr, *sigs = nullptr;
+    double *Tis = nullptr, *Ts = nullptr, *bexp = nullptr, *partial = nullptr, *est = nullptr, *gbuf = nullptr;
+    size_t gbuf_bytes = 0;
+    int *anc = nullptr, *frame = nullptr;
+    unsigned long long *maxkey = nullptr, *cloc = nullptr, *bsum = nullptr;
+    unsigned int *status = nullptr;
+    uint32_t *Upend = nullptr;
+    mkfbt::batch::SeqDesc *seqs = nullptr;
+    std::vector<mkfbt::batch::SeqDesc> h_seqs;
+    int *tile_seq = nullptr, *tile_j0 = nullptr, *n_tiles = nullptr, all_tiles = 0;
+    int *wb_seq = nullptr, *wb_b = nullptr, *n_wblocks = nullptr;
+    mkfbt::FrameScratch *fs = nullptr;   // for the per-sequence estimate reduction
+    float last_ms = 0.f;
+    std::vector<int> h_ntiles;    // active tiles per frame (current schedule)
+    // adjoint (pf_batch_lml_grad): records and checkpoints of the last recorded run
+    int *rec_comp = nullptr, *rec_anc = nullptr, *rec_bins = nullptr;
+    int rec_frames = 0, ckpt_every = 0, n_ckpt = 0, seg_slots = 0;
+    float *ck_x = nullptr, *ck_P = nullptr, *seg_x = nullptr, *seg_P = nullptr;
+    float *adj_x[2] = {nullptr, nullptr}, *adj_P[2] = {nullptr, nullptr};
+    int *first = nullptr;
+    double *tacc = nullptr, *gsum = nullptr;
+    int *t256_seq = nullptr, *t256_j0 = nullptr, *t256_first = nullptr;
+    std::vector<int> h_t256_count;
+    int t256_total = 0;
+    bool recorded = false;
+    double last_fwd_ms = 0.0, last_bwd_ms = 0.0;
+    std::vector<size_t> adj_have = std::vector<size_t>(24, 0);   // element capacities of the buffers above
+    std::vector<double> h_T;      // state bases per sequence (internal order), D x D each
+};
+
+
+namespace mkfbt {
+namespace batch {
+
+int bcheck(pf_batch *b);
+int build_tiles(pf_batch *b);
+BatchArgs batch_args(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uni, double *lml,
+                     int parity, int fused);
+int launch_batch_step(pf_batch *b, const BatchArgs &a);
+int launch_batch_weights(pf_batch *b, const BatchArgs &a);
+int launch_batch_finish(pf_batch *b, const BatchArgs &a);
+// Frames [0, frames) of every sequence from its reset state.  ckpt: copy the
+// input state of every frame t with t % ckpt_every == 0 into checkpoint
+// t / ckpt_every (no graph capture then).
+int run_all(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uni, double *lml, int frames,
+            bool graph, bool record = false);
+
+}  // namespace batch
+}  // namespace mkfbt

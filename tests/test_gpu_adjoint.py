@@ -1,0 +1,145 @@
+"""GPU adjoint of the log-marginal-likelihood (pf_batch_lml_grad) against the
+FP64 oracle gradient of the same recorded run.
+
+The oracle differentiates, by forward-mode dual numbers in FP64, the
+log-marginal-likelihood of the GPU run's own recorded choices (component
+draws, ancestors); that reference is itself pinned by central finite
+differences (tests/test_adjoint_oracle.py).
+
+Tolerances: per parameter group (means, logits, rates, Cholesky factors) and
+sequence, ||g_gpu - g_oracle|| <= 1e-3 ||g_oracle||; per-frame log mean
+weights within 1e-4 max(1, |lml|).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from mkfbodytracker_pdaf_amd import synthetic
+from mkfbodytracker_pdaf_amd.batch import RbpfBatch
+
+pytestmark = pytest.mark.gpu
+
+CAP = 8
+K = 3
+PD, GATE = 0.95, 9.210340371976184
+LAM = 3.0 / (640 * 480)
+NS = [96, 160, 64]
+TS = [6, 9, 4]
+
+
+def _problem():
+    models, seqs, bins = [], [], []
+    for s, (N, T) in enumerate(zip(NS, TS)):
+        mdl = synthetic.make_model(seed=40 + s, K=K)
+        models.append(mdl)
+        seqs.append(synthetic.simulate(mdl, T, seed=60 + s, cand_per_joint=4, cap=CAP))
+        bins.append(oracle.resample_linear(mdl.weights, N, 500 + s)[0])
+    return models, seqs, bins
+
+
+def _inputs(seqs):
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    cnt = np.concatenate([q.count for q in seqs]).astype(np.int32)
+    cand = np.concatenate([q.cand for q in seqs]).astype(np.float32)
+    uni = np.concatenate([q.uniforms for q in seqs]).astype(np.uint32)
+    return DeviceArray(cnt), DeviceArray(cand), DeviceArray(uni), DeviceArray.empty(cnt.shape[0], np.float64)
+
+
+def _batch(models, bins):
+    b = RbpfBatch(NS, TS, K=K, m=6, cand_capacity=CAP, clutter_density=LAM)
+    for s, mdl in enumerate(models):
+        b.load_model(s, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
+    b.set_joint_clutter(models[0].clutter)
+    for s in range(len(NS)):
+        b.reset(s, bins[s])
+    return b
+
+
+@pytest.fixture(scope="module")
+def run():
+    models, seqs, bins = _problem()
+    b = _batch(models, bins)
+    d = _inputs(seqs)
+    g = b.lml_grad(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, checkpoint_every=4)
+    lml = d[3].get()
+    choices = [b.get_choices(s) for s in range(len(NS))]
+    b.close()
+    return models, seqs, bins, g, lml, choices
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_adjoint_matches_oracle_gradient(gpu, run):
+    models, seqs, bins, g, lml, choices = run
+    offs = np.concatenate([[0], np.cumsum(TS)[:-1]])
+    for s, mdl in enumerate(models):
+        comp, anc = choices[s]
+        prob = oracle.AdjProblem(mdl.H(), mdl.BH(), mdl.R, mdl.clutter, seqs[s].count, seqs[s].cand, bins[s],
+                                 comp, anc, PD, GATE, LAM, K=K)
+        theta = oracle.AdjProblem.theta(mdl.means, mdl.weights, mdl.gamma, mdl.covs)
+        F, go = prob.grad(theta)
+        _, lml_o = prob.value(theta)
+        lg = lml[offs[s]:offs[s] + TS[s]]
+        assert np.all(np.abs(lg - lml_o) <= 1e-4 * np.maximum(1.0, np.abs(lml_o))), (s, lg, lml_o)
+        gm, gl, gr, gc = prob.split(go)
+        for name, gpu_g, ref in (("means", g["means"][s], gm), ("logits", g["logits"][s], gl),
+                                 ("rates", g["rates"][s], gr), ("chol", g["chol"][s], gc)):
+            assert _rel(gpu_g, ref) <= 1e-3, (s, name, _rel(gpu_g, ref))
+
+
+def test_adjoint_forward_is_the_plain_run(gpu, run):
+    """The recorded forward of the adjoint gives bit-identical log mean
+    weights to pf_batch_run_dev on the same inputs."""
+    models, seqs, bins, g, lml, choices = run
+    b = _batch(models, bins)
+    d = _inputs(seqs)
+    b.run_dev(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr)
+    assert np.array_equal(d[3].get(), lml)
+    b.close()
+
+
+@pytest.mark.parametrize("C", [1, 3, 100])
+def test_adjoint_independent_of_checkpoint_interval(gpu, run, C):
+    models, seqs, bins, g, lml, choices = run
+    b = _batch(models, bins)
+    d = _inputs(seqs)
+    g2 = b.lml_grad(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, checkpoint_every=C)
+    for k in g:
+        assert np.array_equal(g[k], g2[k]), (C, k)
+    fwd, bwd = b.last_adjoint_ms()
+    assert fwd > 0 and bwd > 0
+    b.close()
+
+
+def test_torch_function_backward(gpu, run):
+    torch = pytest.importorskip("torch")
+    from mkfbodytracker_pdaf_amd.torch_adjoint import BatchProblem, lml as lml_fn
+    models, seqs, bins, g, lml, choices = run
+    m0 = models[0]
+    cnt = np.concatenate([q.count for q in seqs])
+    cand = np.concatenate([q.cand for q in seqs])
+    uni = np.concatenate([q.uniforms for q in seqs])
+    prob = BatchProblem(NS, TS, m0.pca_proj, m0.pca_mean, m0.sel, m0.R, m0.clutter, cnt, cand, uni, bins,
+                        checkpoint_every=4, cand_capacity=CAP, clutter_density=LAM)
+    Ls = np.linalg.cholesky(m0.Sigma)
+    params = [torch.tensor(v, dtype=torch.float64, requires_grad=True)
+              for v in (m0.means, np.log(m0.weights), 1.0 - m0.gamma, Ls)]
+    out = lml_fn(*params, prob)
+    out.backward()
+    # the same problem through the C ABI directly: every sequence shares model 0
+    b = RbpfBatch(NS, TS, K=K, m=6, cand_capacity=CAP, clutter_density=LAM)
+    for s in range(len(NS)):
+        b.load_model(s, m0.means, m0.covs, m0.weights, m0.gamma, m0.pca_proj, m0.pca_mean, m0.sel, m0.R)
+    b.set_joint_clutter(m0.clutter)
+    for s in range(len(NS)):
+        b.reset(s, bins[s])
+    d = _inputs(seqs)
+    gd = b.lml_grad(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, checkpoint_every=4)
+    assert abs(out.item() - d[3].get().sum()) < 1e-7 * max(1.0, abs(out.item()))
+    # the torch path rebuilds weights and covariances from logits and factors
+    # (exp / log, L L^T), so its model differs from model 0 by rounding only
+    for p, k in zip(params, ("means", "logits", "rates", "chol")):
+        assert _rel(p.grad.numpy(), gd[k].sum(axis=0)) <= 1e-6, k
+    b.close()
