@@ -14,6 +14,9 @@ bounded sample of the same workload, on this host).
 ``--workload batched`` instead runs many ragged sequences (lengths
 U{100..2000} frames, N log-uniform in [1k, 64k], own GMM each) through the
 batched engine and reports the default and the autotuned schedule.
+``--workload adjoint`` times the gradient of the log-marginal-likelihood of
+64 sequences x 500 frames x 16k particles (checkpoint every 50 frames)
+against the forward pass.
 """
 from __future__ import annotations
 
@@ -51,17 +54,21 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="step kernel block size (0 = default)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this size (PMC calibration)")
-    ap.add_argument("--workload", choices=("single", "batched"), default="single",
+    ap.add_argument("--workload", choices=("single", "batched", "adjoint"), default="single",
                     help="single: one sequence of N particles (the headline line); batched: many ragged "
                          "sequences in one persistent step per frame, tuned vs default schedule")
     ap.add_argument("--sequences", type=int, default=256, help="batched: number of sequences")
     ap.add_argument("--tune-frames", type=int, default=32, help="batched: frames timed per candidate schedule")
+    ap.add_argument("--adj-shape", type=str, default="64,16384,500",
+                    help="adjoint: sequences,particles,frames (the measured workload is 64,16384,500)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "adjoint":
+        return run_adjoint(args)
 
     from mkfbodytracker_pdaf_amd import synthetic
     from mkfbodytracker_pdaf_amd.devmem import DeviceArray, synchronize
@@ -270,6 +277,88 @@ def run_batched(args):
         out["cpu_baseline"] = cpu_baseline(models[s0], seqs[s0], int(Ns[s0]), cap, lam, args.cpu_seconds,
                                            seed=seed * 1000 + s0)
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+def run_adjoint(args):
+    """Gradient of the summed log-marginal-likelihood of 64 sequences x 500
+    frames x N = 16,384 particles (K=15, d=12, own GMM per sequence) w.r.t.
+    means, logit weights, reversion rates and Cholesky factors, checkpointing
+    every 50 frames; timed against the plain (hipGraph) forward pass."""
+    from mkfbodytracker_pdaf_amd import synthetic
+    from mkfbodytracker_pdaf_amd.batch import RbpfBatch
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray, synchronize
+    from mkfbodytracker_pdaf_amd.rbpf import Rbpf
+
+    S, N, T = (int(v) for v in args.adj_shape.split(","))
+    seed, C, cap, cpj = 11, 50, 16, 8
+    lam = (cpj - 1) / (640.0 * 480.0)
+    models = [synthetic.make_model(seed=seed * 1000 + s) for s in range(S)]
+    seqs = [synthetic.simulate_fast(models[s], T, seed=seed * 1000 + s, cand_per_joint=cpj, cap=cap)
+            for s in range(S)]
+    b = RbpfBatch([N] * S, [T] * S, K=models[0].K, m=models[0].m, cand_capacity=cap, clutter_density=lam)
+    for s, mdl in enumerate(models):
+        b.load_model(s, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
+    b.set_joint_clutter(models[0].clutter)
+    helper = Rbpf(N)
+    bins = [synthetic.init_bins(models[s], N, seed * 1000 + s, lambda w, n, U: helper.resample(w, n, U)[0])
+            for s in range(S)]
+    helper.close()
+
+    def reset_all():
+        for s in range(S):
+            b.reset(s, bins[s])
+
+    d_cnt = DeviceArray(np.concatenate([q.count for q in seqs]).astype(np.int32))
+    d_cand = DeviceArray(np.concatenate([q.cand for q in seqs]).astype(np.float32))
+    d_uni = DeviceArray(np.concatenate([q.uniforms for q in seqs]).astype(np.uint32))
+    d_lml = DeviceArray.empty(S * T, np.float64)
+    # plain forward (warm-up, then timed)
+    for rep in range(2):
+        reset_all()
+        synchronize()
+        t0 = time.perf_counter()
+        b.run_dev(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, graph=True)
+        synchronize()
+        t_fwd = time.perf_counter() - t0
+    lml_fwd = d_lml.get()
+    # gradient (warm-up allocates the records, then timed)
+    for rep in range(2):
+        reset_all()
+        synchronize()
+        t0 = time.perf_counter()
+        g = b.lml_grad(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, checkpoint_every=C)
+        synchronize()
+        t_grad = time.perf_counter() - t0
+    rec_fwd_ms, bwd_ms = b.last_adjoint_ms()
+    lml_grad = d_lml.get()
+    evals = float(S) * N * T
+    out = {
+        "metric": "adjoint gradient time / forward time (log-marginal-likelihood, batched)",
+        "value": t_grad / t_fwd,
+        "unit": "x forward",
+        "n_gpus": 1,
+        "steps": T,
+        "warmup": 1,
+        "ms_per_step": t_grad / T * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (per-sequence seeded GMM K=15 d=12, pinhole-projected 3-joint pose, Philox uniforms)",
+        "config": {"workload": "adjoint: %d sequences x %d frames x N=%d, K=15, d=12, m=6, 8 candidates per hand, "
+                               "checkpoint every %d frames" % (S, T, N, C),
+                   "sequences": S, "frames": T, "particles": N, "checkpoint_every": C},
+        "forward_s": t_fwd,
+        "forward_evals_per_s": evals / t_fwd,
+        "gradient_s": t_grad,
+        "gradient_recorded_forward_ms": rec_fwd_ms,
+        "gradient_backward_ms": bwd_ms,
+        "backward_over_recorded_forward": bwd_ms / rec_fwd_ms,
+        "lml_total": float(np.sum(lml_fwd)),
+        "lml_identical_in_gradient_run": bool(np.array_equal(lml_fwd, lml_grad)),
+        "grad_norms": {k: float(np.linalg.norm(v)) for k, v in g.items()},
+    }
     print(json.dumps(out))
 
 

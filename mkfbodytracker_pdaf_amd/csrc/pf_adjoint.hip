@@ -308,36 +308,46 @@ __global__ void __launch_bounds__(256) k_adj_replay(AdjArgs a)
     const int s = a.t_seq[blockIdx.x];
     const int j0 = a.t_j0[blockIdx.x];
     const SeqDesc sd = ba.seqs[s];
+    const StepArgs v = seq_view(ba, sd);
+    const float *fcand = v.cand + (size_t)a.t * ba.J * ba.cap * 2;
+    CandPrefetch cpf;
+    cand_prefetch(cpf, v.count + (size_t)a.t * ba.J, fcand, ba.J, ba.cap);
     stage_table(sm4, reinterpret_cast<const float4 *>(ba.tables + (size_t)sd.table_ofs), ba.table_floats >> 2);
+    cand_store(cpf, reinterpret_cast<float *>(sm4) + ba.table_floats, fcand, ba.J, ba.cap);
     __syncthreads();
     const int j = j0 + (int)threadIdx.x;
     if (j >= sd.N || a.t >= sd.T) return;
     const int src = a.anc_prev ? a.anc_prev[sd.pofs + j] : j;
     float x[D], P[NP];
     load_state(a.in_x, a.in_P, ba.ld, sd.pofs + src, x, P);
-    const StepArgs v = seq_view(ba, sd);
     const uint32_t U1 = ba.uni[(size_t)(sd.tofs + a.t) * 2];
     (void)particle_update<D, kModeCand, SELJ>(v, a.t, j, U1, reinterpret_cast<const float *>(sm4), x, P);
     store_state(a.out_x, a.out_P, ba.ld, sd.pofs + j, x, P);
 }
 
 // Children ranges of frame t's resampling: first[f + i] = first child of
-// parent i (ancestors are non-decreasing), first[f + N] = N.
+// parent i = lower_bound(anc, i) (ancestors are non-decreasing), i in [0, N].
 __global__ void __launch_bounds__(256) k_adj_first(AdjArgs a)
 {
     const BatchArgs &ba = a.b;
     const int s = a.t_seq[blockIdx.x];
     const int j0 = a.t_j0[blockIdx.x];
     const SeqDesc sd = ba.seqs[s];
-    const int c = j0 + (int)threadIdx.x;
-    if (c >= sd.N || a.t + 1 >= sd.T) return;
-    const int f = sd.pofs + s;
+    const int i = j0 + (int)threadIdx.x;
+    if (a.t + 1 >= sd.T) return;
     const int *anc = a.anc_t + sd.pofs;
-    const int an = anc[c];
-    const int prev = (c == 0) ? -1 : anc[c - 1];
-    for (int i = prev + 1; i <= an; ++i) a.first[f + i] = c;
-    if (c == sd.N - 1)
-        for (int i = an + 1; i <= sd.N; ++i) a.first[f + i] = sd.N;
+    const int f = sd.pofs + s;
+    auto lower = [&](int key) {
+        int lo = 0, hi = sd.N;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (anc[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    if (i < sd.N) a.first[f + i] = lower(i);
+    if (i == sd.N - 1) a.first[f + sd.N] = sd.N;
 }
 
 // Reverse kernel of frame t.
@@ -352,11 +362,16 @@ __global__ void __launch_bounds__(256) k_adj_step(AdjArgs a)
     const int j0 = a.t_j0[tile];
     const SeqDesc sd = ba.seqs[s];
     float *sm = reinterpret_cast<float *>(sm4);
-    float *smu = sm + ba.table_floats;
+    float *smu = sm + ba.stage_floats;
     float *ssg = smu + K * D;
     float *acc = ssg + K * NP;
     unsigned long long *mask = reinterpret_cast<unsigned long long *>(acc + ((K * kGV + 1) & ~1));
+    const StepArgs v = seq_view(ba, sd);
+    const float *fcand = v.cand + (size_t)a.t * ba.J * ba.cap * 2;
+    CandPrefetch cpf;
+    cand_prefetch(cpf, v.count + (size_t)a.t * ba.J, fcand, ba.J, ba.cap);
     stage_table(sm4, reinterpret_cast<const float4 *>(ba.tables + (size_t)sd.table_ofs), ba.table_floats >> 2);
+    cand_store(cpf, sm + ba.table_floats, fcand, ba.J, ba.cap);
     for (int i = threadIdx.x; i < K * D; i += blockDim.x) smu[i] = a.mus[(size_t)s * K * D + i];
     for (int i = threadIdx.x; i < K * NP; i += blockDim.x) ssg[i] = a.sigs[(size_t)s * K * NP + i];
     clear_terms(acc, mask, K);
@@ -370,7 +385,6 @@ __global__ void __launch_bounds__(256) k_adj_step(AdjArgs a)
     for (int i = 0; i < kGV; ++i) term[i] = 0.f;
     int k = 0;
     if (live) {
-        const StepArgs v = seq_view(ba, sd);
         const int src = a.anc_prev ? a.anc_prev[sd.pofs + j] : j;
         float x[D], P[NP];
         load_state(a.in_x, a.in_P, ba.ld, sd.pofs + src, x, P);
@@ -398,7 +412,7 @@ __global__ void __launch_bounds__(256) k_adj_step(AdjArgs a)
             int cnt;
             const float *cz;
             bool clutter;
-            on[oi] = joint_inputs<kModeCand>(v, t, b, cnt, cz, clutter);
+            on[oi] = joint_inputs<kModeCand>(v, t, b, cnt, cz, clutter, sm);
             if (!on[oi]) continue;
             const int r0 = 2 * oi, r1 = r0 + 1;
 #pragma unroll
@@ -448,7 +462,7 @@ __global__ void __launch_bounds__(256) k_adj_step(AdjArgs a)
             int cnt;
             const float *cz;
             bool clutter;
-            (void)joint_inputs<kModeCand>(v, t, b, cnt, cz, clutter);
+            (void)joint_inputs<kModeCand>(v, t, b, cnt, cz, clutter, sm);
             const int r0 = 2 * oi, r1 = r0 + 1;
             const JointRec &R = rec[oi];
             // GC = G+ C with G+ the symmetric gradient of the updated P
@@ -647,8 +661,8 @@ void launch_adj_t(pf_batch *b, const AdjArgs &a, int grid, size_t lds_replay, si
 int launch_adj(pf_batch *b, const AdjArgs &a, int grid, bool replay)
 {
     if (grid <= 0) return PF_OK;
-    const size_t lds_replay = 4 * (size_t)b->table_floats;
-    const size_t lds_step = 4 * ((size_t)b->table_floats + (size_t)b->K * (D + NP) + (((size_t)b->K * kGV + 1) & ~1)) + 8;
+    const size_t lds_replay = 4 * (size_t)b->stage_floats();
+    const size_t lds_step = 4 * ((size_t)b->stage_floats() + (size_t)b->K * (D + NP) + (((size_t)b->K * kGV + 1) & ~1)) + 8;
     switch (b->J) {
     case 1: launch_adj_t<1>(b, a, grid, lds_replay, lds_step, replay); break;
     case 2: launch_adj_t<2>(b, a, grid, lds_replay, lds_step, replay); break;
@@ -696,7 +710,7 @@ int pf_batch_lml_grad(pf_batch *b, const int32_t *count, const float *cand, cons
     if (b->K > kMaxAdjK) return set_error(PF_ERR_UNSUPPORTED, "adjoint supports K <= %d", kMaxAdjK);
     for (int s = 0; s < b->S; ++s)
         if (!b->sel_ok[s]) return set_error(PF_ERR_UNSUPPORTED, "adjoint needs a full-rank measurement matrix");
-    if ((size_t)step_lds_bytes(b->table_floats, b->nb_max) > (size_t)kMaxLdsBytes)
+    if ((size_t)step_lds_bytes(b->stage_floats(), b->nb_max) > (size_t)kMaxLdsBytes)
         return set_error(PF_ERR_UNSUPPORTED, "model image and resampler workspace exceed the LDS budget");
     const int C = std::min<int>(checkpoint_every, b->Tmax);
     if ((st = setup(b, C))) return st;

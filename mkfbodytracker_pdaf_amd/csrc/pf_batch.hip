@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
     const float *xin = odd ? a.buf_x[1] : a.buf_x[0], *Pin = odd ? a.buf_P[1] : a.buf_P[0];
     float *xout = odd ? a.buf_x[0] : a.buf_x[1], *Pout = odd ? a.buf_P[0] : a.buf_P[1];
     const unsigned rowb = (unsigned)a.ld * 4u;
-    unsigned long long *wsp = reinterpret_cast<unsigned long long *>(reinterpret_cast<float *>(sm4) + a.table_floats);
+    unsigned long long *wsp = reinterpret_cast<unsigned long long *>(reinterpret_cast<float *>(sm4) + a.stage_floats);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int s = a.tile_seq[tile];
         const int jt0 = a.tile_j0[tile];
@@ -109,10 +109,14 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
                 for (int p = 0; p < NP; ++p) P[r][p] = soa_ld(rP, voff, rowb * p);
             }
         }
+        StepArgs v = seq_view(a, sd);
+        const float *fcand = v.cand + (size_t)t * a.J * a.cap * 2;
+        CandPrefetch cpf;
+        cand_prefetch(cpf, v.count + (size_t)t * a.J, fcand, a.J, a.cap);
         __syncthreads();          // the previous tile's LDS readers are done
         stage_table(sm4, reinterpret_cast<const float4 *>(a.tables + (size_t)sd.table_ofs), a.table_floats >> 2);
+        cand_store(cpf, reinterpret_cast<float *>(sm4) + a.table_floats, fcand, a.J, a.cap);
         __syncthreads();
-        StepArgs v = seq_view(a, sd);
         if (a.rec_comp) v.comp_out = a.rec_comp + (size_t)t * a.ld + sd.pofs;
         const uint32_t U1 = a.uni[(size_t)(sd.tofs + t) * 2];
         float lmax = -INFINITY;
@@ -356,6 +360,7 @@ BatchArgs batch_args(pf_batch *b, const int32_t *count, const float *cand, const
     a.J = b->J;
     a.cap = b->cap;
     a.table_floats = b->table_floats;
+    a.stage_floats = b->stage_floats();
     a.parity = parity;
     a.fused = fused;
     a.nb_max = b->nb_max;
@@ -378,7 +383,7 @@ void launch_step_t(pf_batch *b, const BatchArgs &a, size_t lds)
 
 int launch_batch_step(pf_batch *b, const BatchArgs &a)
 {
-    const size_t lds = (size_t)step_lds_bytes(b->table_floats, b->nb_max);
+    const size_t lds = (size_t)step_lds_bytes(b->stage_floats(), b->nb_max);
     bool all_sel = true;
     for (int s = 0; s < b->S; ++s) all_sel = all_sel && b->sel_ok[s];
     const int selj = all_sel ? b->J : 0;
@@ -659,7 +664,7 @@ int pf_batch_run_dev(pf_batch *b, const int32_t *count, const float *cand, const
     if (!count || !cand || !uniforms || !lml) return set_error(PF_ERR_ARG, "null run argument");
     b->tracks = false;                        // a run consumes the initial tracks
     std::fill(b->fresh.begin(), b->fresh.end(), 0);
-    if ((size_t)step_lds_bytes(b->table_floats, b->nb_max) > (size_t)kMaxLdsBytes)
+    if ((size_t)step_lds_bytes(b->stage_floats(), b->nb_max) > (size_t)kMaxLdsBytes)
         return set_error(PF_ERR_UNSUPPORTED, "model image and resampler workspace exceed the LDS budget");
     hipEvent_t e0, e1;
     HIP_TRYB(hipEventCreate(&e0));

@@ -50,7 +50,7 @@ struct BatchArgs {
     double *lml;
     int *rec_comp;                // optional record: components of frame t at rec_comp[t * ld + column]
     int *rec_anc;                 // optional record: ancestors of frame t's resampling, same layout
-    int S, ld, K, m, J, cap, table_floats, parity, fused, nb_max, all_tiles;
+    int S, ld, K, m, J, cap, table_floats, stage_floats, parity, fused, nb_max, all_tiles;
     unsigned int order_packed, clutter_mask;
     float pdl, b0, gate;
 };
@@ -63,6 +63,8 @@ __device__ __forceinline__ StepArgs seq_view(const BatchArgs &a, const SeqDesc &
     v.count = a.count + (size_t)sd.tofs * a.J;
     v.cand = a.cand + (size_t)sd.tofs * a.J * a.cap * 2;
     v.pi_total = sd.pi_total;
+    v.table_floats = a.table_floats;
+    v.stage_floats = a.stage_floats;
     v.K = a.K;
     v.N = sd.N;
     v.m = a.m;
@@ -97,6 +99,7 @@ struct pf_batch {
     std::vector<char> loaded, sel_ok, fresh;   // fresh[s]: reset since the last run
     unsigned clutter_mask = 0, order_packed = 0;
     int table_floats = 0;
+    int stage_floats() const { return table_floats + mkfbt::cand_lds_floats(J, cap); }
     bool tracks = false;
     pf_batch_schedule sch{256, 1, 2};
     hipStream_t stream = nullptr;

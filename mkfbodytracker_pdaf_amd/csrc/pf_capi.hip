@@ -149,7 +149,7 @@ int check_launch(const char *what)
 template <int D, int SELJ>
 int launch_step_dj(pf_handle *h, const StepArgs &a, int mode, int block)
 {
-    const size_t lds = (size_t)step_lds_bytes(h->table_floats, a.anc_mode == kAncFused ? a.pend.nb : 0);
+    const size_t lds = (size_t)step_lds_bytes(a.stage_floats, a.anc_mode == kAncFused ? a.pend.nb : 0);
     const int grid = (h->N + block - 1) / block;
     if (mode == kModeRef)
         hipLaunchKernelGGL((k_step<D, kModeRef, SELJ>), dim3(grid), dim3(block), lds, h->stream, a);
@@ -312,6 +312,7 @@ StepArgs step_args(pf_handle *h, int mode, const uint32_t *uni, const float *Z, 
     a.cand = cand;
     a.pi_total = h->pi_total;
     a.table_floats = h->table_floats;
+    a.stage_floats = h->table_floats + (mode == kModeCand ? cand_lds_floats(h->J, h->cap) : 0);
     a.K = h->K;
     a.N = h->N;
     a.ld = h->ld;
@@ -324,7 +325,6 @@ StepArgs step_args(pf_handle *h, int mode, const uint32_t *uni, const float *Z, 
     a.pdl = (float)(h->cfg.p_detect / h->cfg.clutter_density);
     a.b0 = (float)(1.0 - h->cfg.p_detect * pg);
     a.gate = (float)h->cfg.gate_chi2;
-    (void)mode;
     return a;
 }
 
@@ -801,7 +801,8 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
     // image; if both do not fit one workgroup, ancestors are materialised by
     // a separate k_ancestors launch each frame.
     const int nbp = (h->N + kWeightsBlock - 1) / kWeightsBlock;
-    const bool fused = step_lds_bytes(h->table_floats, nbp) <= kMaxLdsBytes;
+    const int stage = h->table_floats + (mode == kModeCand ? cand_lds_floats(h->J, h->cap) : 0);
+    const bool fused = step_lds_bytes(stage, nbp) <= kMaxLdsBytes;
     auto one_frame = [&](int f) -> int {
         const int parity = f & 1;
         const int amode = (f == 0 || !fused) ? kAncArray : kAncFused;

@@ -74,6 +74,7 @@ struct StepArgs {
     const float *__restrict__ cand;
     unsigned long long pi_total;
     int table_floats;   // floats of the LDS model image (multiple of 4)
+    int stage_floats;   // LDS floats before the resolver workspace: image + staged detections (candidate mode)
     int K, N, ld, m, J, cap, zld;
     int parity;         // frame parity: selects maxkey[parity]
     int anc_mode;
@@ -417,19 +418,57 @@ __device__ __forceinline__ int draw_component(const unsigned long long *thr, int
 // Detections of joint b in frame t (candidate mode) or the per-particle
 // measurement (reference mode).  Returns false if a certain joint has no
 // detection this frame (the joint is skipped).
+// Candidate mode stages the frame's detections in LDS after the model image:
+// J x cap x 2 floats, then J int32 counts (padded to an even count).
+__host__ __device__ constexpr int cand_lds_floats(int J, int cap) { return J * cap * 2 + ((J + 1) & ~1); }
+
+// Loads of the staging are issued before the model image's and stored after
+// it, so both share one memory latency.
+struct CandPrefetch {
+    float v[4];
+    int c;
+};
+__device__ __forceinline__ void cand_prefetch(CandPrefetch &pf, const int *cnt, const float *cand, int J, int cap)
+{
+    const int n = J * cap * 2, bd = blockDim.x, tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = tid + r * bd;
+        pf.v[r] = i < n ? cand[i] : 0.f;
+    }
+    pf.c = tid < J ? cnt[tid] : 0;
+}
+__device__ __forceinline__ void cand_store(const CandPrefetch &pf, float *dst, const float *cand, int J, int cap)
+{
+    const int n = J * cap * 2, bd = blockDim.x, tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = tid + r * bd;
+        if (i < n) dst[i] = pf.v[r];
+    }
+    for (int i = tid + 4 * bd; i < n; i += bd) dst[i] = cand[i];
+    if (tid < J) reinterpret_cast<int *>(dst + n)[tid] = pf.c;
+}
+
+// Detections of joint b in frame t (candidate mode, from the LDS staging at
+// sm + table_floats) or the per-particle measurement (reference mode).
+// Returns false if a certain joint has no detection this frame (the joint is
+// skipped).
 template <int MODE>
 __device__ __forceinline__ bool joint_inputs(const StepArgs &a, int t, int b, int &cnt, const float *&cz,
-                                             bool &clutter)
+                                             bool &clutter, const float *sm)
 {
     clutter = (MODE == kModeCand) && ((a.clutter_mask >> b) & 1u);
     cnt = 1;
     cz = nullptr;
     if (MODE == kModeCand) {
-        cnt = a.count[(size_t)t * a.J + b];
-        cz = a.cand + ((size_t)t * a.J + b) * a.cap * 2;
+        const float *sc = sm + a.table_floats;
+        cnt = reinterpret_cast<const int *>(sc + a.J * a.cap * 2)[b];
+        cz = sc + b * a.cap * 2;
         if (cnt <= 0 && !clutter) return false;
         if (cnt > a.cap) cnt = a.cap;
     }
+    (void)t;
     return true;
 }
 
@@ -491,9 +530,9 @@ __device__ __forceinline__ float joint_measure(const StepArgs &a, int t, int j, 
 }
 
 // LDS bytes of k_step: model image + ancestor workspace (fused mode).
-__host__ __device__ constexpr int step_lds_bytes(int table_floats, int nb_pending)
+__host__ __device__ constexpr int step_lds_bytes(int stage_floats, int nb_pending)
 {
-    return 4 * table_floats + (nb_pending > 0 ? 8 * resolve_lds_words(nb_pending) : 0);
+    return 4 * stage_floats + (nb_pending > 0 ? 8 * resolve_lds_words(nb_pending) : 0);
 }
 
 // One particle's frame: component draw (pf2DRao.cpp:128), mean-reverting
@@ -527,7 +566,7 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
             int cnt;
             const float *cz;
             bool clutter;
-            if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter)) continue;
+            if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter, sm)) continue;
             float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
             Joint<D>::prepare_sel(x, P, 2 * oi, BHs[2 * b], BHs[2 * b + 1], Rs[3 * b], Rs[3 * b + 1],
                                   Rs[3 * b + 2], w0, w1, y0, y1, il00, l10, il11, logl);
@@ -542,7 +581,7 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
             int cnt;
             const float *cz;
             bool clutter;
-            if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter)) continue;
+            if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter, sm)) continue;
             float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
             Joint<D>::prepare(x, P, Hs + (2 * b) * D, Hs + (2 * b + 1) * D, BHs[2 * b], BHs[2 * b + 1],
                               Rs[3 * b], Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0, y1, il00, l10,
@@ -571,7 +610,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
         // Consume the previous frame's resampling: a_j, then the frame's
         // statistics (block 0).
         unsigned long long *wsp = reinterpret_cast<unsigned long long *>(
-            reinterpret_cast<float *>(sm4) + a.table_floats);
+            reinterpret_cast<float *>(sm4) + a.stage_floats);
         double es = 0.0;
         src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0, es);
         if (live) a.anc[j] = src;
@@ -592,7 +631,12 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
 #pragma unroll
         for (int p = 0; p < NP; ++p) P[p] = soa_ld(rP, voff, rowb * p);
     }
+    CandPrefetch cpf;
+    if (MODE == kModeCand) cand_prefetch(cpf, a.count + (size_t)t * a.J, a.cand + (size_t)t * a.J * a.cap * 2, a.J, a.cap);
     stage_table(sm4, reinterpret_cast<const float4 *>(a.table), a.table_floats >> 2);
+    if (MODE == kModeCand)
+        cand_store(cpf, reinterpret_cast<float *>(sm4) + a.table_floats, a.cand + (size_t)t * a.J * a.cap * 2, a.J,
+                   a.cap);
     __syncthreads();
 
     float lwj = -INFINITY;
