@@ -39,6 +39,21 @@ def build_cpp_test() -> str:
     return CPP_TEST_BIN
 
 
+ADJ_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_adjoint_local.cpp")
+ADJ_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_adjoint_local")
+
+
+def build_adjoint_local_test() -> str:
+    """Compile the host-only check of the joint-local reverse (hipcc: the
+    header is host- and device-callable)."""
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", ADJ_TEST_SRC, "-o", ADJ_TEST_BIN]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + res.stderr[-4000:])
+    return ADJ_TEST_BIN
+
+
 def build(verbose: bool = False) -> str:
     """Compile every source under csrc/ into libmkfbt.so (always rebuilds)."""
     tmp = LIB + ".tmp"
@@ -51,6 +66,8 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if LIB.endswith("libmkfbt.so") and os.path.exists(CPP_TEST_SRC):
         build_cpp_test()
+    if LIB.endswith("libmkfbt.so") and os.path.exists(ADJ_TEST_SRC):
+        build_adjoint_local_test()
     return LIB
 
 

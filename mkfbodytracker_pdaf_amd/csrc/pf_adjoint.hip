@@ -15,8 +15,8 @@
 // particle the reverse kernel recomputes its frame from the ancestor's state
 // (recording the 2-row joint inputs), then propagates the adjoint of the
 // updated state back through the joint updates (reverse mode over the
-// d-dimensional state, forward-mode duals over each joint's 5 local inputs
-// y, S) and the mean-reverting predict.  Adjoints of a frame's inputs flow to
+// d-dimensional state and, per joint, over its 5 local inputs y, S:
+// pf_adjoint_local.hpp) and the mean-reverting predict.  Adjoints of a frame's inputs flow to
 // the parents by the recorded ancestors: ancestors are sorted, so each
 // parent's children are a contiguous range and the scatter is a deterministic
 // gather.  Per-component gradient terms are reduced per wave (by component),
@@ -28,6 +28,7 @@
 #include <cstring>
 #include <vector>
 
+#include "pf_adjoint_local.hpp"
 #include "pf_batch_impl.hpp"
 
 using namespace mkfbt;
@@ -40,176 +41,6 @@ constexpr int NP = npacked(kBD);
 constexpr int kGV = D + NP + 2;       // per-component terms: mu' (D), Sigma' packed (NP), gamma, sum of wbar
 constexpr int kMaxAdjK = 64;
 constexpr int kAdjTile = 256;
-
-// ------------------------------------------------------------ dual numbers
-// Value and derivatives along the 5 local inputs of a joint (y0, y1, S00,
-// S01, S11), FP32.
-struct D5 {
-    float v, d[5];
-};
-__device__ __forceinline__ D5 dc(float v)
-{
-    D5 r;
-    r.v = v;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) r.d[i] = 0.f;
-    return r;
-}
-__device__ __forceinline__ D5 dseed(float v, int i)
-{
-    D5 r = dc(v);
-    r.d[i] = 1.f;
-    return r;
-}
-__device__ __forceinline__ D5 operator+(const D5 &a, const D5 &b)
-{
-    D5 r;
-    r.v = a.v + b.v;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) r.d[i] = a.d[i] + b.d[i];
-    return r;
-}
-__device__ __forceinline__ D5 operator-(const D5 &a, const D5 &b)
-{
-    D5 r;
-    r.v = a.v - b.v;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) r.d[i] = a.d[i] - b.d[i];
-    return r;
-}
-__device__ __forceinline__ D5 operator*(const D5 &a, const D5 &b)
-{
-    D5 r;
-    r.v = a.v * b.v;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) r.d[i] = fmaf(a.d[i], b.v, a.v * b.d[i]);
-    return r;
-}
-__device__ __forceinline__ D5 operator*(float s, const D5 &a)
-{
-    D5 r;
-    r.v = s * a.v;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) r.d[i] = s * a.d[i];
-    return r;
-}
-__device__ __forceinline__ D5 operator+(float s, const D5 &a)
-{
-    D5 r = a;
-    r.v = s + a.v;
-    return r;
-}
-__device__ __forceinline__ D5 operator-(float s, const D5 &a)
-{
-    D5 r;
-    r.v = s - a.v;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) r.d[i] = -a.d[i];
-    return r;
-}
-__device__ __forceinline__ D5 drcp(const D5 &a)
-{
-    D5 r;
-    r.v = 1.f / a.v;
-    const float g = -r.v * r.v;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) r.d[i] = g * a.d[i];
-    return r;
-}
-__device__ __forceinline__ D5 dsqrt(const D5 &a)
-{
-    D5 r;
-    r.v = sqrtf(a.v);
-    const float g = 0.5f / r.v;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) r.d[i] = g * a.d[i];
-    return r;
-}
-__device__ __forceinline__ D5 dlog(const D5 &a)
-{
-    D5 r;
-    r.v = logf(a.v);
-    const float g = 1.f / a.v;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) r.d[i] = g * a.d[i];
-    return r;
-}
-__device__ __forceinline__ D5 dexp(const D5 &a)
-{
-    D5 r;
-    r.v = __expf(a.v);
-#pragma unroll
-    for (int i = 0; i < 5; ++i) r.d[i] = r.v * a.d[i];
-    return r;
-}
-
-// The local function of one joint: from (y, S) to the update x += C g,
-// P -= C N C^T (C = P[:, rows]) and the log-likelihood term ell, as the
-// forward computes them (Joint::prepare_sel + joint_measure + apply, with
-// g = L^-T v and N = L^-T M L^-1).
-struct Local {
-    D5 g0, g1, n00, n01, n11, ell;
-};
-
-__device__ __forceinline__ Local local_joint(const StepArgs &v, float y0f, float y1f, float s00f, float s01f,
-                                             float s11f, bool clutter, int cnt, const float *cz)
-{
-    const float LOG2PI = 1.8378770664093453f;
-    const D5 y0 = dseed(y0f, 0), y1 = dseed(y1f, 1);
-    const D5 s00 = dseed(s00f, 2), s01 = dseed(s01f, 3), s11 = dseed(s11f, 4);
-    const D5 l00 = dsqrt(s00);
-    const D5 il00 = drcp(l00);
-    const D5 l10 = s01 * il00;
-    const D5 l11 = dsqrt(s11 - l10 * l10);
-    const D5 il11 = drcp(l11);
-    const D5 a10 = dc(0.f) - (l10 * il00) * il11;         // (L^-1)_{10}
-    const D5 lnorm = dc(-LOG2PI) - dlog(l00 * l11);
-    Local o;
-    if (!clutter) {
-        const D5 n0 = dc(cz[0]) - y0, n1 = dc(cz[1]) - y1;
-        const D5 u0 = n0 * il00;
-        const D5 u1 = a10 * n0 + il11 * n1;
-        o.g0 = il00 * u0 + a10 * u1;
-        o.g1 = il11 * u1;
-        o.n00 = il00 * il00 + a10 * a10;
-        o.n01 = a10 * il11;
-        o.n11 = il11 * il11;
-        o.ell = lnorm - 0.5f * (u0 * u0 + u1 * u1);
-        return o;
-    }
-    D5 se = dc(0.f), sv0 = dc(0.f), sv1 = dc(0.f), q00 = dc(0.f), q01 = dc(0.f), q11 = dc(0.f);
-    for (int c = 0; c < cnt; ++c) {
-        const D5 n0 = dc(cz[2 * c]) - y0, n1 = dc(cz[2 * c + 1]) - y1;
-        const D5 u0 = n0 * il00;
-        const D5 u1 = a10 * n0 + il11 * n1;
-        const D5 d2 = u0 * u0 + u1 * u1;
-        if (d2.v <= v.gate) {                               // gating is a discrete choice: held fixed
-            const D5 e = v.pdl * dexp(lnorm - 0.5f * d2);
-            se = se + e;
-            sv0 = sv0 + e * u0;
-            sv1 = sv1 + e * u1;
-            q00 = q00 + (e * u0) * u0;
-            q01 = q01 + (e * u0) * u1;
-            q11 = q11 + (e * u1) * u1;
-        }
-    }
-    const D5 ells = v.b0 + se;
-    const D5 inv = drcp(ells);
-    const D5 w0 = sv0 * inv, w1 = sv1 * inv;
-    const D5 bs = se * inv;
-    const D5 m00 = bs - (q00 * inv - w0 * w0);
-    const D5 m01 = dc(0.f) - (q01 * inv - w0 * w1);
-    const D5 m11 = bs - (q11 * inv - w1 * w1);
-    o.g0 = il00 * w0 + a10 * w1;
-    o.g1 = il11 * w1;
-    const D5 A00 = m00 * il00 + m01 * a10, A01 = m01 * il11;
-    const D5 A10 = m01 * il00 + m11 * a10, A11 = m11 * il11;
-    o.n00 = il00 * A00 + a10 * A10;
-    o.n01 = il00 * A01 + a10 * A11;
-    o.n11 = il11 * A11;
-    o.ell = dlog(ells);
-    return o;
-}
 
 // Inputs of one joint recorded on the way forward.
 struct JointRec {
@@ -233,30 +64,47 @@ struct AdjArgs {
     int t, t0;
 };
 
-// Wave-then-block reduction of the per-particle component terms into the
-// tile's FP64 accumulator.  acc is the block's LDS accumulator (K x kGV,
-// zeroed by the caller), mask the set of components present.
+// Sum over the 16 lanes of each row of the wave, in every lane of the row,
+// by four DPP steps (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror): VALU only, no LDS round trips.
+__device__ __forceinline__ float row_sum16(float v)
+{
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, true));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, true));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, true));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, true));
+    return v;
+}
+
+// Reduction of the per-particle component terms into the block's LDS
+// accumulator acc (K x kGV, zeroed by the caller; mask = components present):
+// per distinct component of the wave, row sums by DPP, then one LDS atomic
+// add per row (4 per value and wave).
 __device__ __forceinline__ void reduce_terms(bool live, int k, const float (&term)[kGV], float *acc,
                                              unsigned long long *mask)
 {
     const int lane = threadIdx.x & 63;
+    const bool row_lead = (lane & 15) == 0;
     unsigned long long rem = __ballot(live);
     while (rem) {
         const int leader = __ffsll((long long)rem) - 1;
         const int kk = __shfl(k, leader, 64);
         const bool sel = live && k == kk;
         rem &= ~__ballot(sel);
-        float keep0 = 0.f, keep1 = 0.f;
+        float *dst = acc + kk * kGV;
+        constexpr int kChunk = 8;
 #pragma unroll
-        for (int v = 0; v < kGV; ++v) {
-            const float s = wave_sum(sel ? term[v] : 0.f);
-            if ((v & 63) == lane) {
-                if (v < 64) keep0 = s;
-                else keep1 = s;
+        for (int v0 = 0; v0 < kGV; v0 += kChunk) {
+            float r[kChunk];
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i)
+                if (v0 + i < kGV) r[i] = row_sum16(sel ? term[v0 + i] : 0.f);
+            if (row_lead) {
+#pragma unroll
+                for (int i = 0; i < kChunk; ++i)
+                    if (v0 + i < kGV) atomicAdd(dst + v0 + i, r[i]);
             }
         }
-        atomicAdd(acc + kk * kGV + lane, keep0);
-        if (lane < kGV - 64) atomicAdd(acc + kk * kGV + 64 + lane, keep1);
         if (lane == 0) atomicOr(mask, 1ull << kk);
     }
 }
@@ -415,16 +263,18 @@ __global__ void __launch_bounds__(256) k_adj_step(AdjArgs a)
             on[oi] = joint_inputs<kModeCand>(v, t, b, cnt, cz, clutter, sm);
             if (!on[oi]) continue;
             const int r0 = 2 * oi, r1 = r0 + 1;
+            if (oi > 0) {            // the first joint's inputs are recomputed from the ancestor's state
 #pragma unroll
-            for (int i = 0; i < D; ++i) {
-                rec[oi].c0[i] = P[pk(D, r0, i)];
-                rec[oi].c1[i] = P[pk(D, r1, i)];
+                for (int i = 0; i < D; ++i) {
+                    rec[oi].c0[i] = P[pk(D, r0, i)];
+                    rec[oi].c1[i] = P[pk(D, r1, i)];
+                }
+                rec[oi].y0 = x[r0] + BHs[2 * b];
+                rec[oi].y1 = x[r1] + BHs[2 * b + 1];
+                rec[oi].s00 = P[pk(D, r0, r0)] + Rs[3 * b];
+                rec[oi].s01 = P[pk(D, r0, r1)] + Rs[3 * b + 1];
+                rec[oi].s11 = P[pk(D, r1, r1)] + Rs[3 * b + 2];
             }
-            rec[oi].y0 = x[r0] + BHs[2 * b];
-            rec[oi].y1 = x[r1] + BHs[2 * b + 1];
-            rec[oi].s00 = P[pk(D, r0, r0)] + Rs[3 * b];
-            rec[oi].s01 = P[pk(D, r0, r1)] + Rs[3 * b + 1];
-            rec[oi].s11 = P[pk(D, r1, r1)] + Rs[3 * b + 2];
             float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
             Joint<D>::prepare_sel(x, P, r0, BHs[2 * b], BHs[2 * b + 1], Rs[3 * b], Rs[3 * b + 1], Rs[3 * b + 2], w0,
                                   w1, y0, y1, il00, l10, il11, logl);
@@ -464,7 +314,23 @@ __global__ void __launch_bounds__(256) k_adj_step(AdjArgs a)
             bool clutter;
             (void)joint_inputs<kModeCand>(v, t, b, cnt, cz, clutter, sm);
             const int r0 = 2 * oi, r1 = r0 + 1;
-            const JointRec &R = rec[oi];
+            JointRec R0;
+            if (oi == 0) {           // rows r0, r1 of the predicted state, as the forward computed them
+                const unsigned rowb = (unsigned)ba.ld * 4u;
+                const __amdgpu_buffer_rsrc_t rx = soa_rsrc(a.in_x, rowb * D), rP = soa_rsrc(a.in_P, rowb * NP);
+                const unsigned off = (unsigned)(sd.pofs + src) * 4u;
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    R0.c0[i] = fmaf(g2, soa_ld(rP, off, rowb * pk(D, r0, i)), ck[2 + D + pk(D, r0, i)]);
+                    R0.c1[i] = fmaf(g2, soa_ld(rP, off, rowb * pk(D, r1, i)), ck[2 + D + pk(D, r1, i)]);
+                }
+                R0.y0 = fmaf(g, soa_ld(rx, off, rowb * r0), ck[2 + r0]) + BHs[2 * b];
+                R0.y1 = fmaf(g, soa_ld(rx, off, rowb * r1), ck[2 + r1]) + BHs[2 * b + 1];
+                R0.s00 = R0.c0[r0] + Rs[3 * b];
+                R0.s01 = R0.c0[r1] + Rs[3 * b + 1];
+                R0.s11 = R0.c1[r1] + Rs[3 * b + 2];
+            }
+            const JointRec &R = (oi == 0) ? R0 : rec[oi];
             // GC = G+ C with G+ the symmetric gradient of the updated P
             // (packed off-diagonal adjoints count both entries).
             float gc0[D], gc1[D];
@@ -489,17 +355,15 @@ __global__ void __launch_bounds__(256) k_adj_step(AdjArgs a)
                 nb01 = fmaf(-R.c0[i], gc1[i], nb01);
                 nb11 = fmaf(-R.c1[i], gc1[i], nb11);
             }
-            const Local L = local_joint(v, R.y0, R.y1, R.s00, R.s01, R.s11, clutter, cnt, cz);
-            float din[5];
-#pragma unroll
-            for (int q = 0; q < 5; ++q)
-                din[q] = gb0 * L.g0.d[q] + gb1 * L.g1.d[q] + nb00 * L.n00.d[q] + 2.f * nb01 * L.n01.d[q] +
-                         nb11 * L.n11.d[q] + wbar * L.ell.d[q];
+            const adj::LocalRev L = adj::local_joint_rev(adj::PdafConst{v.gate, v.pdl, v.b0}, R.y0, R.y1, R.s00,
+                                                         R.s01, R.s11, clutter, cnt, cz, gb0, gb1, nb00, nb01, nb11,
+                                                         wbar);
+            const float din[5] = {L.yb0, L.yb1, L.sb00, L.sb01, L.sb11};
             // C-bar = xbar g^T - 2 GC N, into the packed entries of P read by C
 #pragma unroll
             for (int i = 0; i < D; ++i) {
-                const float cb0 = xb[i] * L.g0.v - 2.f * (gc0[i] * L.n00.v + gc1[i] * L.n01.v);
-                const float cb1 = xb[i] * L.g1.v - 2.f * (gc0[i] * L.n01.v + gc1[i] * L.n11.v);
+                const float cb0 = xb[i] * L.g0 - 2.f * (gc0[i] * L.n00 + gc1[i] * L.n01);
+                const float cb1 = xb[i] * L.g1 - 2.f * (gc0[i] * L.n01 + gc1[i] * L.n11);
                 Pb[pk(D, i, r0)] += cb0;
                 Pb[pk(D, i, r1)] += cb1;
             }
@@ -510,20 +374,21 @@ __global__ void __launch_bounds__(256) k_adj_step(AdjArgs a)
             Pb[pk(D, r1, r1)] += din[4];
         }
         // ---- predict x = g x + (1 - g) mu', P = g^2 P + (1 - g^2) Sigma'
-        float xp[D], Pp[NP];
-        load_state(a.in_x, a.in_P, ba.ld, sd.pofs + src, xp, Pp);
         const float *mu = smu + k * D, *sg = ssg + k * NP;
+        const unsigned rowb = (unsigned)ba.ld * 4u;
+        const __amdgpu_buffer_rsrc_t rx = soa_rsrc(a.in_x, rowb * D), rP = soa_rsrc(a.in_P, rowb * NP);
+        const unsigned off = (unsigned)(sd.pofs + src) * 4u;
         float gbar = 0.f;
 #pragma unroll
         for (int i = 0; i < D; ++i) {
-            gbar = fmaf(xb[i], xp[i] - mu[i], gbar);
+            gbar = fmaf(xb[i], soa_ld(rx, off, rowb * i) - mu[i], gbar);
             term[i] = (1.f - g) * xb[i];
             xb[i] *= g;
         }
         float gpp = 0.f;
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-            gpp = fmaf(Pb[p], Pp[p] - sg[p], gpp);
+            gpp = fmaf(Pb[p], soa_ld(rP, off, rowb * p) - sg[p], gpp);
             term[D + p] = (1.f - g2) * Pb[p];
             Pb[p] *= g2;
         }
@@ -614,13 +479,24 @@ int setup(pf_batch *b, int C)
     if ((st = ensure(&b->rec_anc, (size_t)b->Tmax * ld, h[1]))) return st;
     if ((st = ensure(&b->ck_x, (size_t)b->n_ckpt * xw, h[2]))) return st;
     if ((st = ensure(&b->ck_P, (size_t)b->n_ckpt * Pw, h[3]))) return st;
-    if ((st = ensure(&b->seg_x, (size_t)b->seg_slots * xw, h[4]))) return st;
-    if ((st = ensure(&b->seg_P, (size_t)b->seg_slots * Pw, h[5]))) return st;
+    if ((st = ensure(&b->seg_x[0], (size_t)b->seg_slots * xw, h[4]))) return st;
+    if ((st = ensure(&b->seg_P[0], (size_t)b->seg_slots * Pw, h[5]))) return st;
+    if ((st = ensure(&b->seg_x[1], (size_t)b->seg_slots * xw, h[16]))) return st;
+    if ((st = ensure(&b->seg_P[1], (size_t)b->seg_slots * Pw, h[17]))) return st;
     if ((st = ensure(&b->adj_x[0], xw, h[6]))) return st;
     if ((st = ensure(&b->adj_x[1], xw, h[7]))) return st;
     if ((st = ensure(&b->adj_P[0], Pw, h[8]))) return st;
     if ((st = ensure(&b->adj_P[1], Pw, h[9]))) return st;
-    if ((st = ensure(&b->first, ld + b->S + 1, h[10]))) return st;
+    if ((st = ensure(&b->first, 2 * (size_t)C * (ld + b->S + 1), h[10]))) return st;
+    if (!b->stream2) HIP_TRYB(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
+    if (!b->ev_fwd) HIP_TRYB(hipEventCreateWithFlags(&b->ev_fwd, hipEventDisableTiming));
+    while ((int)b->ev_replay.size() < b->n_ckpt) {
+        hipEvent_t e1, e2;
+        HIP_TRYB(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+        HIP_TRYB(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+        b->ev_replay.push_back(e1);
+        b->ev_back.push_back(e2);
+    }
     // 256-particle tiles, sequences in length order (active ones first)
     std::vector<int> ts, tj, tf(b->S + 1);
     for (int i = 0; i < b->S; ++i) {
@@ -650,26 +526,26 @@ int setup(pf_batch *b, int C)
 }
 
 template <int SELJ>
-void launch_adj_t(pf_batch *b, const AdjArgs &a, int grid, size_t lds_replay, size_t lds_step, bool replay)
+void launch_adj_t(const AdjArgs &a, int grid, size_t lds_replay, size_t lds_step, bool replay, hipStream_t st)
 {
     if (replay)
-        hipLaunchKernelGGL((k_adj_replay<SELJ>), dim3(grid), dim3(kAdjTile), lds_replay, b->stream, a);
+        hipLaunchKernelGGL((k_adj_replay<SELJ>), dim3(grid), dim3(kAdjTile), lds_replay, st, a);
     else
-        hipLaunchKernelGGL((k_adj_step<SELJ>), dim3(grid), dim3(kAdjTile), lds_step, b->stream, a);
+        hipLaunchKernelGGL((k_adj_step<SELJ>), dim3(grid), dim3(kAdjTile), lds_step, st, a);
 }
 
-int launch_adj(pf_batch *b, const AdjArgs &a, int grid, bool replay)
+int launch_adj(pf_batch *b, const AdjArgs &a, int grid, bool replay, hipStream_t st)
 {
     if (grid <= 0) return PF_OK;
     const size_t lds_replay = 4 * (size_t)b->stage_floats();
     const size_t lds_step = 4 * ((size_t)b->stage_floats() + (size_t)b->K * (D + NP) + (((size_t)b->K * kGV + 1) & ~1)) + 8;
     switch (b->J) {
-    case 1: launch_adj_t<1>(b, a, grid, lds_replay, lds_step, replay); break;
-    case 2: launch_adj_t<2>(b, a, grid, lds_replay, lds_step, replay); break;
-    case 3: launch_adj_t<3>(b, a, grid, lds_replay, lds_step, replay); break;
-    case 4: launch_adj_t<4>(b, a, grid, lds_replay, lds_step, replay); break;
-    case 5: launch_adj_t<5>(b, a, grid, lds_replay, lds_step, replay); break;
-    case 6: launch_adj_t<6>(b, a, grid, lds_replay, lds_step, replay); break;
+    case 1: launch_adj_t<1>(a, grid, lds_replay, lds_step, replay, st); break;
+    case 2: launch_adj_t<2>(a, grid, lds_replay, lds_step, replay, st); break;
+    case 3: launch_adj_t<3>(a, grid, lds_replay, lds_step, replay, st); break;
+    case 4: launch_adj_t<4>(a, grid, lds_replay, lds_step, replay, st); break;
+    case 5: launch_adj_t<5>(a, grid, lds_replay, lds_step, replay, st); break;
+    case 6: launch_adj_t<6>(a, grid, lds_replay, lds_step, replay, st); break;
     default: return set_error(PF_ERR_UNSUPPORTED, "adjoint: %d joints", b->J);
     }
     HIP_TRYB(hipGetLastError());
@@ -742,40 +618,73 @@ int pf_batch_lml_grad(pf_batch *b, const int32_t *count, const float *cand, cons
     a.t_j0 = b->t256_j0;
     a.bins = b->rec_bins;
     a.tacc = b->tacc;
-    a.first = b->first;
-    for (int seg = b->n_ckpt - 1; seg >= 0; --seg) {
+    // Segments, last first.  The replay of segment s-1 (and its child ranges)
+    // runs on the second stream while the reverse kernels of segment s run on
+    // the first; segment buffers alternate by segment parity.
+    hipStream_t M = b->stream, R = b->stream2;
+    const size_t fst = ld + (size_t)b->S + 1;
+    HIP_TRYB(hipEventRecord(b->ev_fwd, M));
+    HIP_TRYB(hipStreamWaitEvent(R, b->ev_fwd, 0));
+    auto seg_input_x = [&](int seg, int t, int t0) -> const float * {
+        return t == t0 ? b->ck_x + (size_t)seg * xw : b->seg_x[seg & 1] + (size_t)(t - 1 - t0) * xw;
+    };
+    auto seg_input_P = [&](int seg, int t, int t0) -> const float * {
+        return t == t0 ? b->ck_P + (size_t)seg * Pw : b->seg_P[seg & 1] + (size_t)(t - 1 - t0) * Pw;
+    };
+    auto replay_seg = [&](int seg) -> int {
         const int t0 = seg * C, t1 = std::min(t0 + C, b->Tmax);
-        const float *ckx = b->ck_x + (size_t)seg * xw, *ckP = b->ck_P + (size_t)seg * Pw;
-        auto input_x = [&](int t) { return t == t0 ? ckx : b->seg_x + (size_t)(t - 1 - t0) * xw; };
-        auto input_P = [&](int t) { return t == t0 ? ckP : b->seg_P + (size_t)(t - 1 - t0) * Pw; };
+        AdjArgs r = a;
         for (int t = t0; t + 1 < t1; ++t) {          // states of frames t0 .. t1-2
-            a.t = t;
-            a.t0 = t0;
-            a.in_x = input_x(t);
-            a.in_P = input_P(t);
-            a.out_x = b->seg_x + (size_t)(t - t0) * xw;
-            a.out_P = b->seg_P + (size_t)(t - t0) * Pw;
-            a.anc_prev = t > 0 ? b->rec_anc + (size_t)(t - 1) * ld : nullptr;
-            if ((st = launch_adj(b, a, b->h_t256_count[t], true))) return st;
+            r.t = t;
+            r.t0 = t0;
+            r.in_x = seg_input_x(seg, t, t0);
+            r.in_P = seg_input_P(seg, t, t0);
+            r.out_x = b->seg_x[seg & 1] + (size_t)(t - t0) * xw;
+            r.out_P = b->seg_P[seg & 1] + (size_t)(t - t0) * Pw;
+            r.anc_prev = t > 0 ? b->rec_anc + (size_t)(t - 1) * ld : nullptr;
+            const int rs = launch_adj(b, r, b->h_t256_count[t], true, R);
+            if (rs) return rs;
         }
+        for (int t = t0; t < t1; ++t) {              // child ranges of every frame's resampling
+            const int grid = b->h_t256_count[t];
+            if (grid <= 0) continue;
+            r.t = t;
+            r.anc_t = b->rec_anc + (size_t)t * ld;
+            r.first = b->first + ((size_t)(seg & 1) * C + (t - t0)) * fst;
+            hipLaunchKernelGGL(k_adj_first, dim3(grid), dim3(kAdjTile), 0, R, r);
+            HIP_TRYB(hipGetLastError());
+        }
+        HIP_TRYB(hipEventRecord(b->ev_replay[seg], R));
+        return PF_OK;
+    };
+    auto backward_seg = [&](int seg) -> int {
+        const int t0 = seg * C, t1 = std::min(t0 + C, b->Tmax);
+        HIP_TRYB(hipStreamWaitEvent(M, b->ev_replay[seg], 0));
         for (int t = t1 - 1; t >= t0; --t) {
             a.t = t;
             a.t0 = t0;
-            a.in_x = input_x(t);
-            a.in_P = input_P(t);
+            a.in_x = seg_input_x(seg, t, t0);
+            a.in_P = seg_input_P(seg, t, t0);
             a.anc_prev = t > 0 ? b->rec_anc + (size_t)(t - 1) * ld : nullptr;
-            a.anc_t = b->rec_anc + (size_t)t * ld;
+            a.first = b->first + ((size_t)(seg & 1) * C + (t - t0)) * fst;
             a.adj_in_x = b->adj_x[(t + 1) & 1];
             a.adj_in_P = b->adj_P[(t + 1) & 1];
             a.adj_out_x = b->adj_x[t & 1];
             a.adj_out_P = b->adj_P[t & 1];
-            const int grid = b->h_t256_count[t];
-            if (grid > 0) {
-                hipLaunchKernelGGL(k_adj_first, dim3(grid), dim3(kAdjTile), 0, b->stream, a);
-                HIP_TRYB(hipGetLastError());
-            }
-            if ((st = launch_adj(b, a, grid, false))) return st;
+            const int rs = launch_adj(b, a, b->h_t256_count[t], false, M);
+            if (rs) return rs;
         }
+        HIP_TRYB(hipEventRecord(b->ev_back[seg], M));
+        return PF_OK;
+    };
+    const int last = b->n_ckpt - 1;
+    if ((st = replay_seg(last))) return st;
+    for (int seg = last; seg >= 0; --seg) {
+        if (seg > 0) {
+            if (seg + 1 <= last) HIP_TRYB(hipStreamWaitEvent(R, b->ev_back[seg + 1], 0));
+            if ((st = replay_seg(seg - 1))) return st;
+        }
+        if ((st = backward_seg(seg))) return st;
     }
     a.adj_in_x = b->adj_x[0];
     a.adj_in_P = b->adj_P[0];

@@ -497,12 +497,19 @@ int pf_batch_destroy(pf_batch *b)
     void *ptrs[] = {b->bx[0], b->bx[1], b->bP[0], b->bP[1], b->lw, b->tables, b->mus, b->sigs, b->Tis, b->Ts,
                     b->bexp, b->partial, b->est, b->gbuf, b->anc, b->frame, b->maxkey, b->cloc, b->bsum,
                     b->status, b->Upend, b->seqs, b->tile_seq, b->tile_j0, b->n_tiles, b->wb_seq, b->wb_b,
-                    b->n_wblocks, b->fs, b->rec_comp, b->rec_anc, b->rec_bins, b->ck_x, b->ck_P, b->seg_x,
-                    b->seg_P, b->adj_x[0], b->adj_x[1], b->adj_P[0], b->adj_P[1], b->first, b->tacc, b->gsum,
+                    b->n_wblocks, b->fs, b->rec_comp, b->rec_anc, b->rec_bins, b->ck_x, b->ck_P, b->seg_x[0],
+                    b->seg_x[1], b->seg_P[0], b->seg_P[1], b->adj_x[0], b->adj_x[1], b->adj_P[0], b->adj_P[1], b->first, b->tacc, b->gsum,
                     b->t256_seq, b->t256_j0, b->t256_first};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream2) {
+        (void)hipStreamSynchronize(b->stream2);
+        (void)hipStreamDestroy(b->stream2);
+    }
+    for (hipEvent_t e : b->ev_replay) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->ev_back) (void)hipEventDestroy(e);
+    if (b->ev_fwd) (void)hipEventDestroy(b->ev_fwd);
     delete b;
     return PF_OK;
 }

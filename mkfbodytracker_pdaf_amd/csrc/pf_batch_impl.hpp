@@ -121,7 +121,11 @@ struct pf_batch {
     // adjoint (pf_batch_lml_grad): records and checkpoints of the last recorded run
     int *rec_comp = nullptr, *rec_anc = nullptr, *rec_bins = nullptr;
     int rec_frames = 0, ckpt_every = 0, n_ckpt = 0, seg_slots = 0;
-    float *ck_x = nullptr, *ck_P = nullptr, *seg_x = nullptr, *seg_P = nullptr;
+    float *ck_x = nullptr, *ck_P = nullptr;
+    float *seg_x[2] = {nullptr, nullptr}, *seg_P[2] = {nullptr, nullptr};   // by segment parity
+    hipStream_t stream2 = nullptr;            // the adjoint's replay stream (default priority)
+    std::vector<hipEvent_t> ev_replay, ev_back;
+    hipEvent_t ev_fwd = nullptr;
     float *adj_x[2] = {nullptr, nullptr}, *adj_P[2] = {nullptr, nullptr};
     int *first = nullptr;
     double *tacc = nullptr, *gsum = nullptr;

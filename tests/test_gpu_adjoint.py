@@ -106,8 +106,11 @@ def test_adjoint_independent_of_checkpoint_interval(gpu, run, C):
     b = _batch(models, bins)
     d = _inputs(seqs)
     g2 = b.lml_grad(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, checkpoint_every=C)
+    # the replayed states are bit-identical to the recorded ones; the block
+    # accumulation of the component terms uses LDS float atomics, whose order
+    # varies, so gradients agree to FP32 rounding of those sums
     for k in g:
-        assert np.array_equal(g[k], g2[k]), (C, k)
+        assert _rel(g2[k], g[k]) <= 1e-6, (C, k, _rel(g2[k], g[k]))
     fwd, bwd = b.last_adjoint_ms()
     assert fwd > 0 and bwd > 0
     b.close()
