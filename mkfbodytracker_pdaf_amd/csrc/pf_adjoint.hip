@@ -40,7 +40,10 @@ constexpr int D = kBD;
 constexpr int NP = npacked(kBD);
 constexpr int kGV = D + NP + 2;       // per-component terms: mu' (D), Sigma' packed (NP), gamma, sum of wbar
 constexpr int kMaxAdjK = 64;
-constexpr int kAdjTile = 256;
+#ifndef PF_ADJ_TILE
+#define PF_ADJ_TILE 128
+#endif
+constexpr int kAdjTile = PF_ADJ_TILE;    // particles (threads) per tile of the adjoint kernels
 
 // Inputs of one joint recorded on the way forward.
 struct JointRec {
@@ -149,7 +152,7 @@ __device__ __forceinline__ void store_state(float *xs, float *Ps, int ld, int co
 // Replay of frame t (recorded ancestors, same component draw and update as
 // the forward): writes the updated states of frame t into the segment slot.
 template <int SELJ>
-__global__ void __launch_bounds__(256) k_adj_replay(AdjArgs a)
+__global__ void __launch_bounds__(kAdjTile) k_adj_replay(AdjArgs a)
 {
     extern __shared__ float4 sm4[];
     const BatchArgs &ba = a.b;
@@ -175,7 +178,7 @@ __global__ void __launch_bounds__(256) k_adj_replay(AdjArgs a)
 
 // Children ranges of frame t's resampling: first[f + i] = first child of
 // parent i = lower_bound(anc, i) (ancestors are non-decreasing), i in [0, N].
-__global__ void __launch_bounds__(256) k_adj_first(AdjArgs a)
+__global__ void __launch_bounds__(kAdjTile) k_adj_first(AdjArgs a)
 {
     const BatchArgs &ba = a.b;
     const int s = a.t_seq[blockIdx.x];
@@ -200,7 +203,7 @@ __global__ void __launch_bounds__(256) k_adj_first(AdjArgs a)
 
 // Reverse kernel of frame t.
 template <int SELJ>
-__global__ void __launch_bounds__(256) k_adj_step(AdjArgs a)
+__global__ void __launch_bounds__(kAdjTile) k_adj_step(AdjArgs a)
 {
     extern __shared__ float4 sm4[];
     const BatchArgs &ba = a.b;
@@ -403,7 +406,7 @@ __global__ void __launch_bounds__(256) k_adj_step(AdjArgs a)
 
 // Initial states x_0j = mu'_{bins_j}, P_0j = Sigma'_{bins_j}: the adjoint of
 // frame 0's inputs goes to the components of the reset.
-__global__ void __launch_bounds__(256) k_adj_reset(AdjArgs a)
+__global__ void __launch_bounds__(kAdjTile) k_adj_reset(AdjArgs a)
 {
     extern __shared__ float4 sm4[];
     const BatchArgs &ba = a.b;
