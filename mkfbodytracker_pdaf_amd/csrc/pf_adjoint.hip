@@ -242,7 +242,10 @@ __global__ void __launch_bounds__(kAdjTile) k_adj_step(AdjArgs a)
         // ---- forward recompute, recording each joint's inputs
         constexpr int CS = 2 + D + NP;
         const unsigned long long *sthr = reinterpret_cast<const unsigned long long *>(sm);
-        const float *stab = sm + 4 * K;
+        const float *Hs = sm + 4 * K;
+        const float *BHs = Hs + ba.m * D;
+        const float *Rs = BHs + ba.m;
+        const float *stab = v.comp_global ? v.comp_global : sm + image_head_floats(K, ba.m, D, ba.J);
         const uint32_t U1 = ba.uni[(size_t)(sd.tofs + t) * 2];
         k = draw_component(sthr, K, (uint32_t)j, U1, sd.pi_total);
         const float *ck = stab + k * CS;
@@ -251,9 +254,6 @@ __global__ void __launch_bounds__(kAdjTile) k_adj_step(AdjArgs a)
         for (int i = 0; i < D; ++i) x[i] = fmaf(g, x[i], ck[2 + i]);
 #pragma unroll
         for (int p = 0; p < NP; ++p) P[p] = fmaf(g2, P[p], ck[2 + D + p]);
-        const float *Hs = stab + K * CS;
-        const float *BHs = Hs + ba.m * D;
-        const float *Rs = BHs + ba.m;
         JointRec rec[SELJ];
         bool on[SELJ];
         float lw = 0.f;

@@ -295,7 +295,8 @@ int build_batch_tables(pf_batch *b)
         b->sel_ok[s] = imgs[i].sel_mode;
     }
     const int d = b->D, NP = npacked(d);
-    b->table_floats = (int)tf;
+    b->table_floats = imgs[0].comp_in_lds ? (int)tf : imgs[0].head_floats;
+    b->comp_head = imgs[0].comp_in_lds ? -1 : imgs[0].head_floats;
     std::vector<float> tabs(tf * b->S, 0.f), mus((size_t)b->S * b->K * d), sgs((size_t)b->S * b->K * NP);
     std::vector<double> Ts((size_t)b->S * d * d), Tis((size_t)b->S * d * d);
     for (int i = 0; i < b->S; ++i) {
@@ -360,6 +361,7 @@ BatchArgs batch_args(pf_batch *b, const int32_t *count, const float *cand, const
     a.J = b->J;
     a.cap = b->cap;
     a.table_floats = b->table_floats;
+    a.comp_head = b->comp_head;
     a.stage_floats = b->stage_floats();
     a.parity = parity;
     a.fused = fused;

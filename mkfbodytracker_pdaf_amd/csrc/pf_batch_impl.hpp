@@ -48,6 +48,7 @@ struct BatchArgs {
     const int *count;
     const float *cand;
     double *lml;
+    int comp_head;                // >= 0: component rows read from global memory at tables + table_ofs + comp_head
     int *rec_comp;                // optional record: components of frame t at rec_comp[t * ld + column]
     int *rec_anc;                 // optional record: ancestors of frame t's resampling, same layout
     int S, ld, K, m, J, cap, table_floats, stage_floats, parity, fused, nb_max, all_tiles;
@@ -76,6 +77,7 @@ __device__ __forceinline__ StepArgs seq_view(const BatchArgs &a, const SeqDesc &
     v.b0 = a.b0;
     v.gate = a.gate;
     v.comp_out = nullptr;
+    v.comp_global = a.comp_head >= 0 ? a.tables + sd.table_ofs + a.comp_head : nullptr;
     return v;
 }
 
@@ -98,7 +100,8 @@ struct pf_batch {
     std::vector<mkfbt::HostModel> hm;
     std::vector<char> loaded, sel_ok, fresh;   // fresh[s]: reset since the last run
     unsigned clutter_mask = 0, order_packed = 0;
-    int table_floats = 0;
+    int table_floats = 0;          // floats of each sequence's image staged in LDS
+    int comp_head = -1;            // >= 0: component rows read from global memory (large images)
     int stage_floats() const { return table_floats + mkfbt::cand_lds_floats(J, cap); }
     bool tracks = false;
     pf_batch_schedule sch{256, 1, 2};

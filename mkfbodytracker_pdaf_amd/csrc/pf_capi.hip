@@ -57,7 +57,8 @@ struct pf_handle {
     float *lw = nullptr;
     // model
     float *table = nullptr;
-    int table_floats = 0;
+    int table_floats = 0;          // floats of the image staged in LDS
+    const float *comp_global = nullptr;   // component rows read from global memory (large images)
     unsigned long long pi_total = 0;
     float *mu = nullptr, *sig = nullptr;
     HostModel hm;
@@ -177,6 +178,15 @@ int launch_step_d(pf_handle *h, const StepArgs &a, int mode, int block)
     return launch_step_dj<D, 0>(h, a, mode, block);
 }
 
+// The 24-dimensional state is compiled for 3 and 5 joints in the selection
+// basis (the scaled-state workloads); other joint counts report unsupported.
+int launch_step_d24(pf_handle *h, const StepArgs &a, int mode, int block)
+{
+    if (h->sel_mode && h->J == 3) return launch_step_dj<24, 3>(h, a, mode, block);
+    if (h->sel_mode && h->J == 5) return launch_step_dj<24, 5>(h, a, mode, block);
+    return set_error(PF_ERR_UNSUPPORTED, "d = 24 is compiled for 3 or 5 joints with a full-rank H");
+}
+
 int launch_step(pf_handle *h, const StepArgs &a, int mode)
 {
     int block = h->cfg.block_size > 0 ? h->cfg.block_size : 256;
@@ -184,6 +194,7 @@ int launch_step(pf_handle *h, const StepArgs &a, int mode)
     case 8: return launch_step_d<8>(h, a, mode, block);
     case 12: return launch_step_d<12>(h, a, mode, block);
     case 16: return launch_step_d<16>(h, a, mode, block);
+    case 24: return launch_step_d24(h, a, mode, block);
     default: return set_error(PF_ERR_UNSUPPORTED, "state dimension %d not compiled", h->D);
     }
 }
@@ -195,6 +206,7 @@ int dispatch_d(int D, Args... args)
     case 8: F<8>::run(args...); return PF_OK;
     case 12: F<12>::run(args...); return PF_OK;
     case 16: F<16>::run(args...); return PF_OK;
+    case 24: F<24>::run(args...); return PF_OK;
     default: return set_error(PF_ERR_UNSUPPORTED, "state dimension %d not compiled", D);
     }
 }
@@ -212,8 +224,12 @@ template <int D>
 struct ExportL {
     static void run(pf_handle *h, double *x, double *P)
     {
-        hipLaunchKernelGGL((k_export<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->xA, h->PA,
-                           h->anc, h->Tinv, x, P, h->N, h->ld);
+        if constexpr (D > 16)
+            hipLaunchKernelGGL((k_export_rows<D>), dim3((unsigned)(((long long)h->N * D + 255) / 256)), dim3(256), 0,
+                               h->stream, h->xA, h->PA, h->anc, h->Tinv, x, P, h->N, h->ld);
+        else
+            hipLaunchKernelGGL((k_export<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->xA, h->PA,
+                               h->anc, h->Tinv, x, P, h->N, h->ld);
     }
 };
 
@@ -221,8 +237,12 @@ template <int D>
 struct ImportL {
     static void run(pf_handle *h, const double *x, const double *P)
     {
-        hipLaunchKernelGGL((k_import<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, x, P, h->Tdev,
-                           h->xA, h->PA, h->N, h->ld);
+        if constexpr (D > 16)
+            hipLaunchKernelGGL((k_import_rows<D>), dim3((unsigned)(((long long)h->N * D + 255) / 256)), dim3(256), 0,
+                               h->stream, x, P, h->Tdev, h->xA, h->PA, h->N, h->ld);
+        else
+            hipLaunchKernelGGL((k_import<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, x, P, h->Tdev,
+                               h->xA, h->PA, h->N, h->ld);
     }
 };
 
@@ -312,6 +332,7 @@ StepArgs step_args(pf_handle *h, int mode, const uint32_t *uni, const float *Z, 
     a.cand = cand;
     a.pi_total = h->pi_total;
     a.table_floats = h->table_floats;
+    a.comp_global = h->comp_global;
     a.stage_floats = h->table_floats + (mode == kModeCand ? cand_lds_floats(h->J, h->cap) : 0);
     a.K = h->K;
     a.N = h->N;
@@ -372,7 +393,8 @@ int build_tables(pf_handle *h)
     HIP_TRY(hipMemcpyAsync(h->Tdev, img.T.data(), sizeof(double) * img.T.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->Tinv, img.Ti.data(), sizeof(double) * img.Ti.size(), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    h->table_floats = (int)img.blob.size();
+    h->table_floats = img.comp_in_lds ? (int)img.blob.size() : img.head_floats;
+    h->comp_global = img.comp_in_lds ? nullptr : h->table + img.head_floats;
     return PF_OK;
 }
 }  // namespace

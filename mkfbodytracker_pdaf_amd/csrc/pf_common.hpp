@@ -66,4 +66,15 @@ __host__ __device__ inline u128 sys_target(uint32_t i, uint32_t U, uint64_t tota
     return (u128)((((uint64_t)i) << 32) | U) * total;
 }
 
+// Model image of the step kernels: [component thresholds, 4K floats][H
+// (m x D), BH (m), R 2x2 blocks (3J), padded to 4][component rows,
+// K x (2 + D + D(D+1)/2)].  Floats of the part before the component rows:
+__host__ __device__ constexpr int image_head_floats(int K, int m, int D, int J)
+{
+    return 4 * K + ((m * D + m + 3 * J + 3) & ~3);
+}
+// LDS budget of the staged image: above it only the head is staged and the
+// component rows are read from global memory.
+constexpr int kImageLdsBytes = 48 * 1024;
+
 }  // namespace mkfbt

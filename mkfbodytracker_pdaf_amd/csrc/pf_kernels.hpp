@@ -75,6 +75,7 @@ struct StepArgs {
     unsigned long long pi_total;
     int table_floats;   // floats of the LDS model image (multiple of 4)
     int stage_floats;   // LDS floats before the resolver workspace: image + staged detections (candidate mode)
+    const float *comp_global;  // component rows in global memory when the image head alone is staged, else null
     int K, N, ld, m, J, cap, zld;
     int parity;         // frame parity: selects maxkey[parity]
     int anc_mode;
@@ -337,10 +338,43 @@ struct Joint {
     }
 
     // x += W^T v ; P -= W^T M W with M = [[m00, m01], [m01, m11]].
-    __device__ static __forceinline__ void apply(float (&x)[D], float (&P)[NP], const float (&w0)[D],
-                                                 const float (&w1)[D], float v0, float v1, float m00,
-                                                 float m01, float m11)
+    __device__ static __forceinline__ void apply(float (&x)[D], float (&P)[NP], float (&w0)[D], float (&w1)[D],
+                                                 float v0, float v1, float m00, float m01, float m11)
     {
+        if constexpr (D > 16) {
+            // Large states: M = l1 e1 e1^T + l2 e2 e2^T, so the update is
+            // P -= l1 z1 z1^T + l2 z2 z2^T with z = W^T e overwriting W
+            // (half the temporaries of the W^T (M W) form).
+#pragma unroll
+            for (int c = 0; c < D; ++c) x[c] = fmaf(w0[c], v0, fmaf(w1[c], v1, x[c]));
+            const float h = 0.5f * (m00 - m11), mid = 0.5f * (m00 + m11);
+            const float r = sqrtf(fmaf(h, h, m01 * m01));
+            const float l1 = mid + r, l2 = mid - r;
+            float e0 = 1.f, e1 = 0.f;
+            if (r > 0.f) {
+                const float a0 = h >= 0.f ? h + r : m01, a1 = h >= 0.f ? m01 : r - h;
+                const float inv = 1.f / sqrtf(fmaf(a0, a0, a1 * a1));
+                e0 = a0 * inv;
+                e1 = a1 * inv;
+            }
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const float z1 = fmaf(w0[c], e0, w1[c] * e1);
+                const float z2 = fmaf(w1[c], e0, -w0[c] * e1);
+                w0[c] = z1;
+                w1[c] = z2;
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                const float a1 = l1 * w0[i], a2 = l2 * w1[i];
+#pragma unroll
+                for (int k = i; k < D; ++k) {
+                    float p = fmaf(-a1, w0[k], P[pk(D, i, k)]);
+                    P[pk(D, i, k)] = fmaf(-a2, w1[k], p);
+                }
+            }
+            return;
+        }
         float u0[D], u1[D];
 #pragma unroll
         for (int c = 0; c < D; ++c) {
@@ -535,6 +569,12 @@ __host__ __device__ constexpr int step_lds_bytes(int stage_floats, int nb_pendin
     return 4 * stage_floats + (nb_pending > 0 ? 8 * resolve_lds_words(nb_pending) : 0);
 }
 
+// Model image layout: [component thresholds, 4K floats][H (m x D), BH (m),
+// R 2x2 blocks (3J), padded to 4][component rows, K x (2 + D + D(D+1)/2)].
+// The head is always staged in LDS; the component rows are staged with it
+// when the whole image fits the LDS budget, and read from global memory
+// (StepArgs::comp_global) otherwise.
+
 // One particle's frame: component draw (pf2DRao.cpp:128), mean-reverting
 // predict (KF_model.cpp:11-15) and the joint updates (pf2DRao.cpp:138-140);
 // returns the particle's log-weight.  sm is the staged LDS model image.
@@ -545,7 +585,10 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
     constexpr int NP = npacked(D);
     constexpr int CS = 2 + D + NP;
     const unsigned long long *sthr = reinterpret_cast<const unsigned long long *>(sm);
-    const float *stab = sm + 4 * a.K;
+    const float *Hs = sm + 4 * a.K;
+    const float *BHs = Hs + a.m * D;
+    const float *Rs = BHs + a.m;
+    const float *stab = a.comp_global ? a.comp_global : sm + image_head_floats(a.K, a.m, D, a.J);
     const int k = draw_component(sthr, a.K, (uint32_t)j, U1, a.pi_total);
     if (a.comp_out) a.comp_out[j] = k;
     const float *ck = stab + k * CS;
@@ -555,9 +598,6 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
 #pragma unroll
     for (int p = 0; p < NP; ++p) P[p] = fmaf(g2, P[p], ck[2 + D + p]);
 
-    const float *Hs = stab + a.K * CS;
-    const float *BHs = Hs + a.m * D;
-    const float *Rs = BHs + a.m;
     float acc = 0.f;
     if constexpr (SELJ > 0) {
 #pragma unroll
@@ -859,6 +899,58 @@ __global__ void __launch_bounds__(256) k_export(const float *xs, const float *Ps
             Po[i * D + q] = acc;
             Po[q * D + i] = acc;
         }
+    }
+}
+
+// Row-parallel forms of k_import / k_export for large states: one thread per
+// (particle, row i), FP64, the packed state read from global memory.
+template <int D>
+__global__ void __launch_bounds__(256) k_import_rows(const double *x, const double *P, const double *T, float *xs,
+                                                     float *Ps, int N, int ld)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)N * D) return;
+    const int j = (int)(g / D), i = (int)(g % D);
+    const double *xj = x + (size_t)j * D, *Pj = P + (size_t)j * D * D;
+    double acc = 0.0;
+    for (int k = 0; k < D; ++k) acc = fma(T[i * D + k], xj[k], acc);
+    xs[(size_t)i * ld + j] = (float)acc;
+    double r[D];
+    for (int k = 0; k < D; ++k) {
+        double a = 0.0;
+        for (int l = 0; l < D; ++l) a = fma(T[i * D + l], 0.5 * (Pj[l * D + k] + Pj[k * D + l]), a);
+        r[k] = a;
+    }
+    for (int q = i; q < D; ++q) {
+        double a = 0.0;
+        for (int k = 0; k < D; ++k) a = fma(r[k], T[q * D + k], a);
+        Ps[(size_t)pk(D, i, q) * ld + j] = (float)a;
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_export_rows(const float *xs, const float *Ps, const int *anc,
+                                                     const double *Ti, double *x, double *P, int N, int ld)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)N * D) return;
+    const int j = (int)(g / D), i = (int)(g % D);
+    const int s = anc ? anc[j] : j;
+    double acc = 0.0;
+    for (int k = 0; k < D; ++k) acc = fma(Ti[i * D + k], (double)xs[(size_t)k * ld + s], acc);
+    x[(size_t)j * D + i] = acc;
+    double r[D];
+    for (int k = 0; k < D; ++k) {
+        double a = 0.0;
+        for (int l = 0; l < D; ++l)
+            a = fma(Ti[i * D + l], (double)Ps[(size_t)pk(D, l < k ? l : k, l < k ? k : l) * ld + s], a);
+        r[k] = a;
+    }
+    double *Po = P + (size_t)j * D * D;
+    for (int q = 0; q < D; ++q) {
+        double a = 0.0;
+        for (int k = 0; k < D; ++k) a = fma(r[k], Ti[q * D + k], a);
+        Po[i * D + q] = a;
     }
 }
 

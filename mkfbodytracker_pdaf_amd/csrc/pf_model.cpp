@@ -117,8 +117,8 @@ int host_model_from_args(HostModel &hm, int K, int d, int npose, int m, const in
                          const double *R)
 {
     if (K <= 0 || K > 256) return set_error(PF_ERR_ARG, "K=%d out of range (1..256)", K);
-    if (!(d == 8 || d == 12 || d == 16))
-        return set_error(PF_ERR_UNSUPPORTED, "state dimension %d not supported (8, 12, 16)", d);
+    if (!(d == 8 || d == 12 || d == 16 || d == 24))
+        return set_error(PF_ERR_UNSUPPORTED, "state dimension %d not supported (8, 12, 16, 24)", d);
     static const int32_t ref_sel[6] = {9, 10, 0, 1, 12, 13};  // my_gmm.cpp:61-67
     if (!sel) {
         if (m != 6) return set_error(PF_ERR_ARG, "default selection needs m == 6");
@@ -228,7 +228,9 @@ int make_model_image(const HostModel &hm, unsigned order, int N, ModelImage &img
                 out[(size_t)i * d + j] = acc;
             }
     };
-    std::vector<float> tab((size_t)K * CS + (size_t)m * d + m + 3 * J);
+    const int head = image_head_floats(K, m, d, J);
+    std::vector<float> tab((size_t)K * CS);                     // component rows
+    std::vector<float> hbr((size_t)head - 4 * K, 0.f);         // H, BH, R blocks
     img.mu.assign((size_t)K * d, 0.f);
     img.sg.assign((size_t)K * NP, 0.f);
     std::vector<double> mut(d), sgt((size_t)d * d), sym((size_t)d * d);
@@ -253,7 +255,7 @@ int make_model_image(const HostModel &hm, unsigned order, int N, ModelImage &img
                 img.sg[(size_t)k * NP + pk(d, i, q)] = (float)v;
             }
     }
-    float *Hs = &tab[(size_t)K * CS];
+    float *Hs = hbr.data();
     for (int i = 0; i < m * d; ++i) Hs[i] = (float)hm.H[i];
     for (int a = 0; a < m; ++a) Hs[m * d + a] = (float)hm.BH[a];
     const std::vector<double> &Rm = hm.R;
@@ -272,13 +274,16 @@ int make_model_image(const HostModel &hm, unsigned order, int N, ModelImage &img
         thr[2 * k + 1] = (unsigned long long)(th >> 64);
     }
     img.pi_total = acc;
-    img.blob.assign((size_t)4 * K + tab.size(), 0.f);
+    img.blob.assign((size_t)head + tab.size(), 0.f);
     std::memcpy(img.blob.data(), thr.data(), sizeof(unsigned long long) * thr.size());
-    std::memcpy(img.blob.data() + 4 * K, tab.data(), sizeof(float) * tab.size());
+    std::memcpy(img.blob.data() + 4 * K, hbr.data(), sizeof(float) * hbr.size());
+    std::memcpy(img.blob.data() + head, tab.data(), sizeof(float) * tab.size());
     img.blob.resize((img.blob.size() + 3) & ~(size_t)3, 0.f);
-    if (sizeof(float) * img.blob.size() > 48 * 1024)
-        return set_error(PF_ERR_UNSUPPORTED, "model table of %zu bytes exceeds the 48 KiB LDS budget",
-                         sizeof(float) * img.blob.size());
+    img.head_floats = head;
+    img.comp_in_lds = sizeof(float) * img.blob.size() <= (size_t)kImageLdsBytes;
+    if (sizeof(float) * (size_t)head > (size_t)kImageLdsBytes)
+        return set_error(PF_ERR_UNSUPPORTED, "model image head of %zu bytes exceeds the %d-byte LDS budget",
+                         sizeof(float) * (size_t)head, kImageLdsBytes);
     return PF_OK;
 }
 

@@ -16,11 +16,13 @@ struct HostModel {
 };
 
 struct ModelImage {
-    std::vector<float> blob;        // LDS image: thresholds (4K floats) + component table + H, BH, R
+    std::vector<float> blob;        // image: thresholds (4K floats), H, BH, R, then the component rows
     std::vector<float> mu, sg;      // reset table in the state basis: K x D, K x D(D+1)/2
     std::vector<double> T, Ti;      // state basis x' = T x and its inverse (D x D)
     bool sel_mode = false;          // H is a row selection in the state basis
     unsigned long long pi_total = 0;
+    int head_floats = 0;            // thresholds + H, BH, R (always staged in LDS)
+    bool comp_in_lds = true;        // the component rows fit the LDS budget with the head
 };
 
 // Validates the arguments of my_gmm::loadGaussian x K and fills hm (H = H1
