@@ -172,7 +172,8 @@ __global__ void __launch_bounds__(kAdjTile) k_adj_replay(AdjArgs a)
     float x[D], P[NP];
     load_state(a.in_x, a.in_P, ba.ld, sd.pofs + src, x, P);
     const uint32_t U1 = ba.uni[(size_t)(sd.tofs + a.t) * 2];
-    (void)particle_update<D, kModeCand, SELJ>(v, a.t, j, U1, reinterpret_cast<const float *>(sm4), x, P);
+    (void)particle_update<D, kModeCand, SELJ>(ba.base, a.t, j, U1, reinterpret_cast<const float *>(sm4), x, P,
+                                              sd.pi_total, seq_comp_global(ba, sd), nullptr);
     store_state(a.out_x, a.out_P, ba.ld, sd.pofs + j, x, P);
 }
 
@@ -587,6 +588,7 @@ int pf_batch_lml_grad(pf_batch *b, const int32_t *count, const float *cand, cons
     if (!count || !cand || !uniforms || !lml) return set_error(PF_ERR_ARG, "null run argument");
     if (checkpoint_every < 1) return set_error(PF_ERR_ARG, "checkpoint_every must be >= 1");
     if (b->K > kMaxAdjK) return set_error(PF_ERR_UNSUPPORTED, "adjoint supports K <= %d", kMaxAdjK);
+    if (b->D != D) return set_error(PF_ERR_UNSUPPORTED, "adjoint is compiled for d = %d", D);
     for (int s = 0; s < b->S; ++s)
         if (!b->sel_ok[s]) return set_error(PF_ERR_UNSUPPORTED, "adjoint needs a full-rank measurement matrix");
     if ((size_t)step_lds_bytes(b->stage_floats(), b->nb_max) > (size_t)kMaxLdsBytes)

@@ -67,10 +67,14 @@ __device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc
     }
 }
 
-template <int SELJ, int PPL>
+// PERSIST: a grid of blocks_per_cu x CUs walks the active tiles; otherwise
+// one block per tile (grid = all tiles, the inactive ones exit at once),
+// which keeps the 24-dimensional state's register footprint free of the
+// loop's live values.
+template <int D, int SELJ, int PPL, bool PERSIST = true>
 __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
 {
-    constexpr int D = kBD, NP = npacked(D);
+    constexpr int NP = npacked(D);
     extern __shared__ float4 sm4[];
     const int t = *a.frame;
     const int ntiles = a.n_tiles[t];
@@ -80,10 +84,10 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
     float *xout = odd ? a.buf_x[0] : a.buf_x[1], *Pout = odd ? a.buf_P[0] : a.buf_P[1];
     const unsigned rowb = (unsigned)a.ld * 4u;
     unsigned long long *wsp = reinterpret_cast<unsigned long long *>(reinterpret_cast<float *>(sm4) + a.stage_floats);
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int s = a.tile_seq[tile];
-        const int jt0 = a.tile_j0[tile];
-        const SeqDesc sd = a.seqs[s];
+    for (int tile = blockIdx.x; tile < ntiles; tile += PERSIST ? gridDim.x : ntiles) {
+        const int s = uni_int(a.tile_seq[tile]);
+        const int jt0 = uni_int(a.tile_j0[tile]);
+        const SeqDesc sd = uniform_seq(a.seqs, s);
         int src[PPL];
         if (a.fused) {
             tile_ancestors<PPL>(a, sd, s, t, jt0, wsp, src, a.rec_anc ? a.rec_anc + (size_t)(t - 1) * a.ld : nullptr,
@@ -103,21 +107,28 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
             const int j = jt0 + r * bd + (int)threadIdx.x;
             if (j < sd.N) {
                 const unsigned voff = (unsigned)(sd.pofs + src[r]) * 4u;
+                if constexpr (D > 16) {   // row offsets in the vector offset: no scalar register per row
 #pragma unroll
-                for (int i = 0; i < D; ++i) x[r][i] = soa_ld(rx, voff, rowb * i);
+                    for (int i = 0; i < D; ++i) x[r][i] = soa_ld(rx, voff + rowb * i, 0);
 #pragma unroll
-                for (int p = 0; p < NP; ++p) P[r][p] = soa_ld(rP, voff, rowb * p);
+                    for (int p = 0; p < NP; ++p) P[r][p] = soa_ld(rP, voff + rowb * p, 0);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < D; ++i) x[r][i] = soa_ld(rx, voff, rowb * i);
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) P[r][p] = soa_ld(rP, voff, rowb * p);
+                }
             }
         }
-        StepArgs v = seq_view(a, sd);
-        const float *fcand = v.cand + (size_t)t * a.J * a.cap * 2;
+        const float *fcand = a.cand + ((size_t)sd.tofs + t) * a.J * a.cap * 2;
         CandPrefetch cpf;
-        cand_prefetch(cpf, v.count + (size_t)t * a.J, fcand, a.J, a.cap);
+        cand_prefetch(cpf, a.count + ((size_t)sd.tofs + t) * a.J, fcand, a.J, a.cap);
         __syncthreads();          // the previous tile's LDS readers are done
         stage_table(sm4, reinterpret_cast<const float4 *>(a.tables + (size_t)sd.table_ofs), a.table_floats >> 2);
         cand_store(cpf, reinterpret_cast<float *>(sm4) + a.table_floats, fcand, a.J, a.cap);
         __syncthreads();
-        if (a.rec_comp) v.comp_out = a.rec_comp + (size_t)t * a.ld + sd.pofs;
+        int *comp_out = a.rec_comp ? a.rec_comp + (size_t)t * a.ld + sd.pofs : nullptr;
+        const float *comp_global = seq_comp_global(a, sd);
         const uint32_t U1 = a.uni[(size_t)(sd.tofs + t) * 2];
         float lmax = -INFINITY;
         const __amdgpu_buffer_rsrc_t wx = soa_rsrc(xout, rowb * D), wP = soa_rsrc(Pout, rowb * NP);
@@ -125,13 +136,21 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
         for (int r = 0; r < PPL; ++r) {
             const int j = jt0 + r * bd + (int)threadIdx.x;
             if (j < sd.N) {
-                const float lwj = particle_update<D, kModeCand, SELJ>(v, t, j, U1,
-                                                                      reinterpret_cast<const float *>(sm4), x[r], P[r]);
+                const float lwj = particle_update<D, kModeCand, SELJ>(a.base, t, j, U1,
+                                                                      reinterpret_cast<const float *>(sm4), x[r], P[r],
+                                                                      sd.pi_total, comp_global, comp_out);
                 const unsigned doff = (unsigned)(sd.pofs + j) * 4u;
+                if constexpr (D > 16) {
 #pragma unroll
-                for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[r][i]);
+                    for (int i = 0; i < D; ++i) soa_st(wx, doff + rowb * i, 0, x[r][i]);
 #pragma unroll
-                for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[r][p]);
+                    for (int p = 0; p < NP; ++p) soa_st(wP, doff + rowb * p, 0, P[r][p]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[r][i]);
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[r][p]);
+                }
                 a.lw[sd.pofs + j] = lwj;
                 if (isfinite(lwj)) lmax = fmaxf(lmax, lwj);
             }
@@ -201,9 +220,9 @@ __global__ void __launch_bounds__(256) k_batch_finish(BatchArgs a)
     extern __shared__ float4 sm4[];
     unsigned long long *wsp = reinterpret_cast<unsigned long long *>(sm4);
     for (int tile = blockIdx.x; tile < a.all_tiles; tile += gridDim.x) {
-        const int s = a.tile_seq[tile];
-        const int jt0 = a.tile_j0[tile];
-        const SeqDesc sd = a.seqs[s];
+        const int s = uni_int(a.tile_seq[tile]);
+        const int jt0 = uni_int(a.tile_j0[tile]);
+        const SeqDesc sd = uniform_seq(a.seqs, s);
         int src[PPL];
         tile_ancestors<PPL>(a, sd, s, sd.T - 1, jt0, wsp, src, a.anc,
                             a.rec_anc ? a.rec_anc + (size_t)(sd.T - 1) * a.ld : nullptr, (sd.T - 1) & 1,
@@ -363,6 +382,15 @@ BatchArgs batch_args(pf_batch *b, const int32_t *count, const float *cand, const
     a.table_floats = b->table_floats;
     a.comp_head = b->comp_head;
     a.stage_floats = b->stage_floats();
+    a.base = StepArgs{};
+    a.base.K = b->K;
+    a.base.m = b->m;
+    a.base.J = b->J;
+    a.base.cap = b->cap;
+    a.base.table_floats = b->table_floats;
+    a.base.stage_floats = b->stage_floats();
+    a.base.order_packed = b->order_packed;
+    a.base.clutter_mask = b->clutter_mask;
     a.parity = parity;
     a.fused = fused;
     a.nb_max = b->nb_max;
@@ -373,6 +401,9 @@ BatchArgs batch_args(pf_batch *b, const int32_t *count, const float *cand, const
     a.pdl = (float)(b->pd / b->lambda);
     a.b0 = (float)(1.0 - b->pd * pg);
     a.gate = (float)b->gate;
+    a.base.pdl = a.pdl;
+    a.base.b0 = a.b0;
+    a.base.gate = a.gate;
     return a;
 }
 
@@ -380,7 +411,7 @@ template <int SELJ, int PPL>
 void launch_step_t(pf_batch *b, const BatchArgs &a, size_t lds)
 {
     const int grid = b->sch.blocks_per_cu * b->cus;
-    hipLaunchKernelGGL((k_batch_step<SELJ, PPL>), dim3(grid), dim3(b->sch.block), lds, b->stream, a);
+    hipLaunchKernelGGL((k_batch_step<kBD, SELJ, PPL>), dim3(grid), dim3(b->sch.block), lds, b->stream, a);
 }
 
 int launch_batch_step(pf_batch *b, const BatchArgs &a)
@@ -390,6 +421,17 @@ int launch_batch_step(pf_batch *b, const BatchArgs &a)
     for (int s = 0; s < b->S; ++s) all_sel = all_sel && b->sel_ok[s];
     const int selj = all_sel ? b->J : 0;
     const int ppl = b->sch.ppl;
+    if (b->D == 24) {              // scaled state: 3 or 5 joints in the selection basis, one particle per lane
+        const int grid = b->all_tiles;
+        if (selj == 3)
+            hipLaunchKernelGGL((k_batch_step<24, 3, 1, false>), dim3(grid), dim3(b->sch.block), lds, b->stream, a);
+        else if (selj == 5)
+            hipLaunchKernelGGL((k_batch_step<24, 5, 1, false>), dim3(grid), dim3(b->sch.block), lds, b->stream, a);
+        else
+            return set_error(PF_ERR_UNSUPPORTED, "batched d = 24 is compiled for 3 or 5 joints with a full-rank H");
+        HIP_TRYB(hipGetLastError());
+        return PF_OK;
+    }
 #define PF_BATCH_CASE(J_)                                           \
     case J_:                                                        \
         if (ppl == 2) launch_step_t<J_, 2>(b, a, lds);             \
@@ -522,7 +564,7 @@ int pf_batch_create(pf_batch **out, int32_t S, const int32_t *n_particles, const
     if (!out || !n_particles || !n_frames) return set_error(PF_ERR_ARG, "null argument");
     *out = nullptr;
     if (S <= 0 || S > (1 << 16)) return set_error(PF_ERR_ARG, "S=%d out of range", S);
-    if (d != kBD) return set_error(PF_ERR_UNSUPPORTED, "the batched engine runs d = %d", kBD);
+    if (d != kBD && d != 24) return set_error(PF_ERR_UNSUPPORTED, "the batched engine runs d = 12 or 24");
     if (K <= 0 || K > 256 || m <= 0 || (m & 1) || m > 12) return set_error(PF_ERR_ARG, "bad K/m");
     pf_config c;
     pf_config_default(&c);
@@ -653,7 +695,8 @@ int pf_batch_reset(pf_batch *b, int32_t s, const int32_t *bins)
     for (int j = 0; j < sd.N; ++j)
         if (bins[j] < 0 || bins[j] >= b->K) return set_error(PF_ERR_ARG, "bins[%d] out of range", j);
     HIP_TRYB(hipMemcpyAsync(b->rec_bins + sd.pofs, bins, sizeof(int) * sd.N, hipMemcpyHostToDevice, b->stream));
-    hipLaunchKernelGGL((k_reset<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->rec_bins + sd.pofs,
+    auto reset_kernel = b->D == 24 ? k_reset<24> : k_reset<kBD>;
+    hipLaunchKernelGGL(reset_kernel, dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->rec_bins + sd.pofs,
                        b->mus + (size_t)i * b->K * b->D, b->sigs + (size_t)i * b->K * npacked(b->D),
                        b->bx[0] + sd.pofs, b->bP[0] + sd.pofs, sd.N, b->ld);
     hipLaunchKernelGGL(k_identity_seq, dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->anc, sd.pofs, sd.N);
@@ -714,9 +757,14 @@ int pf_batch_get_tracks(pf_batch *b, int32_t s, double *x, double *P)
         b->gbuf_bytes = sizeof(double) * (nx + np);
     }
     const int buf = sd.T & 1;   // frame f writes buffer (f + 1) & 1; the last frame is T - 1
-    hipLaunchKernelGGL((k_export<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->bx[buf] + sd.pofs,
-                       b->bP[buf] + sd.pofs, b->anc + sd.pofs, b->Tis + (size_t)i * d * d, b->gbuf, b->gbuf + nx,
-                       sd.N, b->ld);
+    if (d == 24)
+        hipLaunchKernelGGL((k_export_rows<24>), dim3((unsigned)(((long long)sd.N * 24 + 255) / 256)), dim3(256), 0,
+                           b->stream, b->bx[buf] + sd.pofs, b->bP[buf] + sd.pofs, b->anc + sd.pofs,
+                           b->Tis + (size_t)i * d * d, b->gbuf, b->gbuf + nx, sd.N, b->ld);
+    else
+        hipLaunchKernelGGL((k_export<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->bx[buf] + sd.pofs,
+                           b->bP[buf] + sd.pofs, b->anc + sd.pofs, b->Tis + (size_t)i * d * d, b->gbuf,
+                           b->gbuf + nx, sd.N, b->ld);
     HIP_TRYB(hipGetLastError());
     if (x) HIP_TRYB(hipMemcpyAsync(x, b->gbuf, sizeof(double) * nx, hipMemcpyDeviceToHost, b->stream));
     if (P) HIP_TRYB(hipMemcpyAsync(P, b->gbuf + nx, sizeof(double) * np, hipMemcpyDeviceToHost, b->stream));
@@ -732,7 +780,8 @@ int pf_batch_estimate(pf_batch *b, int32_t s, double *xbar)
     const int i = b->rank[s];
     const SeqDesc &sd = b->h_seqs[i];
     const int buf = sd.T & 1;
-    hipLaunchKernelGGL((k_estimate<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->bx[buf] + sd.pofs,
+    auto est_kernel = b->D == 24 ? k_estimate<24> : k_estimate<kBD>;
+    hipLaunchKernelGGL(est_kernel, dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->bx[buf] + sd.pofs,
                        b->anc + sd.pofs, sd.N, b->ld, b->partial, b->fs, b->Tis + (size_t)i * b->D * b->D, b->est);
     HIP_TRYB(hipGetLastError());
     HIP_TRYB(hipMemcpyAsync(xbar, b->est, sizeof(double) * b->D, hipMemcpyDeviceToHost, b->stream));
@@ -754,6 +803,7 @@ int pf_batch_set_schedule(pf_batch *b, const pf_batch_schedule *sch)
     if (!sch || !(sch->block == 64 || sch->block == 128 || sch->block == 256) || !(sch->ppl == 1 || sch->ppl == 2) ||
         sch->blocks_per_cu < 1 || sch->blocks_per_cu > 8)
         return set_error(PF_ERR_ARG, "bad schedule");
+    if (b->D == 24 && sch->ppl != 1) return set_error(PF_ERR_ARG, "d = 24 runs one particle per lane");
     b->sch = *sch;
     return build_tiles(b);
 }
@@ -772,7 +822,8 @@ int pf_batch_autotune(pf_batch *b, const int32_t *count, const float *cand, cons
     std::vector<pf_batch_schedule> cands;
     for (int blk : {64, 128, 256})
         for (int ppl : {1, 2})
-            for (int bpc : {1, 2, 3, 4}) cands.push_back({blk, ppl, bpc});
+            for (int bpc : {1, 2, 3, 4})
+                if (b->D != 24 || ppl == 1) cands.push_back({blk, ppl, bpc});
     hipEvent_t e0, e1;
     HIP_TRYB(hipEventCreate(&e0));
     HIP_TRYB(hipEventCreate(&e1));

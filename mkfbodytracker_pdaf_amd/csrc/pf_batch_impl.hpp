@@ -30,6 +30,7 @@ struct SeqDesc {
 };
 
 struct BatchArgs {
+    StepArgs base;                // constants shared by every sequence (K, m, J, cap, PDAF constants, image sizes)
     float *buf_x[2], *buf_P[2];
     int *anc;
     float *lw;
@@ -55,6 +56,33 @@ struct BatchArgs {
     unsigned int order_packed, clutter_mask;
     float pdl, b0, gate;
 };
+
+// A block-uniform descriptor read from global memory, moved to scalar
+// registers (the loads are vector loads, so the values would otherwise
+// occupy vector registers in every lane).
+__device__ __forceinline__ int uni_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ SeqDesc uniform_seq(const SeqDesc *seqs, int s)
+{
+    const SeqDesc d = seqs[s];
+    SeqDesc r;
+    r.pofs = uni_int(d.pofs);
+    r.N = uni_int(d.N);
+    r.T = uni_int(d.T);
+    r.tofs = uni_int(d.tofs);
+    r.bofs = uni_int(d.bofs);
+    r.nb = uni_int(d.nb);
+    r.table_ofs = uni_int(d.table_ofs);
+    r.pad = 0;
+    const unsigned lo = (unsigned)uni_int((int)(unsigned)d.pi_total);
+    const unsigned hi = (unsigned)uni_int((int)(unsigned)(d.pi_total >> 32));
+    r.pi_total = ((unsigned long long)hi << 32) | lo;
+    return r;
+}
+
+__device__ __forceinline__ const float *seq_comp_global(const BatchArgs &a, const SeqDesc &sd)
+{
+    return a.comp_head >= 0 ? a.tables + sd.table_ofs + a.comp_head : nullptr;
+}
 
 // Per-sequence view of the arguments in the single-filter layout, so the
 // per-particle update is shared with k_step.

@@ -578,9 +578,14 @@ __host__ __device__ constexpr int step_lds_bytes(int stage_floats, int nb_pendin
 // One particle's frame: component draw (pf2DRao.cpp:128), mean-reverting
 // predict (KF_model.cpp:11-15) and the joint updates (pf2DRao.cpp:138-140);
 // returns the particle's log-weight.  sm is the staged LDS model image.
+// The shared constants come from a (a kernel argument, read on demand); the
+// per-sequence values pi_total (component thresholds' total), comp_global
+// (component rows outside LDS, or null) and comp_out (optional component
+// record) are passed explicitly.
 template <int D, int MODE, int SELJ>
 __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j, uint32_t U1, const float *sm,
-                                                 float (&x)[D], float (&P)[npacked(D)])
+                                                 float (&x)[D], float (&P)[npacked(D)], unsigned long long pi_total,
+                                                 const float *comp_global, int *comp_out)
 {
     constexpr int NP = npacked(D);
     constexpr int CS = 2 + D + NP;
@@ -588,9 +593,9 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
     const float *Hs = sm + 4 * a.K;
     const float *BHs = Hs + a.m * D;
     const float *Rs = BHs + a.m;
-    const float *stab = a.comp_global ? a.comp_global : sm + image_head_floats(a.K, a.m, D, a.J);
-    const int k = draw_component(sthr, a.K, (uint32_t)j, U1, a.pi_total);
-    if (a.comp_out) a.comp_out[j] = k;
+    const float *stab = comp_global ? comp_global : sm + image_head_floats(a.K, a.m, D, a.J);
+    const int k = draw_component(sthr, a.K, (uint32_t)j, U1, pi_total);
+    if (comp_out) comp_out[j] = k;
     const float *ck = stab + k * CS;
     const float g = ck[0], g2 = ck[1];
 #pragma unroll
@@ -681,7 +686,8 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
 
     float lwj = -INFINITY;
     if (live) {
-        lwj = particle_update<D, MODE, SELJ>(a, t, j, U1, reinterpret_cast<const float *>(sm4), x, P);
+        lwj = particle_update<D, MODE, SELJ>(a, t, j, U1, reinterpret_cast<const float *>(sm4), x, P, a.pi_total,
+                                             a.comp_global, a.comp_out);
         const unsigned doff = (unsigned)j * 4u;
         const __amdgpu_buffer_rsrc_t wx = soa_rsrc(a.xout, rowb * D), wP = soa_rsrc(a.Pout, rowb * NP);
 #pragma unroll

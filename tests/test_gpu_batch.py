@@ -124,7 +124,7 @@ def test_batch_autotune_then_run(gpu, problem):
 def test_batch_errors(gpu):
     from mkfbodytracker_pdaf_amd.capi import PfError
     with pytest.raises(PfError):
-        RbpfBatch([100], [5], K=15, m=6, d=8)           # the batched engine runs d = 12
+        RbpfBatch([100], [5], K=15, m=6, d=8)           # the batched engine runs d = 12 or 24
     with pytest.raises(PfError):
         RbpfBatch([0], [5], K=15, m=6)
     b = RbpfBatch([100, 50], [5, 3], K=15, m=6)
@@ -137,4 +137,43 @@ def test_batch_errors(gpu):
     b.load_model(1, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
     with pytest.raises(PfError):
         b.reset(1, np.full(50, 15, np.int32))           # component index out of range
+    b.close()
+
+
+def test_batch_d24_matches_single_runs(gpu):
+    """The 24-dimensional state (5 joints, K = 64: component rows read from
+    global memory) through the batched engine equals independent runs."""
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    joints = ("head", "left_hand", "right_hand", "left_elbow", "right_elbow")
+    Ns, Ts, cap, lam = [700, 1300], [6, 4], 16, 10.0 / (640 * 480)
+    models = [synthetic.make_model(seed=90 + s, K=64, d=24, joints=joints, clutter=(1, 1, 1, 1, 1))
+              for s in range(2)]
+    seqs = [synthetic.simulate(models[s], Ts[s], seed=70 + s, cand_per_joint=11, cap=cap) for s in range(2)]
+    bins = [oracle.resample_linear(models[s].weights, Ns[s], 40 + s)[0] for s in range(2)]
+    ref = []
+    for s, mdl in enumerate(models):
+        rb = Rbpf(Ns[s], cand_capacity=cap, clutter_density=lam)
+        rb.load_model(mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, sel=mdl.sel, R=mdl.R)
+        rb.set_joint_clutter(mdl.clutter)
+        rb.reset(bins[s])
+        d = [DeviceArray(seqs[s].count.astype(np.int32)), DeviceArray(seqs[s].cand.astype(np.float32)),
+             DeviceArray(seqs[s].uniforms.astype(np.uint32)), DeviceArray.empty(Ts[s], np.float64)]
+        rb.run_candidates_dev(Ts[s], d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, 1)
+        rb.synchronize()
+        ref.append((*rb.get_tracks(), d[3].get()))
+        rb.close()
+    b = RbpfBatch(Ns, Ts, K=64, m=10, d=24, cand_capacity=cap, clutter_density=lam)
+    for s, mdl in enumerate(models):
+        b.load_model(s, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
+    b.set_joint_clutter(models[0].clutter)
+    for s in range(2):
+        b.reset(s, bins[s])
+    d = _batch_inputs(seqs)
+    b.run_dev(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr)
+    lml = d[3].get()
+    offs = b.frame_offsets
+    for s in range(2):
+        x, P = b.get_tracks(s)
+        assert np.array_equal(x, ref[s][0]) and np.array_equal(P, ref[s][1]), s
+        assert np.array_equal(lml[offs[s]:offs[s] + Ts[s]], ref[s][2]), s
     b.close()
