@@ -21,7 +21,7 @@ PF_ERR_STATE = -4
 PF_ERR_UNSUPPORTED = -5
 PF_ERR_IO = -6
 
-# Every symbol declared in include/pf_rbpf.h, include/pf2d.h and include/pf_batch.h.
+# Every symbol declared in include/pf_rbpf.h, pf2d.h, pf_batch.h and pf_tracker.h.
 EXPORTS = [
     "pf_config_default", "pf_last_error", "pf_create", "pf_destroy", "pf_load_model",
     "pf_set_joint_clutter", "pf_reset", "pf_set_tracks", "pf_get_tracks", "pf_step",
@@ -35,6 +35,9 @@ EXPORTS = [
     "pf_batch_reset", "pf_batch_run_dev", "pf_batch_get_tracks", "pf_batch_estimate",
     "pf_batch_get_schedule", "pf_batch_set_schedule", "pf_batch_autotune", "pf_batch_last_run_ms",
     "pf_batch_lml_grad", "pf_batch_get_choices", "pf_batch_last_adjoint_ms",
+    "pf_tracker_config_default", "pf_tracker_create", "pf_tracker_destroy", "pf_tracker_reset",
+    "pf_tracker_frame", "pf_tracker_run_dev", "pf_tracker_synchronize", "pf_tracker_last_run_ms",
+    "pf_tracker_get_last", "pf_tracker_pose", "pf_tracker_blur", "pf_tracker_num_proposals",
 ]
 
 
@@ -55,6 +58,13 @@ class Config(C.Structure):
     _fields_ = [("n_particles", C.c_int32), ("device", C.c_int32), ("cand_capacity", C.c_int32),
                 ("block_size", C.c_int32), ("p_detect", C.c_double), ("gate_chi2", C.c_double),
                 ("clutter_density", C.c_double)]
+
+
+class TrackerConfig(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("proposals_per_particle", C.c_int32),
+                ("p_assoc", C.c_double), ("z_floor", C.c_double), ("spread", C.c_double),
+                ("blur_ksize", C.c_int32), ("blur_sigma", C.c_double), ("fx", C.c_double),
+                ("fy", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("seed", C.c_uint64)]
 
 
 _lib = None
@@ -128,6 +138,19 @@ def lib():
     _sig(L, "pf_batch_lml_grad", C.c_int, _vp, _vp, _vp, _vp, _vp, i32, _vp, _vp, _vp, _vp)
     _sig(L, "pf_batch_get_choices", C.c_int, _vp, i32, _vp, _vp)
     _sig(L, "pf_batch_last_adjoint_ms", C.c_int, _vp, C.POINTER(dbl), C.POINTER(dbl))
+    _sig(L, "pf_tracker_config_default", None, C.POINTER(TrackerConfig))
+    _sig(L, "pf_tracker_create", C.c_int, C.POINTER(_vp), C.POINTER(TrackerConfig), _vp, _vp, i32, _vp, _vp,
+         _vp, _vp)
+    _sig(L, "pf_tracker_destroy", C.c_int, _vp)
+    _sig(L, "pf_tracker_reset", C.c_int, _vp)
+    _sig(L, "pf_tracker_frame", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp)
+    _sig(L, "pf_tracker_run_dev", C.c_int, _vp, i32, _vp, _vp, _vp, _vp, _vp, i32)
+    _sig(L, "pf_tracker_synchronize", C.c_int, _vp)
+    _sig(L, "pf_tracker_last_run_ms", C.c_int, _vp, C.POINTER(dbl))
+    _sig(L, "pf_tracker_get_last", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp)
+    _sig(L, "pf_tracker_pose", C.c_int, _vp, _vp, _vp)
+    _sig(L, "pf_tracker_blur", C.c_int, _vp, _vp, _vp)
+    _sig(L, "pf_tracker_num_proposals", C.c_int, _vp, C.POINTER(i32))
     _lib = L
     return L
 

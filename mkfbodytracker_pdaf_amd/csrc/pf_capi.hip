@@ -15,6 +15,7 @@
 
 #include "../../include/pf_rbpf.h"
 #include "pf_internal.hpp"
+#include "pf_engine.hpp"
 #include "pf_kernels.hpp"
 #include "pf_model.hpp"
 
@@ -289,14 +290,17 @@ int launch_weights(pf_handle *h, const uint32_t *uni, int parity)
     return check_launch("k_weights");
 }
 
-// Materialise the pending resampling of L weights into N ancestors.
-int launch_ancestors(pf_handle *h, int L, int N, int *out, int parity, int kind_linear, double *lml)
+// Materialise the pending resampling of L weights into N ancestors.  The log
+// mean weight goes to lml, or to lml[*lml_frame] when a frame counter is given.
+int launch_ancestors(pf_handle *h, int L, int N, int *out, int parity, int kind_linear, double *lml,
+                     const int *lml_frame = nullptr)
 {
     AncArgs an{};
     an.pend = pending_of(h, L);
     an.fs = h->fs;
     an.anc = out;
     an.lml = lml;
+    an.lml_frame = lml_frame;
     an.N = N;
     an.parity = parity;
     an.kind_linear = kind_linear;
@@ -398,6 +402,89 @@ int build_tables(pf_handle *h)
     return PF_OK;
 }
 }  // namespace
+
+// ------------------------------------------- interface for chained handles
+namespace mkfbt {
+namespace eng {
+
+int info(pf_handle *h, Info *out)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    out->N = h->N;
+    out->D = h->D;
+    out->m = h->m;
+    out->J = h->J;
+    out->npose = h->npose;
+    out->device = h->cfg.device;
+    out->model = h->model;
+    out->tracks = h->tracks;
+    out->stream = h->stream;
+    return PF_OK;
+}
+
+int reserve_resample(pf_handle *h, int L) { return ensure_resample_scratch(h, L); }
+
+const double *estimate_ptr(pf_handle *h) { return h->est_out; }
+
+unsigned long long *maxkey_ptr(pf_handle *h, int parity) { return &h->fs->maxkey[parity & 1]; }
+
+int set_frame(pf_handle *h, int f)
+{
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, f);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame + 1, f);
+    return check_launch("k_set_int");
+}
+
+int estimate(pf_handle *h)
+{
+    int st = dispatch_d<EstimateL>(h->D, h);
+    if (st) return st;
+    return check_launch("k_estimate");
+}
+
+int resample_linear(pf_handle *h, const double *w, int L, int N, int parity, const int *frame,
+                    const uint32_t *uni, int ustride, int uoff, int *out)
+{
+    if (L > h->Lcap) return set_error(PF_ERR_STATE, "resampler scratch holds %d weights, %d asked", h->Lcap, L);
+    WeightArgs wa{};
+    wa.w64 = w;
+    wa.cloc = h->cloc;
+    wa.bsum = h->bsum;
+    wa.bexp = h->bexp;
+    wa.fs = h->fs;
+    wa.frame = frame;
+    wa.frame_next = nullptr;
+    wa.uni = uni;
+    wa.L = L;
+    wa.parity = parity & 1;
+    wa.ustride = ustride;
+    wa.uoff = uoff;
+    const int nb = (L + kWeightsBlock - 1) / kWeightsBlock;
+    hipLaunchKernelGGL((k_weights<kLinF64>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, wa);
+    int st = check_launch("k_weights");
+    if (st) return st;
+    return launch_ancestors(h, L, N, out, parity & 1, 1, nullptr);
+}
+
+int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, double *lml)
+{
+    const int p = parity & 1;
+    StepArgs a = step_args(h, kModeRef, uni, Z, 0, h->N, nullptr, nullptr, nullptr, p, kAncArray, nullptr);
+    int st = launch_step(h, a, kModeRef);
+    if (st) return st;
+    st = launch_weights(h, uni, p);
+    if (st) return st;
+    st = launch_ancestors(h, h->N, h->N, h->anc, p, 0, lml, h->frame + p);
+    if (st) return st;
+    swap_state(h);
+    return PF_OK;
+}
+
+int take_status(pf_handle *h, int *degenerate) { return read_status(h, degenerate); }
+
+}  // namespace eng
+}  // namespace mkfbt
 
 // =================================================================== C ABI
 extern "C" {
@@ -836,7 +923,9 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
         s = launch_weights(h, uniforms, parity);
         if (s) return s;
         if (!fused && f + 1 < T) {
-            s = launch_ancestors(h, h->N, h->N, h->anc, parity, 0, lml + f);
+            // counter f & 1 still holds this frame's index (k_weights wrote
+            // the next one into the other counter)
+            s = launch_ancestors(h, h->N, h->N, h->anc, parity, 0, lml, h->frame + parity);
             if (s) return s;
         }
         swap_state(h);

@@ -77,4 +77,26 @@ __host__ __device__ constexpr int image_head_floats(int K, int m, int D, int J)
 // component rows are read from global memory.
 constexpr int kImageLdsBytes = 48 * 1024;
 
+// Ordered 64-bit key of a double (unsigned comparison = numeric order), for
+// maxima taken with 64-bit atomicMax, and its inverse.
+__device__ __forceinline__ unsigned long long order_key(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double key_value(unsigned long long k)
+{
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)b);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 }  // namespace mkfbt

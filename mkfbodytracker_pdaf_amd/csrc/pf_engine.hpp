@@ -1,0 +1,49 @@
+// pf_engine.hpp -- internal interface of one filter handle for code that
+// chains several handles on the device (the pose tracker of pf_tracker.hip).
+//
+// Every call enqueues on the handle's own stream, never synchronises the host
+// and allocates nothing (except reserve_resample), so a caller may record the
+// calls into a HIP graph.  The handle stays opaque here.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pf_rbpf.h"
+
+namespace mkfbt {
+namespace eng {
+
+struct Info {
+    int N, D, m, J, npose, device;
+    bool model, tracks;
+    hipStream_t stream;
+};
+
+int info(pf_handle *h, Info *out);
+// Resampler scratch for weight vectors of up to L entries (allocates).
+int reserve_resample(pf_handle *h, int L);
+// Device address of the estimate written by estimate(): D doubles, model basis.
+const double *estimate_ptr(pf_handle *h);
+// Address of the handle's largest-weight slot `parity` (an ordered 64-bit key,
+// raised with atomicMax by the producer of the weights resample_linear reads).
+unsigned long long *maxkey_ptr(pf_handle *h, int parity);
+// Sets both frame counters of the handle to f.
+int set_frame(pf_handle *h, int f);
+// getEstimator (pf2DRao.cpp:23-31) of the current tracks into estimate_ptr().
+int estimate(pf_handle *h);
+// resample(w, N) (pf2DRao.cpp:175-210) of L linear weights already on the
+// device whose maximum key is in maxkey slot `parity`; the uniform is
+// uni[*frame * ustride + uoff].  Writes N indices to out.
+int resample_linear(pf_handle *h, const double *w, int L, int N, int parity, const int *frame,
+                    const uint32_t *uni, int ustride, int uoff, int *out);
+// ParticleFilter::update (pf2DRao.cpp:125-158) with one measurement column per
+// particle: Z (m x N floats, leading dimension N), uniforms uni[2t], uni[2t+1]
+// for t = the handle's frame counter `parity`; the frame's log mean weight goes
+// to lml[t].  Ancestors are materialised (estimate() may follow).
+int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, double *lml);
+// Reads and clears the handle's status word (synchronises the handle's stream).
+int take_status(pf_handle *h, int *degenerate);
+
+}  // namespace eng
+}  // namespace mkfbt

@@ -86,26 +86,6 @@ struct StepArgs {
     float gate;   // chi-square gate
 };
 
-__device__ __forceinline__ unsigned long long order_key(double v)
-{
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
-__device__ __forceinline__ double key_value(unsigned long long k)
-{
-    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
-    return __longlong_as_double((long long)b);
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_max(T v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {
@@ -784,6 +764,7 @@ struct AncArgs {
     FrameScratch *fs;
     int *anc;
     double *lml;     // optional: written with the frame's log mean weight
+    const int *lml_frame;   // optional: the slot is lml[*lml_frame] (read at run time, graph replays)
     int N, parity, kind_linear;
 };
 
@@ -795,7 +776,9 @@ static __global__ void __launch_bounds__(256) k_ancestors(AncArgs a)
     double es = 0.0;
     const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0, es);
     if (j < a.N) a.anc[j] = src;
-    if (blockIdx.x == 0 && threadIdx.x == 0) finish_resampling(a.fs, a.parity, es, a.N, a.kind_linear, a.lml);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        finish_resampling(a.fs, a.parity, es, a.N, a.kind_linear,
+                          (a.lml && a.lml_frame) ? a.lml + *a.lml_frame : a.lml);
 }
 
 // block maximum of FP64 host weights (for the stand-alone resample entry points)

@@ -1,0 +1,872 @@
+// pf_tracker.hip -- the pose tracker's per-frame pipeline (include/pf_tracker.h)
+// on gfx950: PFTracker::getMeasurementProposal (src/pfPose.cpp:210-329), the
+// two arm updates it ends with, and the pose output of callback
+// (pfPose.cpp:347-348) with get3Dpose (pfPose.cpp:93-127).
+//
+// Kernels, per frame:
+//   k_blur        separable Gaussian of the likelihood image, LDS-tiled
+//   k_propose     2 proposals per lane for one arm: Philox words -> position,
+//                 association weight against the blurred image and both arms'
+//                 proposal densities; block maxima for the resampler
+//   (resampler)   the filter library's systematic resampling of the n_c
+//                 weights into N proposal indices (per arm, own stream)
+//   k_build_meas  the 6 x N measurement from the resampled indices; the
+//                 proposal positions are recomputed from their Philox words
+//                 rather than stored (nothing but the weights reaches HBM)
+//   (filter)      update + ancestors + estimate of each arm (own stream)
+//   k_pose        e = pca_proj^T xbar + pca_mean and get3Dpose for both arms
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/pf_tracker.h"
+#include "pf_common.hpp"
+#include "pf_engine.hpp"
+#include "pf_internal.hpp"
+#include "pf_rng.hpp"
+
+using namespace mkfbt;
+
+#define TR_TRY(expr)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_error(PF_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+    } while (0)
+
+namespace {
+
+constexpr uint32_t kStreamProposal = 16;  // + arm
+constexpr uint32_t kStreamFrame = 24;     // 8 words per frame
+constexpr int kMaxRadius = 15;
+constexpr int kBlurW = 64, kBlurH = 32;
+constexpr int kProposeBlock = 256;
+constexpr int kMaxProposals = 1 << 20;    // resampler scan limit
+
+// Frame record written by k_propose: what k_build_meas and the inspection
+// call need to recompute a proposal.
+struct FrameRec {
+    double y[4];     // projected hand mean of arm 0 (u, v), arm 1 (u, v)
+    int roi[4];
+    long long g;     // global frame index
+    int uniform;     // first-frame (uniform) proposals
+    int pad;
+};
+
+struct ArmDev {
+    const double *xbar;       // estimate, model basis (D)
+    const double *hp;         // npose x D
+    const double *pm;         // npose
+    double *w;                // n_c association weights
+    unsigned long long *maxkey;
+    int D;
+};
+
+// ------------------------------------------------------------------ blur
+struct BlurArgs {
+    const uint8_t *src;
+    uint8_t *dst;
+    const float *taps;        // 2 radius + 1
+    const int *frame;         // source frame = *frame (nullptr: 0)
+    long long frame_stride;
+    int rows, cols, radius;
+};
+
+__device__ __forceinline__ int reflect101(int i, int n)
+{
+    i = i < 0 ? -i : i;
+    i = i >= n ? 2 * n - 2 - i : i;
+    return min(max(i, 0), n - 1);
+}
+
+// GaussianBlur(ksize, sigma, BORDER_REFLECT_101) (pfPose.cpp:213): row pass
+// then column pass, each an FP32 fma chain over the taps in index order,
+// round to nearest even and saturate to uint8.  One 64 x 32 output tile per
+// block, source tile and row-pass results in LDS.
+__global__ void __launch_bounds__(256) k_blur(BlurArgs a)
+{
+    __shared__ uint8_t s_in[(kBlurH + 2 * kMaxRadius) * (kBlurW + 2 * kMaxRadius)];
+    __shared__ float s_row[(kBlurH + 2 * kMaxRadius) * kBlurW];
+    __shared__ float s_tap[2 * kMaxRadius + 1];
+    const int R = a.radius, IW = kBlurW + 2 * R, IH = kBlurH + 2 * R, NT = 2 * R + 1;
+    const int x0 = blockIdx.x * kBlurW, y0 = blockIdx.y * kBlurH;
+    const long long f = a.frame ? (long long)*a.frame : 0;
+    const uint8_t *src = a.src + f * a.frame_stride;
+    if ((int)threadIdx.x < NT) s_tap[threadIdx.x] = a.taps[threadIdx.x];
+    for (int e = threadIdx.x; e < IH * IW; e += blockDim.x) {
+        const int iy = e / IW, ix = e - iy * IW;
+        const int gy = reflect101(y0 + iy - R, a.rows), gx = reflect101(x0 + ix - R, a.cols);
+        s_in[e] = src[(size_t)gy * a.cols + gx];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < IH * kBlurW; e += blockDim.x) {
+        const int iy = e / kBlurW, ox = e - iy * kBlurW;
+        const uint8_t *p = s_in + iy * IW + ox;
+        float acc = 0.f;
+        for (int k = 0; k < NT; ++k) acc = fmaf(s_tap[k], (float)p[k], acc);
+        s_row[e] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kBlurH * kBlurW; e += blockDim.x) {
+        const int oy = e / kBlurW, ox = e - oy * kBlurW;
+        const int gy = y0 + oy, gx = x0 + ox;
+        if (gy >= a.rows || gx >= a.cols) continue;
+        const float *p = s_row + oy * kBlurW + ox;
+        float acc = 0.f;
+        for (int k = 0; k < NT; ++k) acc = fmaf(s_tap[k], p[k * kBlurW], acc);
+        a.dst[(size_t)gy * a.cols + gx] = (uint8_t)fminf(fmaxf(rintf(acc), 0.f), 255.f);
+    }
+}
+
+// --------------------------------------------------------------- proposals
+struct PropParams {
+    uint64_t seed;
+    long long g0;             // global index of frame t = 0 of this run
+    long long uniform_frame;  // global index of the uniform-proposal frame (-1: none)
+    const int *roi;           // frames x 4
+    const int *frame;         // t
+    int n_c, rows, cols;
+    double spread, pa, zfloor;
+};
+
+// y = Hp[r] . xbar + M[r], products accumulated in index order.
+__device__ inline double project_row(const double *hp, const double *pm, const double *xbar, int D, int r)
+{
+#pragma clang fp contract(off)
+    double s = 0.0;
+    for (int k = 0; k < D; ++k) s = s + hp[r * D + k] * xbar[k];
+    return s + pm[r];
+}
+
+// Proposal c of arm `arm` in frame g: getSamples (pf2DRao.cpp:85-103) or the
+// uniform first-frame box (pfPose.cpp:222-235).
+__device__ inline void proposal(const PropParams &p, int arm, long long g, bool uniform, const int *roi, double y0,
+                                double y1, long long c, double &px, double &py)
+{
+    uint32_t wd[4];
+    philox_stream_block(p.seed, kStreamProposal + arm, (uint64_t)g * (uint64_t)(p.n_c / 2) + (uint64_t)(c >> 1), wd);
+    const uint32_t w0 = wd[2 * (c & 1)], w1 = wd[2 * (c & 1) + 1];
+    if (uniform) {
+        const long long xo = roi[0], yo = roi[1], wi = roi[2], he = roi[3];
+        const long long x_lo = xo - 4 * wi, x_hi = xo + 5 * wi, y_lo = yo + he, y_hi = yo + 7 * he;
+        const double xmin = (double)(x_lo > 0 ? x_lo : 0), xmax = (double)(x_hi < p.cols ? x_hi : p.cols);
+        const double ymin = (double)(y_lo < p.rows ? y_lo : p.rows), ymax = (double)(y_hi < p.rows ? y_hi : p.rows);
+        px = fma(xmax - xmin, unit_interval(w0), xmin);
+        py = fma(ymax - ymin, unit_interval(w1), ymin);
+    } else {
+        const double sd = p.spread * (double)roi[2];
+        double z0, z1;
+        normal_pair(w0, w1, z0, z1);
+        px = fma(sd, z0, y0);
+        py = fma(sd, z1, y1);
+    }
+}
+
+// N((px, py); y, var I) (pf2DRao.cpp:69-83,105-122): getSampleProb takes
+// spread x width as the variance, while getSamples draws with it as the
+// standard deviation (pf2DRao.cpp:90 vs :114).
+__device__ inline double density(double px, double py, double y0, double y1, double var, double inv2piv)
+{
+#pragma clang fp contract(off)
+    const double d0 = px - y0, d1 = py - y1;
+    const double q = fma(d0, d0, d1 * d1);
+    return inv2piv * det_exp((-0.5 * q) / var);
+}
+
+__global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev a0, ArmDev a1, const uint8_t *img,
+                                                          FrameRec *rec)
+{
+    __shared__ double s_y[4];
+    __shared__ double s_max[kProposeBlock / 64];
+    const int t = *p.frame;
+    const long long g = p.g0 + t;
+    const bool uniform = (g == p.uniform_frame);
+    const int *roi = p.roi + 4 * (size_t)t;
+    const int arm = blockIdx.y;
+    if (threadIdx.x < 4) {
+        const ArmDev &A = (threadIdx.x >> 1) ? a1 : a0;
+        s_y[threadIdx.x] = project_row(A.hp, A.pm, A.xbar, A.D, threadIdx.x & 1);
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && arm == 0 && threadIdx.x < 4) {
+        rec->y[threadIdx.x] = s_y[threadIdx.x];
+        rec->roi[threadIdx.x] = roi[threadIdx.x];
+        if (threadIdx.x == 0) {
+            rec->g = g;
+            rec->uniform = uniform ? 1 : 0;
+        }
+    }
+    const double yo0 = s_y[2 * arm], yo1 = s_y[2 * arm + 1];
+    const double yt0 = s_y[2 * (1 - arm)], yt1 = s_y[2 * (1 - arm) + 1];
+    const double var = p.spread * (double)roi[2];
+    const double inv2piv = 1.0 / (6.283185307179586 * var);
+    const long long k = (long long)blockIdx.x * kProposeBlock + threadIdx.x;
+    double wv[2] = {0.0, 0.0};
+    if (2 * k < p.n_c) {
+        for (int i = 0; i < 2; ++i) {
+            double px, py;
+            proposal(p, arm, g, uniform, roi, yo0, yo1, 2 * k + i, px, py);
+            // pfPose.cpp:251-266 (arm 1) and :274-289 (arm 2)
+            if (py > 0.0 && py < (double)p.rows && px > 0.0 && px < (double)p.cols) {
+                const unsigned v = img[(size_t)(int)py * p.cols + (int)px];
+                if (v != 0u) {
+                    const double L = (double)v / 255.0;
+                    const double po = density(px, py, yo0, yo1, var, inv2piv);
+                    const double pt = density(px, py, yt0, yt1, var, inv2piv);
+                    wv[i] = L / fma(p.pa, po, fma(p.pa, pt, p.zfloor));
+                }
+            }
+        }
+        double2 *dst = reinterpret_cast<double2 *>((arm ? a1.w : a0.w) + 2 * k);
+        *dst = make_double2(wv[0], wv[1]);
+    }
+    double m = fmax(wv[0], wv[1]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = s_max[0];
+        for (int w = 1; w < kProposeBlock / 64; ++w) b = fmax(b, s_max[w]);
+        if (b > 0.0 && isfinite(b)) atomicMax(arm ? a1.maxkey : a0.maxkey, order_key(b));
+    }
+}
+
+// Measurement of arm `arm` (pfPose.cpp:303-320), one lane per particle.
+__global__ void __launch_bounds__(256) k_build_meas(PropParams p, int arm, const int *bins, const FrameRec *rec,
+                                                    float *Z, int N)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const int *roi = rec->roi;
+    double px, py;
+    proposal(p, arm, rec->g, rec->uniform != 0, roi, rec->y[2 * arm], rec->y[2 * arm + 1], bins[j], px, py);
+    const double fx = (double)roi[0] + (double)roi[2] / 2.0;
+    Z[j] = (float)fx;
+    Z[(size_t)N + j] = (float)((double)roi[1] + 0.5 * (double)roi[3]);
+    Z[2 * (size_t)N + j] = (float)px;
+    Z[3 * (size_t)N + j] = (float)py;
+    Z[4 * (size_t)N + j] = (float)fx;
+    Z[5 * (size_t)N + j] = (float)((double)roi[1] + 1.65 * (double)roi[3]);
+}
+
+// All proposals of the recorded frame (inspection only): out 2 x 2 x n_c.
+__global__ void __launch_bounds__(256) k_dump_proposals(PropParams p, const FrameRec *rec, double *out)
+{
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int arm = blockIdx.y;
+    if (c >= p.n_c) return;
+    double px, py;
+    proposal(p, arm, rec->g, rec->uniform != 0, rec->roi, rec->y[2 * arm], rec->y[2 * arm + 1], c, px, py);
+    out[(size_t)arm * 2 * p.n_c + c] = px;
+    out[(size_t)arm * 2 * p.n_c + p.n_c + c] = py;
+}
+
+// -------------------------------------------------------------------- pose
+struct PoseArgs {
+    ArmDev arm[2];
+    int *frame;               // t (incremented at the end)
+    double *pose;             // frames x 2 x npose (nullable)
+    double *pos3d;            // frames x 2 x 15 (nullable)
+    int npose;
+    double fx, fy, cx, cy;
+};
+
+// get3Dpose (pfPose.cpp:93-127): joint k of e at pixel (u, v) and depth d,
+// X = (K R)^-1 ((u d, v d, d) - K t) = R^T ((d (u - cx)/fx, d (v - cy)/fy, d) - t)
+// with R = rpy(e15..17) (pfPose.cpp:84-91) and t = e18..20.
+__device__ void pose3d(const double *e, double fx, double fy, double cx, double cy, double *out)
+{
+    const double roll = e[16], pitch = e[17], yaw = e[15];
+    const double cr = cos(roll), sr = sin(roll), cp = cos(pitch), sp = sin(pitch), cyw = cos(yaw), syw = sin(yaw);
+    // R = R3(yaw) R2(pitch) R1(roll)
+    const double R[3][3] = {{cyw * cp, cyw * sp * sr - syw * cr, cyw * sp * cr + syw * sr},
+                            {syw * cp, syw * sp * sr + cyw * cr, syw * sp * cr - cyw * sr},
+                            {-sp, cp * sr, cp * cr}};
+    for (int k = 0; k < 5; ++k) {
+        const double u = e[3 * k], v = e[3 * k + 1], d = e[3 * k + 2];
+        const double q0 = d * (u - cx) / fx - e[18], q1 = d * (v - cy) / fy - e[19], q2 = d - e[20];
+        for (int r = 0; r < 3; ++r) out[r * 5 + k] = R[0][r] * q0 + R[1][r] * q1 + R[2][r] * q2;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_pose(PoseArgs a)
+{
+    __shared__ double s_e[2][64];
+    __shared__ double s_p[2][15];
+    const int t = *a.frame;
+    for (int arm = 0; arm < 2; ++arm) {
+        const ArmDev &A = a.arm[arm];
+        for (int r = threadIdx.x; r < a.npose; r += blockDim.x) s_e[arm][r] = project_row(A.hp, A.pm, A.xbar, A.D, r);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) pose3d(s_e[threadIdx.x], a.fx, a.fy, a.cx, a.cy, s_p[threadIdx.x]);
+    __syncthreads();
+    for (int arm = 0; arm < 2; ++arm) {
+        if (a.pose)
+            for (int r = threadIdx.x; r < a.npose; r += blockDim.x)
+                a.pose[((size_t)t * 2 + arm) * a.npose + r] = s_e[arm][r];
+        if (a.pos3d && threadIdx.x < 15) a.pos3d[((size_t)t * 2 + arm) * 15 + threadIdx.x] = s_p[arm][threadIdx.x];
+    }
+    if (threadIdx.x == 0) *a.frame = t + 1;
+}
+
+__global__ void k_tr_set(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+template <typename T>
+hipError_t talloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1));
+}
+
+}  // namespace
+
+struct pf_tracker {
+    pf_tracker_config cfg{};
+    pf_handle *arm[2] = {nullptr, nullptr};
+    hipStream_t s[2] = {nullptr, nullptr};
+    int N = 0, D[2] = {0, 0}, npose = 0, n_c = 0, device = 0, radius = 0;
+    double *hp[2] = {nullptr, nullptr}, *pm[2] = {nullptr, nullptr};
+    float *taps = nullptr;
+    uint8_t *img_stage = nullptr, *blur = nullptr;
+    int *roi_stage = nullptr;
+    double *w[2] = {nullptr, nullptr};
+    int *bins[2] = {nullptr, nullptr};
+    float *Z[2] = {nullptr, nullptr};
+    FrameRec *rec = nullptr;
+    int *fctr = nullptr, *fctr_aux = nullptr;
+    uint32_t *uni = nullptr;          // [arm 0: cap x 2][arm 1: cap x 2][proposals: cap x 2]
+    int uni_cap = 0;
+    std::vector<uint32_t> h_uni;
+    double *lml_stage = nullptr, *pose_stage = nullptr, *p3d_stage = nullptr;
+    hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr}, ev_pre = nullptr;
+    hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
+    long long frames_done = 0, uniform_frame = 0;
+    bool ran = false;
+};
+
+namespace {
+
+int tr_check(pf_tracker *tr)
+{
+    if (!tr) return set_error(PF_ERR_ARG, "null tracker");
+    if (hipSetDevice(tr->device) != hipSuccess) return set_error(PF_ERR_HIP, "hipSetDevice failed");
+    return PF_OK;
+}
+
+ArmDev arm_dev(pf_tracker *tr, int a, int parity)
+{
+    ArmDev d{};
+    d.xbar = eng::estimate_ptr(tr->arm[a]);
+    d.hp = tr->hp[a];
+    d.pm = tr->pm[a];
+    d.w = tr->w[a];
+    d.maxkey = eng::maxkey_ptr(tr->arm[a], parity);
+    d.D = tr->D[a];
+    return d;
+}
+
+PropParams prop_params(pf_tracker *tr, const int *roi, const int *frame)
+{
+    PropParams p{};
+    p.seed = tr->cfg.seed;
+    p.g0 = tr->frames_done;
+    p.uniform_frame = tr->uniform_frame;
+    p.roi = roi;
+    p.frame = frame;
+    p.n_c = tr->n_c;
+    p.rows = tr->cfg.rows;
+    p.cols = tr->cfg.cols;
+    p.spread = tr->cfg.spread;
+    p.pa = tr->cfg.p_assoc;
+    p.zfloor = tr->cfg.z_floor * (1.0 - 2.0 * tr->cfg.p_assoc);
+    return p;
+}
+
+int launch_check(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(PF_ERR_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
+    return PF_OK;
+}
+
+int launch_blur(pf_tracker *tr, const uint8_t *src, const int *frame, uint8_t *dst)
+{
+    BlurArgs b{};
+    b.src = src;
+    b.dst = dst;
+    b.taps = tr->taps;
+    b.frame = frame;
+    b.frame_stride = (long long)tr->cfg.rows * tr->cfg.cols;
+    b.rows = tr->cfg.rows;
+    b.cols = tr->cfg.cols;
+    b.radius = tr->radius;
+    const dim3 grid((tr->cfg.cols + kBlurW - 1) / kBlurW, (tr->cfg.rows + kBlurH - 1) / kBlurH);
+    hipLaunchKernelGGL(k_blur, grid, dim3(256), 0, tr->s[0], b);
+    return launch_check("k_blur");
+}
+
+int launch_pose(pf_tracker *tr, int *frame, double *pose, double *pos3d)
+{
+    PoseArgs pa{};
+    pa.arm[0] = arm_dev(tr, 0, 0);
+    pa.arm[1] = arm_dev(tr, 1, 0);
+    pa.frame = frame;
+    pa.pose = pose;
+    pa.pos3d = pos3d;
+    pa.npose = tr->npose;
+    pa.fx = tr->cfg.fx;
+    pa.fy = tr->cfg.fy;
+    pa.cx = tr->cfg.cx;
+    pa.cy = tr->cfg.cy;
+    hipLaunchKernelGGL(k_pose, dim3(1), dim3(64), 0, tr->s[0], pa);
+    return launch_check("k_pose");
+}
+
+// One frame of the pipeline; `parity` alternates from frame to frame (it
+// selects the handles' frame counter and maximum slot).
+int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const int *roi, double *lml, double *pose,
+                  double *pos3d)
+{
+    int st = launch_blur(tr, lik, tr->fctr, tr->blur);
+    if (st) return st;
+    const PropParams p = prop_params(tr, roi, tr->fctr);
+    const ArmDev a0 = arm_dev(tr, 0, parity), a1 = arm_dev(tr, 1, parity);
+    const int pairs = tr->n_c / 2;
+    hipLaunchKernelGGL(k_propose, dim3((pairs + kProposeBlock - 1) / kProposeBlock, 2), dim3(kProposeBlock), 0,
+                       tr->s[0], p, a0, a1, (const uint8_t *)tr->blur, tr->rec);
+    st = launch_check("k_propose");
+    if (st) return st;
+    TR_TRY(hipEventRecord(tr->ev_fork[parity], tr->s[0]));
+    TR_TRY(hipStreamWaitEvent(tr->s[1], tr->ev_fork[parity], 0));
+    const uint32_t *u_prop = tr->uni + 4 * (size_t)tr->uni_cap;
+    for (int a = 0; a < 2; ++a) {
+        pf_handle *h = tr->arm[a];
+        st = eng::resample_linear(h, tr->w[a], tr->n_c, tr->N, parity, tr->fctr, u_prop, 2, a, tr->bins[a]);
+        if (st) return st;
+        hipLaunchKernelGGL(k_build_meas, dim3((tr->N + 255) / 256), dim3(256), 0, tr->s[a], p, a,
+                           (const int *)tr->bins[a], (const FrameRec *)tr->rec, tr->Z[a], tr->N);
+        st = launch_check("k_build_meas");
+        if (st) return st;
+        st = eng::step_ref(h, tr->Z[a], tr->uni + 2 * (size_t)tr->uni_cap * a, parity, lml + (size_t)a * T);
+        if (st) return st;
+        st = eng::estimate(h);
+        if (st) return st;
+    }
+    TR_TRY(hipEventRecord(tr->ev_join[parity], tr->s[1]));
+    TR_TRY(hipStreamWaitEvent(tr->s[0], tr->ev_join[parity], 0));
+    return launch_pose(tr, tr->fctr, pose, pos3d);
+}
+
+// Frame uniforms of global frames [g0, g0 + T) into the device table.
+int upload_uniforms(pf_tracker *tr, int T)
+{
+    if (T > tr->uni_cap) {
+        (void)hipFree(tr->uni);
+        tr->uni = nullptr;
+        const int cap = std::max(T, 64);
+        TR_TRY(talloc(&tr->uni, (size_t)6 * cap));
+        tr->uni_cap = cap;
+    }
+    const size_t cap = (size_t)tr->uni_cap;
+    tr->h_uni.assign(6 * cap, 0u);
+    std::vector<uint32_t> w(8 * (size_t)T);
+    philox_u32(tr->cfg.seed, kStreamFrame, 8 * (uint64_t)tr->frames_done, 8 * (int64_t)T, w.data());
+    for (int t = 0; t < T; ++t) {
+        const uint32_t *f = w.data() + 8 * (size_t)t;
+        tr->h_uni[2 * (size_t)t] = f[0];
+        tr->h_uni[2 * (size_t)t + 1] = f[1];
+        tr->h_uni[2 * cap + 2 * (size_t)t] = f[2];
+        tr->h_uni[2 * cap + 2 * (size_t)t + 1] = f[3];
+        tr->h_uni[4 * cap + 2 * (size_t)t] = f[4];
+        tr->h_uni[4 * cap + 2 * (size_t)t + 1] = f[5];
+    }
+    TR_TRY(hipMemcpyAsync(tr->uni, tr->h_uni.data(), sizeof(uint32_t) * tr->h_uni.size(), hipMemcpyHostToDevice,
+                          tr->s[0]));
+    TR_TRY(hipStreamSynchronize(tr->s[0]));
+    return PF_OK;
+}
+
+// Estimates of both arms' current tracks, stream 1 joined into stream 0.
+int prepare_estimates(pf_tracker *tr)
+{
+    for (int a = 0; a < 2; ++a) {
+        int st = eng::estimate(tr->arm[a]);
+        if (st) return st;
+    }
+    TR_TRY(hipEventRecord(tr->ev_pre, tr->s[1]));
+    TR_TRY(hipStreamWaitEvent(tr->s[0], tr->ev_pre, 0));
+    return PF_OK;
+}
+
+int status_of_arms(pf_tracker *tr)
+{
+    int warn = 0;
+    for (int a = 0; a < 2; ++a) {
+        int deg = 0;
+        int st = eng::take_status(tr->arm[a], &deg);
+        if (st) return st;
+        warn |= deg;
+    }
+    return warn ? PF_WARN_DEGENERATE : PF_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+void pf_tracker_config_default(pf_tracker_config *cfg)
+{
+    if (!cfg) return;
+    cfg->rows = 480;
+    cfg->cols = 640;
+    cfg->proposals_per_particle = 10;
+    cfg->p_assoc = 0.05;
+    cfg->z_floor = 1e-4;
+    cfg->spread = 0.8;
+    cfg->blur_ksize = 15;
+    cfg->blur_sigma = 3.0;
+    cfg->fx = 525.0;
+    cfg->fy = 525.0;
+    cfg->cx = 319.5;
+    cfg->cy = 239.5;
+    cfg->seed = 1;
+}
+
+int pf_tracker_destroy(pf_tracker *tr)
+{
+    if (!tr) return PF_OK;
+    (void)hipSetDevice(tr->device);
+    for (int a = 0; a < 2; ++a) {
+        (void)hipFree(tr->hp[a]);
+        (void)hipFree(tr->pm[a]);
+        (void)hipFree(tr->w[a]);
+        (void)hipFree(tr->bins[a]);
+        (void)hipFree(tr->Z[a]);
+        if (tr->ev_fork[a]) (void)hipEventDestroy(tr->ev_fork[a]);
+        if (tr->ev_join[a]) (void)hipEventDestroy(tr->ev_join[a]);
+    }
+    (void)hipFree(tr->taps);
+    (void)hipFree(tr->img_stage);
+    (void)hipFree(tr->blur);
+    (void)hipFree(tr->roi_stage);
+    (void)hipFree(tr->rec);
+    (void)hipFree(tr->fctr);
+    (void)hipFree(tr->fctr_aux);
+    (void)hipFree(tr->uni);
+    (void)hipFree(tr->lml_stage);
+    (void)hipFree(tr->pose_stage);
+    (void)hipFree(tr->p3d_stage);
+    if (tr->ev_pre) (void)hipEventDestroy(tr->ev_pre);
+    if (tr->ev_t0) (void)hipEventDestroy(tr->ev_t0);
+    if (tr->ev_t1) (void)hipEventDestroy(tr->ev_t1);
+    delete tr;
+    return PF_OK;
+}
+
+int pf_tracker_create(pf_tracker **out, const pf_tracker_config *cfg, pf_handle *arm1, pf_handle *arm2,
+                      int32_t npose, const double *pca_proj1, const double *pca_mean1, const double *pca_proj2,
+                      const double *pca_mean2)
+{
+    if (!out) return set_error(PF_ERR_ARG, "null output");
+    *out = nullptr;
+    if (!arm1 || !arm2 || arm1 == arm2) return set_error(PF_ERR_ARG, "the tracker needs two distinct arm handles");
+    if (!pca_proj1 || !pca_mean1 || !pca_proj2 || !pca_mean2) return set_error(PF_ERR_ARG, "null PCA matrices");
+    pf_tracker_config c;
+    pf_tracker_config_default(&c);
+    if (cfg) c = *cfg;
+    eng::Info in[2];
+    for (int a = 0; a < 2; ++a) {
+        int st = eng::info(a ? arm2 : arm1, &in[a]);
+        if (st) return st;
+        if (!in[a].model) return set_error(PF_ERR_STATE, "arm %d has no model", a + 1);
+        if (in[a].m != 6) return set_error(PF_ERR_ARG, "arm %d measures %d rows; the tracker's measurement has 6", a + 1, in[a].m);
+    }
+    if (in[0].N != in[1].N) return set_error(PF_ERR_ARG, "arms with different particle counts (%d, %d)", in[0].N, in[1].N);
+    if (in[0].device != in[1].device) return set_error(PF_ERR_ARG, "arms on different devices");
+    if (npose < 21) return set_error(PF_ERR_ARG, "npose = %d: get3Dpose reads 21 pose entries", npose);
+    if (npose > 64) return set_error(PF_ERR_ARG, "npose = %d > 64", npose);
+    if (c.blur_ksize < 1 || c.blur_ksize > 2 * kMaxRadius + 1 || !(c.blur_ksize & 1))
+        return set_error(PF_ERR_ARG, "blur_ksize %d: odd, at most %d", c.blur_ksize, 2 * kMaxRadius + 1);
+    if (!(c.blur_sigma > 0.0)) return set_error(PF_ERR_ARG, "blur_sigma must be positive");
+    const int R = c.blur_ksize / 2;
+    if (c.rows <= R || c.cols <= R) return set_error(PF_ERR_ARG, "image %d x %d smaller than the blur radius", c.rows, c.cols);
+    if (c.proposals_per_particle < 1) return set_error(PF_ERR_ARG, "proposals_per_particle must be >= 1");
+    const long long nc = (long long)c.proposals_per_particle * in[0].N;
+    if (nc > kMaxProposals) return set_error(PF_ERR_ARG, "%lld proposals per arm exceed %d", nc, kMaxProposals);
+    if (nc & 1) return set_error(PF_ERR_ARG, "the proposal count per arm (%lld) must be even", nc);
+    if (!(c.p_assoc >= 0.0 && c.p_assoc < 0.5) || !(c.z_floor > 0.0) || !(c.spread > 0.0))
+        return set_error(PF_ERR_ARG, "bad association constants");
+    if (!(c.fx != 0.0 && c.fy != 0.0)) return set_error(PF_ERR_ARG, "zero focal length");
+
+    pf_tracker *tr = new pf_tracker();
+    tr->cfg = c;
+    tr->arm[0] = arm1;
+    tr->arm[1] = arm2;
+    tr->s[0] = in[0].stream;
+    tr->s[1] = in[1].stream;
+    tr->N = in[0].N;
+    tr->D[0] = in[0].D;
+    tr->D[1] = in[1].D;
+    tr->npose = npose;
+    tr->n_c = (int)nc;
+    tr->device = in[0].device;
+    tr->radius = R;
+    auto fail = [&](int st) {
+        pf_tracker_destroy(tr);
+        return st;
+    };
+    auto hip_ok = [&](hipError_t e, const char *what) {
+        return e == hipSuccess ? PF_OK : set_error(PF_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
+    };
+    int st = tr_check(tr);
+    if (st) return fail(st);
+    // Hp = pca_proj^T (npose x D), pca_mean as given
+    for (int a = 0; a < 2; ++a) {
+        const double *P = a ? pca_proj2 : pca_proj1;
+        const int D = tr->D[a];
+        std::vector<double> hp((size_t)npose * D);
+        for (int r = 0; r < npose; ++r)
+            for (int k = 0; k < D; ++k) hp[(size_t)r * D + k] = P[(size_t)k * npose + r];
+        st = hip_ok(talloc(&tr->hp[a], hp.size()), "hipMalloc");
+        if (!st) st = hip_ok(talloc(&tr->pm[a], (size_t)npose), "hipMalloc");
+        if (!st) st = hip_ok(hipMemcpy(tr->hp[a], hp.data(), sizeof(double) * hp.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        if (!st) st = hip_ok(hipMemcpy(tr->pm[a], a ? pca_mean2 : pca_mean1, sizeof(double) * npose, hipMemcpyHostToDevice), "hipMemcpy");
+        if (!st) st = hip_ok(talloc(&tr->w[a], (size_t)tr->n_c), "hipMalloc");
+        if (!st) st = hip_ok(talloc(&tr->bins[a], (size_t)tr->N), "hipMalloc");
+        if (!st) st = hip_ok(talloc(&tr->Z[a], (size_t)6 * tr->N), "hipMalloc");
+        if (!st) st = eng::reserve_resample(a ? arm2 : arm1, tr->n_c);
+        if (!st) st = hip_ok(hipEventCreateWithFlags(&tr->ev_fork[a], hipEventDisableTiming), "hipEventCreate");
+        if (!st) st = hip_ok(hipEventCreateWithFlags(&tr->ev_join[a], hipEventDisableTiming), "hipEventCreate");
+        if (st) return fail(st);
+    }
+    // Gaussian taps exp(-x^2 / (2 sigma^2)), x = -R..R, normalised by their sum
+    std::vector<double> g(2 * R + 1);
+    double sum = 0.0;
+    const double s2 = -0.5 / (c.blur_sigma * c.blur_sigma);
+    for (int i = -R; i <= R; ++i) {
+        g[i + R] = det_exp((double)(i * i) * s2);
+        sum += g[i + R];
+    }
+    std::vector<float> taps(2 * R + 1);
+    for (int i = 0; i <= 2 * R; ++i) taps[i] = (float)(g[i] / sum);
+    const size_t px = (size_t)c.rows * c.cols;
+    st = hip_ok(talloc(&tr->taps, taps.size()), "hipMalloc");
+    if (!st) st = hip_ok(hipMemcpy(tr->taps, taps.data(), sizeof(float) * taps.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    if (!st) st = hip_ok(talloc(&tr->img_stage, px), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->blur, px), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->roi_stage, 4), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->rec, 1), "hipMalloc");
+    if (!st) st = hip_ok(hipMemset(tr->rec, 0, sizeof(FrameRec)), "hipMemset");
+    if (!st) st = hip_ok(talloc(&tr->fctr, 1), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->fctr_aux, 1), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->lml_stage, 2), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->pose_stage, (size_t)2 * npose), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->p3d_stage, 30), "hipMalloc");
+    if (!st) st = hip_ok(hipEventCreateWithFlags(&tr->ev_pre, hipEventDisableTiming), "hipEventCreate");
+    if (!st) st = hip_ok(hipEventCreate(&tr->ev_t0), "hipEventCreate");
+    if (!st) st = hip_ok(hipEventCreate(&tr->ev_t1), "hipEventCreate");
+    if (st) return fail(st);
+    tr->frames_done = 0;
+    tr->uniform_frame = 0;
+    *out = tr;
+    return PF_OK;
+}
+
+int pf_tracker_reset(pf_tracker *tr)
+{
+    int st = tr_check(tr);
+    if (st) return st;
+    tr->frames_done = 0;
+    tr->uniform_frame = 0;
+    return PF_OK;
+}
+
+int pf_tracker_num_proposals(pf_tracker *tr, int32_t *n_c)
+{
+    if (!tr || !n_c) return set_error(PF_ERR_ARG, "null argument");
+    *n_c = tr->n_c;
+    return PF_OK;
+}
+
+int pf_tracker_run_dev(pf_tracker *tr, int32_t T, const uint8_t *likelihood, const int32_t *roi, double *lml,
+                       double *pose, double *pos3d, int32_t flags)
+{
+    int st = tr_check(tr);
+    if (st) return st;
+    if (T <= 0 || !likelihood || !roi || !lml) return set_error(PF_ERR_ARG, "bad run arguments");
+    for (int a = 0; a < 2; ++a) {
+        eng::Info in;
+        st = eng::info(tr->arm[a], &in);
+        if (st) return st;
+        if (!in.tracks) return set_error(PF_ERR_STATE, "arm %d tracks not initialised", a + 1);
+        if (in.N != tr->N) return set_error(PF_ERR_STATE, "arm %d particle count changed", a + 1);
+    }
+    const bool graph = (flags & 1) != 0;
+    st = upload_uniforms(tr, T);
+    if (st) return st;
+    TR_TRY(hipEventRecord(tr->ev_t0, tr->s[0]));
+    hipLaunchKernelGGL(k_tr_set, dim3(1), dim3(64), 0, tr->s[0], tr->fctr, 0);
+    st = launch_check("k_tr_set");
+    for (int a = 0; !st && a < 2; ++a) st = eng::set_frame(tr->arm[a], 0);
+    if (!st) st = prepare_estimates(tr);
+    if (st) return st;
+    auto frame = [&](int f) { return enqueue_frame(tr, f & 1, T, likelihood, roi, lml, pose, pos3d); };
+    st = frame(0);
+    if (st) return st;
+    const int pairs = graph ? (T - 1) / 2 : 0;
+    if (pairs > 0) {
+        hipGraph_t gr;
+        hipGraphExec_t ge;
+        TR_TRY(hipStreamBeginCapture(tr->s[0], hipStreamCaptureModeThreadLocal));
+        int s1 = frame(1);
+        int s2 = s1 ? s1 : frame(2);
+        hipError_t ec = hipStreamEndCapture(tr->s[0], &gr);
+        if (s2) return s2;
+        if (ec != hipSuccess) return set_error(PF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        TR_TRY(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+        for (int q = 0; q < pairs; ++q) TR_TRY(hipGraphLaunch(ge, tr->s[0]));
+        TR_TRY(hipStreamSynchronize(tr->s[0]));
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(gr);
+    }
+    for (int f = 1 + 2 * pairs; f < T; ++f) {
+        st = frame(f);
+        if (st) return st;
+    }
+    TR_TRY(hipEventRecord(tr->ev_t1, tr->s[0]));
+    tr->frames_done += T;
+    tr->ran = true;
+    return PF_OK;
+}
+
+int pf_tracker_synchronize(pf_tracker *tr)
+{
+    int st = tr_check(tr);
+    if (st) return st;
+    TR_TRY(hipStreamSynchronize(tr->s[0]));
+    TR_TRY(hipStreamSynchronize(tr->s[1]));
+    return PF_OK;
+}
+
+int pf_tracker_last_run_ms(pf_tracker *tr, double *ms)
+{
+    int st = tr_check(tr);
+    if (st) return st;
+    if (!ms) return set_error(PF_ERR_ARG, "null output");
+    if (!tr->ran) return set_error(PF_ERR_STATE, "no run yet");
+    float v = 0.f;
+    TR_TRY(hipEventSynchronize(tr->ev_t1));
+    TR_TRY(hipEventElapsedTime(&v, tr->ev_t0, tr->ev_t1));
+    *ms = v;
+    return PF_OK;
+}
+
+int pf_tracker_frame(pf_tracker *tr, const uint8_t *likelihood, const int32_t roi[4], double *lml, double *pose,
+                     double *pos3d)
+{
+    int st = tr_check(tr);
+    if (st) return st;
+    if (!likelihood || !roi) return set_error(PF_ERR_ARG, "null image or ROI");
+    if (roi[2] <= 0 || roi[3] <= 0) return set_error(PF_ERR_ARG, "empty face ROI");
+    const size_t px = (size_t)tr->cfg.rows * tr->cfg.cols;
+    TR_TRY(hipMemcpyAsync(tr->img_stage, likelihood, px, hipMemcpyHostToDevice, tr->s[0]));
+    TR_TRY(hipMemcpyAsync(tr->roi_stage, roi, sizeof(int32_t) * 4, hipMemcpyHostToDevice, tr->s[0]));
+    TR_TRY(hipStreamSynchronize(tr->s[0]));
+    st = pf_tracker_run_dev(tr, 1, tr->img_stage, tr->roi_stage, tr->lml_stage, tr->pose_stage, tr->p3d_stage, 0);
+    if (st) return st;
+    if (lml) TR_TRY(hipMemcpyAsync(lml, tr->lml_stage, sizeof(double) * 2, hipMemcpyDeviceToHost, tr->s[0]));
+    if (pose)
+        TR_TRY(hipMemcpyAsync(pose, tr->pose_stage, sizeof(double) * 2 * tr->npose, hipMemcpyDeviceToHost, tr->s[0]));
+    if (pos3d) TR_TRY(hipMemcpyAsync(pos3d, tr->p3d_stage, sizeof(double) * 30, hipMemcpyDeviceToHost, tr->s[0]));
+    st = pf_tracker_synchronize(tr);
+    if (st) return st;
+    return status_of_arms(tr);
+}
+
+int pf_tracker_get_last(pf_tracker *tr, uint8_t *blurred, double *proposals, double *weights, int32_t *bins,
+                        float *measurements)
+{
+    int st = pf_tracker_synchronize(tr);
+    if (st) return st;
+    if (!tr->ran) return set_error(PF_ERR_STATE, "no frame yet");
+    hipStream_t s = tr->s[0];
+    const size_t px = (size_t)tr->cfg.rows * tr->cfg.cols;
+    if (blurred) TR_TRY(hipMemcpyAsync(blurred, tr->blur, px, hipMemcpyDeviceToHost, s));
+    if (proposals) {
+        double *d = nullptr;
+        TR_TRY(talloc(&d, (size_t)4 * tr->n_c));
+        PropParams p = prop_params(tr, nullptr, nullptr);
+        hipLaunchKernelGGL(k_dump_proposals, dim3((tr->n_c + 255) / 256, 2), dim3(256), 0, s, p,
+                           (const FrameRec *)tr->rec, d);
+        st = launch_check("k_dump_proposals");
+        hipError_t e = hipSuccess;
+        if (!st) e = hipMemcpyAsync(proposals, d, sizeof(double) * 4 * tr->n_c, hipMemcpyDeviceToHost, s);
+        if (!st && e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(d);
+        if (st) return st;
+        if (e != hipSuccess) return set_error(PF_ERR_HIP, "proposal copy failed: %s", hipGetErrorString(e));
+    }
+    for (int a = 0; a < 2; ++a) {
+        if (weights)
+            TR_TRY(hipMemcpyAsync(weights + (size_t)a * tr->n_c, tr->w[a], sizeof(double) * tr->n_c,
+                                  hipMemcpyDeviceToHost, s));
+        if (bins)
+            TR_TRY(hipMemcpyAsync(bins + (size_t)a * tr->N, tr->bins[a], sizeof(int) * tr->N, hipMemcpyDeviceToHost, s));
+        if (measurements)
+            TR_TRY(hipMemcpyAsync(measurements + (size_t)a * 6 * tr->N, tr->Z[a], sizeof(float) * 6 * tr->N,
+                                  hipMemcpyDeviceToHost, s));
+    }
+    TR_TRY(hipStreamSynchronize(s));
+    return PF_OK;
+}
+
+int pf_tracker_pose(pf_tracker *tr, double *pose, double *pos3d)
+{
+    int st = tr_check(tr);
+    if (st) return st;
+    for (int a = 0; a < 2; ++a) {
+        eng::Info in;
+        st = eng::info(tr->arm[a], &in);
+        if (st) return st;
+        if (!in.tracks) return set_error(PF_ERR_STATE, "arm %d tracks not initialised", a + 1);
+    }
+    st = prepare_estimates(tr);
+    if (st) return st;
+    hipLaunchKernelGGL(k_tr_set, dim3(1), dim3(64), 0, tr->s[0], tr->fctr_aux, 0);
+    st = launch_check("k_tr_set");
+    if (!st) st = launch_pose(tr, tr->fctr_aux, tr->pose_stage, tr->p3d_stage);
+    if (st) return st;
+    if (pose)
+        TR_TRY(hipMemcpyAsync(pose, tr->pose_stage, sizeof(double) * 2 * tr->npose, hipMemcpyDeviceToHost, tr->s[0]));
+    if (pos3d) TR_TRY(hipMemcpyAsync(pos3d, tr->p3d_stage, sizeof(double) * 30, hipMemcpyDeviceToHost, tr->s[0]));
+    return pf_tracker_synchronize(tr);
+}
+
+int pf_tracker_blur(pf_tracker *tr, const uint8_t *src, uint8_t *dst)
+{
+    int st = tr_check(tr);
+    if (st) return st;
+    if (!src || !dst) return set_error(PF_ERR_ARG, "null image");
+    const size_t px = (size_t)tr->cfg.rows * tr->cfg.cols;
+    uint8_t *d = nullptr;
+    TR_TRY(talloc(&d, px));
+    hipError_t e = hipMemcpyAsync(tr->img_stage, src, px, hipMemcpyHostToDevice, tr->s[0]);
+    if (e == hipSuccess) {
+        st = launch_blur(tr, tr->img_stage, nullptr, d);
+        if (!st) e = hipMemcpyAsync(dst, d, px, hipMemcpyDeviceToHost, tr->s[0]);
+        if (!st && e == hipSuccess) e = hipStreamSynchronize(tr->s[0]);
+    }
+    (void)hipFree(d);
+    if (st) return st;
+    if (e != hipSuccess) return set_error(PF_ERR_HIP, "blur copy failed: %s", hipGetErrorString(e));
+    return PF_OK;
+}
+
+}  // extern "C"

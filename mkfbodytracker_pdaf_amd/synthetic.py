@@ -219,3 +219,42 @@ def per_particle_measurements(z, N: int, seed: int, jitter_px: float = 5.0):
     src/pfPose.cpp:303-323)."""
     rng = np.random.default_rng(seed)
     return z[:, None] + jitter_px * rng.standard_normal((z.size, N))
+
+
+# ------------------------------------------------- pose tracker inputs
+# Face ROI of the shipped models' mean pose (head at about (324, 75) px).
+FACE_ROI = (304, 50, 40, 48)
+HANDS = ((388.0, 281.0), (264.0, 268.0))
+
+
+def likelihood_image(points, rows: int = 480, cols: int = 640, seed: int = 0, amp: float = 230.0,
+                     sigma: float = 7.0, clutter: int = 2) -> np.ndarray:
+    """uint8 hand-likelihood image: Gaussian blobs at the (x, y) points and at
+    `clutter` random places, plus sparse low-level noise."""
+    rng = np.random.default_rng(seed)
+    pts = [tuple(p) for p in points] + [(rng.uniform(0, cols), rng.uniform(0, rows)) for _ in range(clutter)]
+    yy = np.arange(rows)[:, None].astype(np.float64)
+    xx = np.arange(cols)[None, :].astype(np.float64)
+    img = np.zeros((rows, cols))
+    for x, y in pts:
+        img += amp * np.exp(-((xx - x) ** 2 + (yy - y) ** 2) / (2.0 * sigma ** 2))
+    noise = rng.random((rows, cols)) < 0.01
+    img[noise] += rng.uniform(0, 60, int(noise.sum()))
+    return np.clip(np.rint(img), 0, 255).astype(np.uint8)
+
+
+def tracker_sequence(T: int, seed: int, rows: int = 480, cols: int = 640):
+    """T frames of (likelihood image, face ROI): both hands move on smooth
+    random paths around their mean positions, the face ROI jitters by a pixel
+    or two.  Returns (images T x rows x cols uint8, rois T x 4 int32)."""
+    rng = np.random.default_rng(seed)
+    imgs = np.empty((T, rows, cols), np.uint8)
+    rois = np.empty((T, 4), np.int32)
+    ph = rng.uniform(0, 2 * np.pi, 4)
+    for t in range(T):
+        hands = [(hx + 25.0 * np.sin(0.07 * t + ph[2 * i]), hy + 20.0 * np.cos(0.05 * t + ph[2 * i + 1]))
+                 for i, (hx, hy) in enumerate(HANDS)]
+        imgs[t] = likelihood_image(hands, rows, cols, seed=seed * 1000 + t)
+        rois[t] = (FACE_ROI[0] + int(rng.integers(-2, 3)), FACE_ROI[1] + int(rng.integers(-2, 3)),
+                   FACE_ROI[2], FACE_ROI[3])
+    return imgs, rois

@@ -22,6 +22,8 @@ _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 
 def build() -> None:
@@ -84,6 +86,19 @@ def lib():
         L.orc_pf2d_update.argtypes = [C.POINTER(_Pf2dGmm), C.c_int, _dp, _dp, _dp, _ip, _dp,
                                       C.c_uint32, _dp, _dp, C.c_int, C.c_int]
         L.orc_pf2d_estimate.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_det_log.argtypes = [C.c_double]
+        L.orc_det_log.restype = C.c_double
+        L.orc_det_sincos_turn.argtypes = [C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_normal_pair.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_gauss_taps.argtypes = [C.c_int, C.c_double, _fp]
+        L.orc_blur_u8.argtypes = [_u8p, C.c_int, C.c_int, C.c_int, _fp, _fp, _u8p]
+        L.orc_tracker_proposals.argtypes = [C.c_uint64, C.c_int, C.c_int64, C.c_int, C.c_int, _ip, C.c_int,
+                                            C.c_int, _dp, C.c_double, _dp]
+        L.orc_tracker_weights.argtypes = [C.c_int, _dp, _dp, _dp, _u8p, C.c_int, C.c_int, C.c_double,
+                                          C.c_double, _dp]
+        L.orc_tracker_measurement.argtypes = [C.c_int, _ip, C.c_int, _dp, _ip, _dp]
+        L.orc_rpy.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_get3Dpose.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         _lib = L
@@ -340,3 +355,140 @@ class AdjProblem:
         rc = lib().orc_adj_grad(C.byref(self.s), theta.ctypes.data, C.byref(F), g.ctypes.data)
         assert rc == 0
         return F.value, g
+
+
+# ------------------------------------------------- pose tracker stage
+def det_log(x: float) -> float:
+    return lib().orc_det_log(float(x))
+
+
+def det_sincos_turn(w: int):
+    s = C.c_double(); c = C.c_double()
+    lib().orc_det_sincos_turn(int(w), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def normal_pair(w0: int, w1: int):
+    a = C.c_double(); b = C.c_double()
+    lib().orc_normal_pair(int(w0), int(w1), C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def gauss_taps(ksize: int = 15, sigma: float = 3.0):
+    t = np.zeros(ksize, np.float32)
+    lib().orc_gauss_taps(int(ksize), float(sigma), t)
+    return t
+
+
+def blur_u8(img, ksize: int = 15, sigma: float = 3.0):
+    img = np.ascontiguousarray(img, np.uint8)
+    rows, cols = img.shape
+    tmp = np.zeros((rows, cols), np.float32)
+    out = np.zeros_like(img)
+    if lib().orc_blur_u8(img, rows, cols, int(ksize), gauss_taps(ksize, sigma), tmp, out) != 0:
+        raise ValueError("image smaller than the blur radius")
+    return out
+
+
+def tracker_proposals(seed, arm, g, n_c, uniform, roi, rows, cols, y, spread=0.8):
+    out = np.zeros((2, n_c))
+    lib().orc_tracker_proposals(int(seed), int(arm), int(g), int(n_c), int(bool(uniform)), _i32(roi), int(rows),
+                                int(cols), _f64(y), float(spread), out)
+    return out
+
+
+def tracker_weights(props, p_own, p_other, img, pa=0.05, z_floor=1e-4):
+    props = _f64(props); img = np.ascontiguousarray(img, np.uint8)
+    n_c = props.shape[1]
+    w = np.zeros(n_c)
+    lib().orc_tracker_weights(n_c, props, _f64(p_own), _f64(p_other), img, img.shape[0], img.shape[1],
+                              float(pa), float(z_floor), w)
+    return w
+
+
+def tracker_measurement(bins, props, roi):
+    bins = _i32(bins); props = _f64(props)
+    Z = np.zeros((6, bins.size))
+    lib().orc_tracker_measurement(bins.size, bins, props.shape[1], props, _i32(roi), Z)
+    return Z
+
+
+def rpy(roll, pitch, yaw):
+    R = np.zeros(9)
+    lib().orc_rpy(float(roll), float(pitch), float(yaw), R)
+    return R.reshape(3, 3)
+
+
+def get3Dpose(e, fx=525.0, fy=525.0, cx=319.5, cy=239.5):
+    out = np.zeros(15)
+    if lib().orc_get3Dpose(_f64(e), float(fx), float(fy), float(cx), float(cy), out) != 0:
+        raise ValueError("singular projection")
+    return out.reshape(3, 5)
+
+
+def project(Hp, M, xbar):
+    """y = Hp xbar + M with the products of each row summed in index order
+    (the order of orc_get_samples and of the tracker's kernels)."""
+    Hp = _f64(Hp); M = _f64(M); xbar = _f64(xbar)
+    y = np.zeros(Hp.shape[0])
+    for r in range(Hp.shape[0]):
+        s = 0.0
+        for k in range(Hp.shape[1]):
+            s = s + float(Hp[r, k]) * float(xbar[k])
+        y[r] = s + float(M[r])
+    return y
+
+
+class PoseTrackerOracle:
+    """PFTracker's per-frame stage around two arm models (pfPose.cpp:210-348),
+    FP64.  Hp[a] = pca_proj[a]^T (npose x d), M[a] = pca_mean[a]."""
+
+    def __init__(self, arms, Hp, M, N, seed=1, rows=480, cols=640, ppp=10, pa=0.05, z_floor=1e-4,
+                 spread=0.8, ksize=15, sigma=3.0, camera=(525.0, 525.0, 319.5, 239.5)):
+        self.arms, self.Hp, self.M = list(arms), [_f64(h) for h in Hp], [_f64(m) for m in M]
+        self.N, self.seed, self.rows, self.cols = N, seed, rows, cols
+        self.n_c = ppp * N
+        self.pa, self.z_floor, self.spread = pa, z_floor, spread
+        self.ksize, self.sigma, self.camera = ksize, sigma, camera
+
+    def frame_words(self, g):
+        """u_comp, u_res of arm 1, of arm 2, proposal uniforms of arm 1, arm 2."""
+        return philox_u32(self.seed, 24, 8 * g, 6)
+
+    def proposal_stage(self, xbars, img, roi, g, uniform, weights_from=None):
+        """Blur, proposals, weights, proposal resampling and measurements.
+        xbars: the two arms' estimates.  weights_from: optional pair of weight
+        vectors to resample instead of the oracle's own (teacher forcing)."""
+        blurred = blur_u8(img, self.ksize, self.sigma)
+        words = self.frame_words(g)
+        width = float(roi[2])
+        ys = [project(self.Hp[a][:2], self.M[a][:2], xbars[a]) for a in range(2)]
+        props = [tracker_proposals(self.seed, a, g, self.n_c, uniform, roi, self.rows, self.cols, ys[a],
+                                   self.spread) for a in range(2)]
+        out = dict(blurred=blurred, y=ys, props=props, w=[], bins=[], Z=[])
+        for a in range(2):
+            p_own = get_sample_prob(self.Hp[a], self.M[a], xbars[a], props[a], width)
+            p_oth = get_sample_prob(self.Hp[1 - a], self.M[1 - a], xbars[1 - a], props[a], width)
+            w = tracker_weights(props[a], p_own, p_oth, blurred, self.pa, self.z_floor)
+            out["w"].append(w)
+            wr = w if weights_from is None else weights_from[a]
+            bins, _ = resample_linear(wr, self.N, int(words[4 + a]))
+            out["bins"].append(bins)
+            out["Z"].append(tracker_measurement(bins, props[a], roi))
+        return out
+
+    def update_arms(self, states, Zs, g):
+        """ParticleFilter::update of both arms with their measurements (Z
+        rounded to FP32, the filter's input precision)."""
+        words = self.frame_words(g)
+        res = []
+        for a in range(2):
+            x, P = states[a]
+            Z32 = np.asarray(Zs[a], np.float32).astype(np.float64)
+            res.append(self.arms[a].update_ref(x, P, Z32, int(words[2 * a]), int(words[2 * a + 1])))
+        return res
+
+    def pose(self, xbars):
+        fx, fy, cx, cy = self.camera
+        es = [project(self.Hp[a], self.M[a], xbars[a]) for a in range(2)]
+        return es, [get3Dpose(e, fx, fy, cx, cy) for e in es]

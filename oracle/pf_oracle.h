@@ -206,6 +206,27 @@ int orc_adj_value(const orc_adj_problem *p, const double *theta, const double *l
  * dual numbers, 8 directions per pass. */
 int orc_adj_grad(const orc_adj_problem *p, const double *theta, double *F, double *grad);
 
+/* ------------------------------------------------- pose tracker stage */
+/* (pf_pose_ref.c) IEEE-exact-operation log, sin/cos of 2 pi w 2^-32, and the
+ * Box-Muller pair (u1 = (w0 + 1) 2^-32, angle from w1). */
+double orc_det_log(double x);
+void orc_det_sincos_turn(uint32_t w, double *sn, double *cs);
+void orc_normal_pair(uint32_t w0, uint32_t w1, double *z0, double *z1);
+/* GaussianBlur taps and the separable reflect-101 blur (pfPose.cpp:213). */
+void orc_gauss_taps(int ksize, double sigma, float *taps);
+int orc_blur_u8(const uint8_t *src, int rows, int cols, int ksize, const float *taps, float *tmp, uint8_t *dst);
+/* Proposals of one arm for global frame g (pfPose.cpp:218-235). */
+void orc_tracker_proposals(uint64_t seed, int arm, int64_t g, int n_c, int uniform, const int roi[4], int rows,
+                           int cols, const double y[2], double spread, double *out);
+/* Association weights (pfPose.cpp:244-293), unnormalised. */
+void orc_tracker_weights(int n_c, const double *props, const double *p_own, const double *p_other, const uint8_t *img,
+                         int rows, int cols, double pa, double z_floor, double *w);
+/* 6 x N measurement from resampled proposals (pfPose.cpp:303-320). */
+void orc_tracker_measurement(int N, const int *bins, int n_c, const double *props, const int roi[4], double *Z);
+/* rpy and get3Dpose (pfPose.cpp:84-127). */
+void orc_rpy(double roll, double pitch, double yaw, double R[9]);
+int orc_get3Dpose(const double *e, double fx, double fy, double cx, double cy, double out[15]);
+
 /* ------------------------------------------------------------- misc */
 /* Number of OpenMP threads used by the particle loops (1 = sequential). */
 void orc_set_threads(int n);
