@@ -20,6 +20,10 @@ against the forward pass.
 ``--workload scaled`` runs the 24-dimensional state: K=64, m=10 (5 joints,
 PDAF on each, 16 candidates, Poisson(10) clutter), 32 sequences x 31,250
 particles x 1000 frames through the batched engine (FP32 VALU).
+``--workload tracker`` runs the pose tracker's whole frame for both arms
+(likelihood blur, 10 N hand proposals per arm, association weights, proposal
+resampling, measurements, both filter updates, pose output) with the shipped
+GMM models at N = 100,000 particles per arm.
 """
 from __future__ import annotations
 
@@ -57,13 +61,15 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="step kernel block size (0 = default)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this size (PMC calibration)")
-    ap.add_argument("--workload", choices=("single", "batched", "adjoint", "scaled"), default="single",
+    ap.add_argument("--workload", choices=("single", "batched", "adjoint", "scaled", "tracker"), default="single",
                     help="single: one sequence of N particles (the headline line); batched: many ragged "
                          "sequences in one persistent step per frame, tuned vs default schedule")
     ap.add_argument("--sequences", type=int, default=256, help="batched: number of sequences")
     ap.add_argument("--tune-frames", type=int, default=32, help="batched: frames timed per candidate schedule")
     ap.add_argument("--scaled-shape", type=str, default="32,31250,1000",
                     help="scaled: sequences,particles,frames (the measured workload is 32,31250,1000)")
+    ap.add_argument("--tracker-shape", type=str, default="100000,400",
+                    help="tracker: particles per arm,frames (the measured workload is 100000,400)")
     ap.add_argument("--adj-shape", type=str, default="64,16384,500",
                     help="adjoint: sequences,particles,frames (the measured workload is 64,16384,500)")
     args = ap.parse_args()
@@ -76,6 +82,8 @@ def main():
         return run_adjoint(args)
     if args.workload == "scaled":
         return run_scaled(args)
+    if args.workload == "tracker":
+        return run_tracker(args)
 
     from mkfbodytracker_pdaf_amd import synthetic
     from mkfbodytracker_pdaf_amd.devmem import DeviceArray, synchronize
@@ -432,6 +440,121 @@ def run_scaled(args):
                      "kernel": "k_batch_step<24> + k_batch_weights (whole run)", "bytes_per_particle": bpp},
     }
     print(json.dumps(out))
+
+
+# Algorithmic HBM bytes of one tracker frame, per particle of one arm (d = 12,
+# 10 proposals per particle): proposals 10 x (weight write 8 + read 8 + block
+# scan write 8); measurement build (index 4 + 6 rows x 4); the filter step
+# (step_bytes_per_particle + the measurement column 24); its weights scan
+# (log-weight read 4, scan write 8) and ancestors (4); the estimate (12 floats
+# + index).
+def tracker_bytes_per_particle(d: int = 12, ppp: int = 10) -> int:
+    return ppp * 24 + 28 + step_bytes_per_particle(d) + 24 + 12 + 4 + 4 * d + 4
+
+
+def run_tracker(args):
+    """PFTracker's frame (pfPose.cpp:210-349) for both arms on the device: the
+    shipped GMM files (left and right arm), N particles per arm, 10 N hand
+    proposals per arm, synthetic 480 x 640 likelihood images with moving
+    hand blobs and clutter, face ROIs; T frames in one device-resident run
+    (a captured frame pair replayed)."""
+    from mkfbodytracker_pdaf_amd import synthetic
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray, synchronize
+    from mkfbodytracker_pdaf_amd.tracker import PoseTracker
+
+    N, T = (int(v) for v in args.tracker_shape.split(","))
+    left = os.path.join(ROOT, "tests", "golden", "data13D_PCA_100000_15_12.yml")
+    right = os.path.join(ROOT, "tests", "golden", "data23D_PCA_100000_15_12.yml")
+    distinct = min(T, 64)
+    imgs, rois = synthetic.tracker_sequence(distinct, seed=21)
+    reps = (T + distinct - 1) // distinct
+    d_img = DeviceArray(np.tile(imgs, (reps, 1, 1))[:T])
+    d_roi = DeviceArray(np.tile(rois, (reps, 1))[:T])
+    d_lml = DeviceArray.empty(2 * T, np.float64)
+    d_pose = DeviceArray.empty(T * 2 * 22, np.float64)
+    d_p3 = DeviceArray.empty(T * 2 * 15, np.float64)
+    for rep in range(2):                          # warm-up run, then the timed run
+        tr = PoseTracker.from_yml(left, right, n_particles=N, seed=21)
+        synchronize()
+        t0 = time.perf_counter()
+        tr.run_dev(T, d_img.ptr, d_roi.ptr, d_lml.ptr, d_pose.ptr, d_p3.ptr, graph=not args.no_graph)
+        tr.synchronize()
+        dt = time.perf_counter() - t0
+        dev_ms = tr.last_run_ms()
+        n_c = tr.n_c
+        tr.close()
+    evals = 2.0 * N * T
+    bpp = tracker_bytes_per_particle()
+    achieved = bpp * evals / (dev_ms * 1e-3) / 1e9
+    lml = d_lml.get().reshape(2, T)
+    out = {
+        "metric": "likelihood evals/s, pose tracker frame (both arms, proposals + association + updates + pose)",
+        "value": evals / dt,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": T,
+        "warmup": 1,
+        "ms_per_step": dt / T * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 proposals / f32 state",
+        "data": "shipped GMM models (data13D/data23D), synthetic likelihood images and face ROIs, Philox words",
+        "config": {"workload": "pose tracker: 2 arms x N=%d particles, %d proposals per arm per frame, "
+                               "480 x 640 likelihood, %d frames" % (N, n_c, T),
+                   "particles_per_arm": N, "proposals_per_arm": n_c, "frames": T},
+        "frames_per_s": T / dt,
+        "proposals_per_s": 2.0 * n_c * T / dt,
+        "device_ms_per_frame": dev_ms / T,
+        "lml_mean": float(np.mean(lml)),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "whole frame (all kernels of both arms, device events)", "bytes_per_particle": bpp},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_tracker(left, right, N, imgs, rois, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+def cpu_baseline_tracker(left, right, N, imgs, rois, budget_s, seed=21):
+    """The FP64 oracle of the tracker frame (1 thread): blur, proposals,
+    weights, proposal resampling, measurements and both arm updates, N
+    particles per arm, frames until the budget is spent (at least 1)."""
+    import oracle
+    from mkfbodytracker_pdaf_amd import capi
+    oracle.set_threads(1)
+    arms, Hp, M, states = [], [], [], []
+    for a, path in enumerate((left, right)):
+        means = capi.yml_read_matrix(path, "means")
+        covs = capi.yml_read_matrix(path, "covs")
+        w = capi.yml_read_matrix(path, "weights").ravel()
+        g = capi.yml_read_matrix(path, "gamma").ravel()
+        P = capi.yml_read_matrix(path, "pca_proj")
+        m = capi.yml_read_matrix(path, "pca_mean").ravel()
+        K, d = means.shape
+        kf = oracle.load_gaussian(means[0], covs[:d], P, m, g[0], np.array([9, 10, 0, 1, 12, 13], np.int32),
+                                  100.0 * np.eye(6))
+        mdl = oracle.Model(w, g, means, covs.reshape(K, d, d), kf["H"], kf["BH"], kf["R"])
+        bins, _ = oracle.resample_linear(w, N, int(oracle.philox_u32(seed, 32 + a, 0, 1)[0]))
+        arms.append(mdl); Hp.append(P.T); M.append(m)
+        states.append(mdl.reset(bins))
+    orc = oracle.PoseTrackerOracle(arms, Hp, M, N, seed=seed)
+    frames = 0
+    t0 = time.perf_counter()
+    while True:
+        g = frames
+        xbars = [oracle.rbpf_estimate(x) for x, _ in states]
+        st = orc.proposal_stage(xbars, imgs[g % len(imgs)], rois[g % len(rois)], g, uniform=(g == 0))
+        res = orc.update_arms(states, st["Z"], g)
+        states = [(r["x"], r["P"]) for r in res]
+        orc.pose([oracle.rbpf_estimate(x) for x, _ in states])
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": 2.0 * N * frames / el, "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": f"{frames} tracker frames x 2 arms x N={N} (10 N proposals per arm), FP64 oracle, 1 thread",
+            "cpu": _cpu_model(), "host_nproc": os.cpu_count()}
 
 
 def cpu_baseline(mdl, seq, N, cap, lam, budget_s, seed=2024):

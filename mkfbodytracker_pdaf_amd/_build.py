@@ -54,15 +54,34 @@ def build_adjoint_local_test() -> str:
     return ADJ_TEST_BIN
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every source under csrc/ into libmkfbt.so (always rebuilds)."""
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    """Compile every source under csrc/ into libmkfbt.so (always rebuilds):
+    one hipcc per source in parallel, then one link."""
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(HERE, "..", "include"), *sources(), "-o", tmp]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + res.stderr[-8000:])
-    if verbose and res.stderr:
-        print(res.stderr)
+    inc = ["-I", os.path.join(HERE, "..", "include")]
+    cflags = [f for f in FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="mkfbt_build_") as objdir:
+        srcs = sources()
+        objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
+
+        def compile_one(so):
+            src, obj = so
+            return subprocess.run([HIPCC, *cflags, *inc, "-c", src, "-o", obj], capture_output=True, text=True)
+
+        n = jobs or min(len(srcs), os.cpu_count() or 1, 16)
+        with ThreadPoolExecutor(max_workers=max(1, n)) as ex:
+            results = list(ex.map(compile_one, zip(srcs, objs)))
+        for src, res in zip(srcs, results):
+            if res.returncode != 0:
+                raise RuntimeError(f"hipcc failed on {os.path.basename(src)}:\n" + res.stderr[-8000:])
+            if verbose and res.stderr:
+                print(res.stderr)
+        res = subprocess.run([HIPCC, *FLAGS, *objs, "-o", tmp], capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError("link failed:\n" + res.stderr[-8000:])
     os.replace(tmp, LIB)
     if LIB.endswith("libmkfbt.so") and os.path.exists(CPP_TEST_SRC):
         build_cpp_test()

@@ -780,9 +780,12 @@ int pf_batch_estimate(pf_batch *b, int32_t s, double *xbar)
     const int i = b->rank[s];
     const SeqDesc &sd = b->h_seqs[i];
     const int buf = sd.T & 1;
-    auto est_kernel = b->D == 24 ? k_estimate<24> : k_estimate<kBD>;
-    hipLaunchKernelGGL(est_kernel, dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->bx[buf] + sd.pofs,
-                       b->anc + sd.pofs, sd.N, b->ld, b->partial, b->fs, b->Tis + (size_t)i * b->D * b->D, b->est);
+    if (b->D == 24)
+        launch_estimate<24>(b->stream, b->bx[buf] + sd.pofs, b->anc + sd.pofs, sd.N, b->ld, b->partial,
+                            b->Tis + (size_t)i * b->D * b->D, b->est);
+    else
+        launch_estimate<kBD>(b->stream, b->bx[buf] + sd.pofs, b->anc + sd.pofs, sd.N, b->ld, b->partial,
+                             b->Tis + (size_t)i * b->D * b->D, b->est);
     HIP_TRYB(hipGetLastError());
     HIP_TRYB(hipMemcpyAsync(xbar, b->est, sizeof(double) * b->D, hipMemcpyDeviceToHost, b->stream));
     HIP_TRYB(hipStreamSynchronize(b->stream));

@@ -251,8 +251,7 @@ template <int D>
 struct EstimateL {
     static void run(pf_handle *h)
     {
-        hipLaunchKernelGGL((k_estimate<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->xA,
-                           h->anc, h->N, h->ld, h->est_partial, h->fs, h->Tinv, h->est_out);
+        launch_estimate<D>(h->stream, h->xA, h->anc, h->N, h->ld, h->est_partial, h->Tinv, h->est_out);
     }
 };
 

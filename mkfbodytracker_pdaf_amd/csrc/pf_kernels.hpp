@@ -121,8 +121,15 @@ __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, 
     unsigned long long *misc = stg + kStageBlocks * kWeightsBlock;
     const int per = (nb + nt - 1) / nt;
     const int c0 = min(nb, tid * per), c1 = min(nb, c0 + per);
+    // loads in groups of 8 so that their latencies overlap
     unsigned long long loc = 0;
-    for (int k = c0; k < c1; ++k) loc += pd.bsum[k];
+    for (int k = c0; k < c1; k += 8) {
+        unsigned long long t8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t8[u] = (k + u < c1) ? pd.bsum[k + u] : 0ull;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) loc += t8[u];
+    }
     unsigned long long sc = loc;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -133,7 +140,14 @@ __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, 
     // blocks l, l + 64, ...), so it does not depend on the block size.
     double es = 0.0;
     if (want_es && wid == 0) {
-        for (int k = lane; k < nb; k += 64) es += pd.bexp[k];
+        for (int k = lane; k < nb; k += 8 * 64) {       // same order, 8 loads in flight
+            double t8[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) t8[u] = (k + 64 * u < nb) ? pd.bexp[k + 64 * u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k + 64 * u < nb) es += t8[u];
+        }
         es = wave_sum(es);
     }
     __syncthreads();                     // LDS of a previous use is free
@@ -145,9 +159,16 @@ __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, 
     unsigned long long total = 0;
     for (int k = 0; k < nwaves; ++k) total += misc[k];
     if (want_es) es_out = reinterpret_cast<double *>(stg)[0];
-    for (int k = c0; k < c1; ++k) {
-        pre[k] = run;
-        run += pd.bsum[k];
+    for (int k = c0; k < c1; k += 8) {
+        unsigned long long t8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t8[u] = (k + u < c1) ? pd.bsum[k + u] : 0ull;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k + u < c1) {
+                pre[k + u] = run;
+                run += t8[u];
+            }
     }
     if (tid == 0) pre[nb] = total;
     __syncthreads();
@@ -189,7 +210,16 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
     if (bhi - blo < kStageBlocks) {
         const int base = blo * kWeightsBlock;
         const int n = min(pd.L, (bhi + 1) * kWeightsBlock) - base;
-        for (int k = tid; k < n; k += nt) stg[k] = pre[blo + k / kWeightsBlock] + pd.cloc[base + k];
+        for (int k0 = tid; k0 < n; k0 += 4 * nt) {      // 4 loads in flight per lane
+            unsigned long long c4[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c4[u] = (k0 + u * nt < n) ? pd.cloc[base + k0 + u * nt] : 0ull;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = k0 + u * nt;
+                if (k < n) stg[k] = pre[blo + k / kWeightsBlock] + c4[u];
+            }
+        }
         __syncthreads();
         int l = (bj - blo) * kWeightsBlock, h = min(n, (bj - blo + 1) * kWeightsBlock) - 1;
         while (l < h) {
@@ -774,7 +804,7 @@ static __global__ void __launch_bounds__(256) k_ancestors(AncArgs a)
     const int j0 = blockIdx.x * blockDim.x;
     const int j = j0 + threadIdx.x;
     double es = 0.0;
-    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0, es);
+    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0 && a.lml, es);
     if (j < a.N) a.anc[j] = src;
     if (blockIdx.x == 0 && threadIdx.x == 0)
         finish_resampling(a.fs, a.parity, es, a.N, a.kind_linear,
@@ -943,52 +973,85 @@ __global__ void __launch_bounds__(256) k_export_rows(const float *xs, const floa
     }
 }
 
-// getEstimator (pf2DRao.cpp:23-31): xbar = sum_j x[anc_j] / N, FP64
-// accumulation, fixed reduction order (block partials, then the last block).
+// Sum over the 256 threads of a block of D values per thread, in a fixed
+// order: the values go to LDS, lane (i, k) adds the 16 values of threads
+// 16k .. 16k + 15 for dimension i in order, one lane per dimension adds the
+// 16 run sums in order.  Result in s_out (all threads, after the call).
 template <int D>
-__global__ void __launch_bounds__(256) k_estimate(const float *x, const int *anc, int N, int ld,
-                                                  double *partial, FrameScratch *fs, const double *Ti,
-                                                  double *out)
+__device__ __forceinline__ void block_sum256(const double (&v)[D], double (*s_v)[16][17], double (*s_run)[16],
+                                             double *s_out)
 {
-    __shared__ double s_w[4][D];
-    __shared__ int s_last;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s_v[i][t >> 4][t & 15] = v[i];
+    __syncthreads();
+    for (int q = t; q < D * 16; q += 256) {
+        const int i = q >> 4, k = q & 15;
+        double r = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) r += s_v[i][k][u];
+        s_run[i][k] = r;
+    }
+    __syncthreads();
+    if (t < D) {
+        double r = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) r += s_run[t][k];
+        s_out[t] = r;
+    }
+    __syncthreads();
+}
+
+// getEstimator (pf2DRao.cpp:23-31): xbar = sum_j x[anc_j] / N, FP64
+// accumulation in a fixed order: block sums (block_sum256) into partial,
+// then k_estimate_final adds them (thread t takes blocks t, t + 256, ... in
+// order, then block_sum256).  256 threads per block.
+template <int D>
+__global__ void __launch_bounds__(256) k_estimate(const float *x, const int *anc, int N, int ld, double *partial)
+{
+    __shared__ double s_v[D][16][17];
+    __shared__ double s_run[D][16];
+    __shared__ double s_sum[D];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double v[D];
     const int s = (j < N) ? (anc ? anc[j] : j) : 0;
 #pragma unroll
     for (int i = 0; i < D; ++i) v[i] = (j < N) ? (double)x[(size_t)i * ld + s] : 0.0;
+    block_sum256<D>(v, s_v, s_run, s_sum);
+    if (threadIdx.x < D) partial[(size_t)blockIdx.x * D + threadIdx.x] = s_sum[threadIdx.x];
+}
+
+// Sum of the nb block partials, / N, back to the model basis (xbar = Ti xbar').
+template <int D>
+__global__ void __launch_bounds__(256) k_estimate_final(const double *partial, int nb, int N, const double *Ti,
+                                                        double *out)
+{
+    __shared__ double s_v[D][16][17];
+    __shared__ double s_run[D][16];
+    __shared__ double s_sum[D];
+    double v[D];
 #pragma unroll
-    for (int i = 0; i < D; ++i) {
-        const double r = wave_sum(v[i]);
-        if (lane == 0) s_w[wid][i] = r;
+    for (int i = 0; i < D; ++i) v[i] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) v[i] += partial[(size_t)b * D + i];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < D; ++i) {
-            double r = 0.0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += s_w[w][i];
-            partial[(size_t)blockIdx.x * D + i] = r;
-        }
-        __threadfence();
-        s_last = (atomicAdd(&fs->done_est, 1u) == gridDim.x - 1);
-        if (s_last) __threadfence();
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __shared__ double s_mean[D];
+    block_sum256<D>(v, s_v, s_run, s_sum);
     if (threadIdx.x < D) {
         double r = 0.0;
-        for (int b = 0; b < (int)gridDim.x; ++b) r += ((volatile double *)partial)[(size_t)b * D + threadIdx.x];
-        s_mean[threadIdx.x] = r / (double)N;
-    }
-    __syncthreads();
-    if (threadIdx.x < D) {          // back to the model basis: xbar = Ti xbar'
-        double r = 0.0;
-        for (int k = 0; k < D; ++k) r = fma(Ti[threadIdx.x * D + k], s_mean[k], r);
+        for (int k = 0; k < D; ++k) r = fma(Ti[threadIdx.x * D + k], s_sum[k] / (double)N, r);
         out[threadIdx.x] = r;
     }
-    if (threadIdx.x == 0) fs->done_est = 0u;
+}
+
+// Both estimate kernels on one stream.
+template <int D>
+inline void launch_estimate(hipStream_t st, const float *x, const int *anc, int N, int ld, double *partial,
+                            const double *Ti, double *out)
+{
+    const int nb = (N + 255) / 256;
+    hipLaunchKernelGGL((k_estimate<D>), dim3(nb), dim3(256), 0, st, x, anc, N, ld, partial);
+    hipLaunchKernelGGL((k_estimate_final<D>), dim3(1), dim3(256), 0, st, (const double *)partial, nb, N, Ti, out);
 }
 
 // getSamples (pf2DRao.cpp:85-103): out = y[0:2] + 0.8 scale n, y = Hp xbar + M.

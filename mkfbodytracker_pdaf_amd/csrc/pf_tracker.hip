@@ -42,7 +42,7 @@ namespace {
 constexpr uint32_t kStreamProposal = 16;  // + arm
 constexpr uint32_t kStreamFrame = 24;     // 8 words per frame
 constexpr int kMaxRadius = 15;
-constexpr int kBlurW = 64, kBlurH = 32;
+constexpr int kBlurW = 32, kBlurH = 16, kBlurThreads = 128;
 constexpr int kProposeBlock = 256;
 constexpr int kMaxProposals = 1 << 20;    // resampler scan limit
 
@@ -72,7 +72,7 @@ struct BlurArgs {
     const float *taps;        // 2 radius + 1
     const int *frame;         // source frame = *frame (nullptr: 0)
     long long frame_stride;
-    int rows, cols, radius;
+    int rows, cols;
 };
 
 __device__ __forceinline__ int reflect101(int i, int n)
@@ -84,14 +84,15 @@ __device__ __forceinline__ int reflect101(int i, int n)
 
 // GaussianBlur(ksize, sigma, BORDER_REFLECT_101) (pfPose.cpp:213): row pass
 // then column pass, each an FP32 fma chain over the taps in index order,
-// round to nearest even and saturate to uint8.  One 64 x 32 output tile per
-// block, source tile and row-pass results in LDS.
-__global__ void __launch_bounds__(256) k_blur(BlurArgs a)
+// round to nearest even and saturate to uint8.  One 32 x 16 output tile per
+// block (1200 blocks for 640 x 480), source tile and row-pass results in LDS.
+template <int R>
+__global__ void __launch_bounds__(kBlurThreads) k_blur(BlurArgs a)
 {
-    __shared__ uint8_t s_in[(kBlurH + 2 * kMaxRadius) * (kBlurW + 2 * kMaxRadius)];
-    __shared__ float s_row[(kBlurH + 2 * kMaxRadius) * kBlurW];
-    __shared__ float s_tap[2 * kMaxRadius + 1];
-    const int R = a.radius, IW = kBlurW + 2 * R, IH = kBlurH + 2 * R, NT = 2 * R + 1;
+    constexpr int IW = kBlurW + 2 * R, IH = kBlurH + 2 * R, NT = 2 * R + 1;
+    __shared__ uint8_t s_in[IH * IW];
+    __shared__ float s_row[IH * kBlurW];
+    __shared__ float s_tap[NT];
     const int x0 = blockIdx.x * kBlurW, y0 = blockIdx.y * kBlurH;
     const long long f = a.frame ? (long long)*a.frame : 0;
     const uint8_t *src = a.src + f * a.frame_stride;
@@ -106,6 +107,7 @@ __global__ void __launch_bounds__(256) k_blur(BlurArgs a)
         const int iy = e / kBlurW, ox = e - iy * kBlurW;
         const uint8_t *p = s_in + iy * IW + ox;
         float acc = 0.f;
+#pragma unroll
         for (int k = 0; k < NT; ++k) acc = fmaf(s_tap[k], (float)p[k], acc);
         s_row[e] = acc;
     }
@@ -116,6 +118,7 @@ __global__ void __launch_bounds__(256) k_blur(BlurArgs a)
         if (gy >= a.rows || gx >= a.cols) continue;
         const float *p = s_row + oy * kBlurW + ox;
         float acc = 0.f;
+#pragma unroll
         for (int k = 0; k < NT; ++k) acc = fmaf(s_tap[k], p[k * kBlurW], acc);
         a.dst[(size_t)gy * a.cols + gx] = (uint8_t)fminf(fmaxf(rintf(acc), 0.f), 255.f);
     }
@@ -168,12 +171,12 @@ __device__ inline void proposal(const PropParams &p, int arm, long long g, bool 
 // N((px, py); y, var I) (pf2DRao.cpp:69-83,105-122): getSampleProb takes
 // spread x width as the variance, while getSamples draws with it as the
 // standard deviation (pf2DRao.cpp:90 vs :114).
-__device__ inline double density(double px, double py, double y0, double y1, double var, double inv2piv)
+__device__ inline double density(double px, double py, double y0, double y1, double nhiv, double inv2piv)
 {
 #pragma clang fp contract(off)
     const double d0 = px - y0, d1 = py - y1;
     const double q = fma(d0, d0, d1 * d1);
-    return inv2piv * det_exp((-0.5 * q) / var);
+    return inv2piv * det_exp(q * nhiv);
 }
 
 __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev a0, ArmDev a1, const uint8_t *img,
@@ -202,7 +205,7 @@ __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev 
     const double yo0 = s_y[2 * arm], yo1 = s_y[2 * arm + 1];
     const double yt0 = s_y[2 * (1 - arm)], yt1 = s_y[2 * (1 - arm) + 1];
     const double var = p.spread * (double)roi[2];
-    const double inv2piv = 1.0 / (6.283185307179586 * var);
+    const double inv2piv = 1.0 / (6.283185307179586 * var), nhiv = -0.5 / var;
     const long long k = (long long)blockIdx.x * kProposeBlock + threadIdx.x;
     double wv[2] = {0.0, 0.0};
     if (2 * k < p.n_c) {
@@ -212,11 +215,10 @@ __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev 
             // pfPose.cpp:251-266 (arm 1) and :274-289 (arm 2)
             if (py > 0.0 && py < (double)p.rows && px > 0.0 && px < (double)p.cols) {
                 const unsigned v = img[(size_t)(int)py * p.cols + (int)px];
-                if (v != 0u) {
-                    const double L = (double)v / 255.0;
-                    const double po = density(px, py, yo0, yo1, var, inv2piv);
-                    const double pt = density(px, py, yt0, yt1, var, inv2piv);
-                    wv[i] = L / fma(p.pa, po, fma(p.pa, pt, p.zfloor));
+                if (v != 0u) {     // (v / 255) / Z with one division
+                    const double po = density(px, py, yo0, yo1, nhiv, inv2piv);
+                    const double pt = density(px, py, yt0, yt1, nhiv, inv2piv);
+                    wv[i] = (double)v / (255.0 * fma(p.pa, po, fma(p.pa, pt, p.zfloor)));
                 }
             }
         }
@@ -276,40 +278,55 @@ struct PoseArgs {
 
 // get3Dpose (pfPose.cpp:93-127): joint k of e at pixel (u, v) and depth d,
 // X = (K R)^-1 ((u d, v d, d) - K t) = R^T ((d (u - cx)/fx, d (v - cy)/fy, d) - t)
-// with R = rpy(e15..17) (pfPose.cpp:84-91) and t = e18..20.
-__device__ void pose3d(const double *e, double fx, double fy, double cx, double cy, double *out)
+// with R = rpy(e16, e17, e15) = R3(yaw) R2(pitch) R1(roll) (pfPose.cpp:84-91)
+// and t = e18..20.  Row r of R^T q is column r of R dotted with q.
+__device__ double pose3d_entry(const double *e, const double *tg, int r, int k, double fx, double fy, double cx,
+                               double cy)
 {
-    const double roll = e[16], pitch = e[17], yaw = e[15];
-    const double cr = cos(roll), sr = sin(roll), cp = cos(pitch), sp = sin(pitch), cyw = cos(yaw), syw = sin(yaw);
-    // R = R3(yaw) R2(pitch) R1(roll)
-    const double R[3][3] = {{cyw * cp, cyw * sp * sr - syw * cr, cyw * sp * cr + syw * sr},
-                            {syw * cp, syw * sp * sr + cyw * cr, syw * sp * cr - cyw * sr},
-                            {-sp, cp * sr, cp * cr}};
-    for (int k = 0; k < 5; ++k) {
-        const double u = e[3 * k], v = e[3 * k + 1], d = e[3 * k + 2];
-        const double q0 = d * (u - cx) / fx - e[18], q1 = d * (v - cy) / fy - e[19], q2 = d - e[20];
-        for (int r = 0; r < 3; ++r) out[r * 5 + k] = R[0][r] * q0 + R[1][r] * q1 + R[2][r] * q2;
+    const double cr = tg[0], sr = tg[1], cp = tg[2], sp = tg[3], cyw = tg[4], syw = tg[5];
+    double c0, c1, c2;     // column r of R
+    if (r == 0) {
+        c0 = cyw * cp; c1 = syw * cp; c2 = -sp;
+    } else if (r == 1) {
+        c0 = cyw * sp * sr - syw * cr; c1 = syw * sp * sr + cyw * cr; c2 = cp * sr;
+    } else {
+        c0 = cyw * sp * cr + syw * sr; c1 = syw * sp * cr - cyw * sr; c2 = cp * cr;
     }
+    const double u = e[3 * k], v = e[3 * k + 1], d = e[3 * k + 2];
+    const double q0 = d * (u - cx) / fx - e[18], q1 = d * (v - cy) / fy - e[19], q2 = d - e[20];
+    return c0 * q0 + c1 * q1 + c2 * q2;
 }
 
+// Pose output of both arms: e (one lane per row), the six angle functions
+// of each arm (one lane each), the 3 x 5 joint positions (one lane each).
 __global__ void __launch_bounds__(64) k_pose(PoseArgs a)
 {
     __shared__ double s_e[2][64];
-    __shared__ double s_p[2][15];
+    __shared__ double s_tg[2][6];
     const int t = *a.frame;
-    for (int arm = 0; arm < 2; ++arm) {
-        const ArmDev &A = a.arm[arm];
-        for (int r = threadIdx.x; r < a.npose; r += blockDim.x) s_e[arm][r] = project_row(A.hp, A.pm, A.xbar, A.D, r);
+    for (int i = threadIdx.x; i < 2 * a.npose; i += blockDim.x) {
+        const int arm = i / a.npose, r = i - arm * a.npose;
+        const ArmDev &A = arm ? a.arm[1] : a.arm[0];
+        s_e[arm][r] = project_row(A.hp, A.pm, A.xbar, A.D, r);
     }
     __syncthreads();
-    if (threadIdx.x < 2) pose3d(s_e[threadIdx.x], a.fx, a.fy, a.cx, a.cy, s_p[threadIdx.x]);
-    __syncthreads();
-    for (int arm = 0; arm < 2; ++arm) {
-        if (a.pose)
-            for (int r = threadIdx.x; r < a.npose; r += blockDim.x)
-                a.pose[((size_t)t * 2 + arm) * a.npose + r] = s_e[arm][r];
-        if (a.pos3d && threadIdx.x < 15) a.pos3d[((size_t)t * 2 + arm) * 15 + threadIdx.x] = s_p[arm][threadIdx.x];
+    if (threadIdx.x < 12) {
+        const int arm = threadIdx.x / 6, f = threadIdx.x % 6;
+        const double ang = s_e[arm][f < 2 ? 16 : (f < 4 ? 17 : 15)];   // roll, pitch, yaw
+        s_tg[arm][f] = (f & 1) ? sin(ang) : cos(ang);
     }
+    __syncthreads();
+    if (threadIdx.x < 30) {
+        const int arm = threadIdx.x / 15, i = threadIdx.x % 15, r = i / 5, k = i % 5;
+        const double v = pose3d_entry(s_e[arm], s_tg[arm], r, k, a.fx, a.fy, a.cx, a.cy);
+        if (a.pos3d) a.pos3d[((size_t)t * 2 + arm) * 15 + i] = v;
+    }
+    if (a.pose)
+        for (int i = threadIdx.x; i < 2 * a.npose; i += blockDim.x) {
+            const int arm = i / a.npose, r = i - arm * a.npose;
+            a.pose[((size_t)t * 2 + arm) * a.npose + r] = s_e[arm][r];
+        }
+    __syncthreads();
     if (threadIdx.x == 0) *a.frame = t + 1;
 }
 
@@ -405,9 +422,12 @@ int launch_blur(pf_tracker *tr, const uint8_t *src, const int *frame, uint8_t *d
     b.frame_stride = (long long)tr->cfg.rows * tr->cfg.cols;
     b.rows = tr->cfg.rows;
     b.cols = tr->cfg.cols;
-    b.radius = tr->radius;
     const dim3 grid((tr->cfg.cols + kBlurW - 1) / kBlurW, (tr->cfg.rows + kBlurH - 1) / kBlurH);
-    hipLaunchKernelGGL(k_blur, grid, dim3(256), 0, tr->s[0], b);
+    using K = void (*)(BlurArgs);
+    static const K kernels[kMaxRadius + 1] = {k_blur<0>, k_blur<1>, k_blur<2>, k_blur<3>, k_blur<4>, k_blur<5>,
+                                              k_blur<6>, k_blur<7>, k_blur<8>, k_blur<9>, k_blur<10>, k_blur<11>,
+                                              k_blur<12>, k_blur<13>, k_blur<14>, k_blur<15>};
+    hipLaunchKernelGGL(kernels[tr->radius], grid, dim3(kBlurThreads), 0, tr->s[0], b);
     return launch_check("k_blur");
 }
 
