@@ -16,7 +16,8 @@
  *      (pfPose.cpp:303-320).
  *   5. ParticleFilter::update of both arms (pfPose.cpp:325-326).
  *   6. The pose output e = pca_proj^T getEstimator() + pca_mean
- *      (pfPose.cpp:347-348) and get3Dpose (pfPose.cpp:93-127).
+ *      (pfPose.cpp:347-348), get3Dpose (pfPose.cpp:93-127) and the joint
+ *      positions publishTFtree and publish2Dpos send (pfPose.cpp:129-208).
  *
  * The tracker borrows two filter handles (include/pf_rbpf.h) with loaded
  * models and initialised tracks (pf_reset with the bins of pfPose.cpp:68-71);
@@ -78,21 +79,42 @@ int pf_tracker_destroy(pf_tracker *tr);
  * tracks are the caller's, e.g. pf_reset). */
 int pf_tracker_reset(pf_tracker *tr);
 
+/* Per-frame outputs; every pointer may be NULL.  For T frames:
+ *   lml     2 x T      log mean weight of each arm's update (arm-major)
+ *   pose    T x 2 x npose   e of each arm (pfPose.cpp:347-348)
+ *   pos3d   T x 2 x 15      get3Dpose of each arm, 3 x 5 row-major: rows
+ *                           x, y, z; columns hand, elbow, shoulder, head, neck
+ *   tf      T x 24     the joints of publishTFtree (pfPose.cpp:129-171)
+ *                      relative to its Head frame, in the order and axes
+ *                      extractTFdata records them (extractTFdata.cpp:87-103):
+ *                      Head, Left_Hand, Right_Hand, Left_Elbow, Right_Elbow,
+ *                      Right_Shoulder, Left_Shoulder, Neck, each (x, y, z);
+ *                      arm 1 is published as Right_*, arm 2 as Left_*
+ *   pose2d  T x 16     the 8 image points of publish2Dpos (pfPose.cpp:
+ *                      173-208): hand 1, hand 2, head, neck, elbow 1,
+ *                      elbow 2, shoulder 1, shoulder 2, each (x, y) */
+typedef struct pf_tracker_outputs {
+    double *lml;
+    double *pose;
+    double *pos3d;
+    double *tf;
+    double *pose2d;
+} pf_tracker_outputs;
+
 /* One frame with host buffers: likelihood rows x cols uint8, roi = face ROI
- * {x_offset, y_offset, width, height} (facetracking ROIs[0]).  Outputs (each
- * may be NULL): lml[2] (log mean weight of each arm's update), pose[2 x
- * npose] (e of each arm), pos3d[2 x 15] (get3Dpose of each arm, 3 x 5
- * row-major: rows x, y, z; columns hand, elbow, shoulder, head, neck). */
-int pf_tracker_frame(pf_tracker *tr, const uint8_t *likelihood, const int32_t roi[4], double *lml,
-                     double *pose, double *pos3d);
+ * {x_offset, y_offset, width, height} (facetracking ROIs[0]); outputs for
+ * T = 1 (host pointers).  roi == NULL is a frame without a face (the
+ * reference's callback else-branch, pfPose.cpp:370-377): no update, pose2d
+ * zero, and the next frame with a face starts from uniform proposals. */
+int pf_tracker_frame(pf_tracker *tr, const uint8_t *likelihood, const int32_t *roi,
+                     const pf_tracker_outputs *out);
 
 /* T frames back to back, all pointers on the device: likelihood T x rows x
- * cols, roi T x 4; outputs lml 2 x T (arm-major), pose T x 2 x npose, pos3d
- * T x 2 x 15 (pose and pos3d may be NULL).  flags bit 0: capture a frame
- * pair into a HIP graph and replay it.  Returns after enqueueing; call
- * pf_tracker_synchronize. */
+ * cols, roi T x 4 (every frame has a face), outputs as above.  flags bit 0:
+ * capture a frame pair into a HIP graph and replay it.  Returns after
+ * enqueueing; call pf_tracker_synchronize. */
 int pf_tracker_run_dev(pf_tracker *tr, int32_t T, const uint8_t *likelihood, const int32_t *roi,
-                       double *lml, double *pose, double *pos3d, int32_t flags);
+                       const pf_tracker_outputs *out, int32_t flags);
 int pf_tracker_synchronize(pf_tracker *tr);
 /* Wall time of the last pf_tracker_run_dev between its first and last
  * kernel (ms), valid after pf_tracker_synchronize. */
@@ -105,9 +127,10 @@ int pf_tracker_last_run_ms(pf_tracker *tr, double *ms);
 int pf_tracker_get_last(pf_tracker *tr, uint8_t *blurred, double *proposals, double *weights,
                         int32_t *bins, float *measurements);
 
-/* The pose output of the arms' current tracks without a filter update:
- * pose[2 x npose] and pos3d[2 x 15] as in pf_tracker_frame. */
-int pf_tracker_pose(pf_tracker *tr, double *pose, double *pos3d);
+/* The pose output of the arms' current tracks without a filter update
+ * (outputs pose, pos3d, tf and pose2d for T = 1, host pointers; lml is
+ * ignored). */
+int pf_tracker_pose(pf_tracker *tr, const pf_tracker_outputs *out);
 
 /* GaussianBlur(ksize, sigma, BORDER_REFLECT_101) of the tracker's kernel on
  * a host image of the tracker's size (the pipeline's step 1 alone). */

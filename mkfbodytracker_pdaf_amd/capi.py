@@ -67,6 +67,11 @@ class TrackerConfig(C.Structure):
                 ("fy", C.c_double), ("cx", C.c_double), ("cy", C.c_double), ("seed", C.c_uint64)]
 
 
+class TrackerOutputs(C.Structure):
+    _fields_ = [("lml", C.c_void_p), ("pose", C.c_void_p), ("pos3d", C.c_void_p), ("tf", C.c_void_p),
+                ("pose2d", C.c_void_p)]
+
+
 _lib = None
 _vp = C.c_void_p
 
@@ -143,12 +148,12 @@ def lib():
          _vp, _vp)
     _sig(L, "pf_tracker_destroy", C.c_int, _vp)
     _sig(L, "pf_tracker_reset", C.c_int, _vp)
-    _sig(L, "pf_tracker_frame", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp)
-    _sig(L, "pf_tracker_run_dev", C.c_int, _vp, i32, _vp, _vp, _vp, _vp, _vp, i32)
+    _sig(L, "pf_tracker_frame", C.c_int, _vp, _vp, _vp, C.POINTER(TrackerOutputs))
+    _sig(L, "pf_tracker_run_dev", C.c_int, _vp, i32, _vp, _vp, C.POINTER(TrackerOutputs), i32)
     _sig(L, "pf_tracker_synchronize", C.c_int, _vp)
     _sig(L, "pf_tracker_last_run_ms", C.c_int, _vp, C.POINTER(dbl))
     _sig(L, "pf_tracker_get_last", C.c_int, _vp, _vp, _vp, _vp, _vp, _vp)
-    _sig(L, "pf_tracker_pose", C.c_int, _vp, _vp, _vp)
+    _sig(L, "pf_tracker_pose", C.c_int, _vp, C.POINTER(TrackerOutputs))
     _sig(L, "pf_tracker_blur", C.c_int, _vp, _vp, _vp)
     _sig(L, "pf_tracker_num_proposals", C.c_int, _vp, C.POINTER(i32))
     _lib = L

@@ -272,6 +272,8 @@ struct PoseArgs {
     int *frame;               // t (incremented at the end)
     double *pose;             // frames x 2 x npose (nullable)
     double *pos3d;            // frames x 2 x 15 (nullable)
+    double *tf;               // frames x 24 (nullable)
+    double *pose2d;           // frames x 16 (nullable)
     int npose;
     double fx, fy, cx, cy;
 };
@@ -297,12 +299,57 @@ __device__ double pose3d_entry(const double *e, const double *tg, int r, int k, 
     return c0 * q0 + c1 * q1 + c2 * q2;
 }
 
+// Entry i of publishTFtree's joints relative to its Head frame (pfPose.cpp:
+// 129-171 as extractTFdata.cpp:87-103 records them): joint i / 3 of Head,
+// Left_Hand, Right_Hand, Left_Elbow, Right_Elbow, Right_Shoulder,
+// Left_Shoulder, Neck (arm 0 = Right_*, arm 1 = Left_*); the published
+// offsets map a camera-frame vector v to (v.x, v.z, -v.y), every rotation is
+// the identity, so a joint relative to Head is that map of (joint - head),
+// head and neck being the means of the two arms' head and neck columns.
+__device__ double tf_entry(const double (*p)[15], int i)
+{
+    const int jt = i / 3, c = i % 3;
+    const int ax = c == 0 ? 0 : (c == 1 ? 2 : 1);
+    const double head = (p[0][ax * 5 + 3] + p[1][ax * 5 + 3]) / 2.0;
+    double v;
+    switch (jt) {
+    case 0: return 0.0;
+    case 1: v = p[1][ax * 5 + 0]; break;
+    case 2: v = p[0][ax * 5 + 0]; break;
+    case 3: v = p[1][ax * 5 + 1]; break;
+    case 4: v = p[0][ax * 5 + 1]; break;
+    case 5: v = p[0][ax * 5 + 2]; break;
+    case 6: v = p[1][ax * 5 + 2]; break;
+    default: v = (p[0][ax * 5 + 4] + p[1][ax * 5 + 4]) / 2.0; break;
+    }
+    return c == 2 ? head - v : v - head;
+}
+
+// Entry i of publish2Dpos's points (pfPose.cpp:173-208): hand 1, hand 2,
+// head, neck (means of the arms), elbow 1, elbow 2, shoulder 1, shoulder 2.
+__device__ double pose2d_entry(const double (*e)[64], int i)
+{
+    const int pt = i >> 1, c = i & 1;
+    switch (pt) {
+    case 0: return e[0][c];
+    case 1: return e[1][c];
+    case 2: return 0.5 * (e[0][9 + c] + e[1][9 + c]);
+    case 3: return 0.5 * (e[0][12 + c] + e[1][12 + c]);
+    case 4: return e[0][3 + c];
+    case 5: return e[1][3 + c];
+    case 6: return e[0][6 + c];
+    default: return e[1][6 + c];
+    }
+}
+
 // Pose output of both arms: e (one lane per row), the six angle functions
-// of each arm (one lane each), the 3 x 5 joint positions (one lane each).
+// of each arm (one lane each), the 3 x 5 joint positions (one lane each),
+// then the TF-tree and 2-D point lists.
 __global__ void __launch_bounds__(64) k_pose(PoseArgs a)
 {
     __shared__ double s_e[2][64];
     __shared__ double s_tg[2][6];
+    __shared__ double s_p[2][15];
     const int t = *a.frame;
     for (int i = threadIdx.x; i < 2 * a.npose; i += blockDim.x) {
         const int arm = i / a.npose, r = i - arm * a.npose;
@@ -319,8 +366,12 @@ __global__ void __launch_bounds__(64) k_pose(PoseArgs a)
     if (threadIdx.x < 30) {
         const int arm = threadIdx.x / 15, i = threadIdx.x % 15, r = i / 5, k = i % 5;
         const double v = pose3d_entry(s_e[arm], s_tg[arm], r, k, a.fx, a.fy, a.cx, a.cy);
+        s_p[arm][i] = v;
         if (a.pos3d) a.pos3d[((size_t)t * 2 + arm) * 15 + i] = v;
     }
+    __syncthreads();
+    if (a.tf && threadIdx.x < 24) a.tf[(size_t)t * 24 + threadIdx.x] = tf_entry(s_p, threadIdx.x);
+    if (a.pose2d && threadIdx.x < 16) a.pose2d[(size_t)t * 16 + threadIdx.x] = pose2d_entry(s_e, threadIdx.x);
     if (a.pose)
         for (int i = threadIdx.x; i < 2 * a.npose; i += blockDim.x) {
             const int arm = i / a.npose, r = i - arm * a.npose;
@@ -360,7 +411,7 @@ struct pf_tracker {
     uint32_t *uni = nullptr;          // [arm 0: cap x 2][arm 1: cap x 2][proposals: cap x 2]
     int uni_cap = 0;
     std::vector<uint32_t> h_uni;
-    double *lml_stage = nullptr, *pose_stage = nullptr, *p3d_stage = nullptr;
+    pf_tracker_outputs stage_out{};     // device outputs of one host-driven frame
     hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr}, ev_pre = nullptr;
     hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
     long long frames_done = 0, uniform_frame = 0;
@@ -431,14 +482,16 @@ int launch_blur(pf_tracker *tr, const uint8_t *src, const int *frame, uint8_t *d
     return launch_check("k_blur");
 }
 
-int launch_pose(pf_tracker *tr, int *frame, double *pose, double *pos3d)
+int launch_pose(pf_tracker *tr, int *frame, const pf_tracker_outputs &out)
 {
     PoseArgs pa{};
     pa.arm[0] = arm_dev(tr, 0, 0);
     pa.arm[1] = arm_dev(tr, 1, 0);
     pa.frame = frame;
-    pa.pose = pose;
-    pa.pos3d = pos3d;
+    pa.pose = out.pose;
+    pa.pos3d = out.pos3d;
+    pa.tf = out.tf;
+    pa.pose2d = out.pose2d;
     pa.npose = tr->npose;
     pa.fx = tr->cfg.fx;
     pa.fy = tr->cfg.fy;
@@ -450,8 +503,7 @@ int launch_pose(pf_tracker *tr, int *frame, double *pose, double *pos3d)
 
 // One frame of the pipeline; `parity` alternates from frame to frame (it
 // selects the handles' frame counter and maximum slot).
-int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const int *roi, double *lml, double *pose,
-                  double *pos3d)
+int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const int *roi, const pf_tracker_outputs &out)
 {
     int st = launch_blur(tr, lik, tr->fctr, tr->blur);
     if (st) return st;
@@ -473,14 +525,15 @@ int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const i
                            (const int *)tr->bins[a], (const FrameRec *)tr->rec, tr->Z[a], tr->N);
         st = launch_check("k_build_meas");
         if (st) return st;
-        st = eng::step_ref(h, tr->Z[a], tr->uni + 2 * (size_t)tr->uni_cap * a, parity, lml + (size_t)a * T);
+        st = eng::step_ref(h, tr->Z[a], tr->uni + 2 * (size_t)tr->uni_cap * a, parity,
+                           out.lml ? out.lml + (size_t)a * T : nullptr);
         if (st) return st;
         st = eng::estimate(h);
         if (st) return st;
     }
     TR_TRY(hipEventRecord(tr->ev_join[parity], tr->s[1]));
     TR_TRY(hipStreamWaitEvent(tr->s[0], tr->ev_join[parity], 0));
-    return launch_pose(tr, tr->fctr, pose, pos3d);
+    return launch_pose(tr, tr->fctr, out);
 }
 
 // Frame uniforms of global frames [g0, g0 + T) into the device table.
@@ -521,6 +574,22 @@ int prepare_estimates(pf_tracker *tr)
     }
     TR_TRY(hipEventRecord(tr->ev_pre, tr->s[1]));
     TR_TRY(hipStreamWaitEvent(tr->s[0], tr->ev_pre, 0));
+    return PF_OK;
+}
+
+// Copies the staged device outputs of one frame to the host pointers of o
+// (each may be NULL) and waits for both streams.
+int copy_stage_out(pf_tracker *tr, const pf_tracker_outputs &o)
+{
+    const pf_tracker_outputs &d = tr->stage_out;
+    hipStream_t s = tr->s[0];
+    if (o.lml) TR_TRY(hipMemcpyAsync(o.lml, d.lml, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    if (o.pose) TR_TRY(hipMemcpyAsync(o.pose, d.pose, sizeof(double) * 2 * tr->npose, hipMemcpyDeviceToHost, s));
+    if (o.pos3d) TR_TRY(hipMemcpyAsync(o.pos3d, d.pos3d, sizeof(double) * 30, hipMemcpyDeviceToHost, s));
+    if (o.tf) TR_TRY(hipMemcpyAsync(o.tf, d.tf, sizeof(double) * 24, hipMemcpyDeviceToHost, s));
+    if (o.pose2d) TR_TRY(hipMemcpyAsync(o.pose2d, d.pose2d, sizeof(double) * 16, hipMemcpyDeviceToHost, s));
+    TR_TRY(hipStreamSynchronize(tr->s[0]));
+    TR_TRY(hipStreamSynchronize(tr->s[1]));
     return PF_OK;
 }
 
@@ -580,9 +649,11 @@ int pf_tracker_destroy(pf_tracker *tr)
     (void)hipFree(tr->fctr);
     (void)hipFree(tr->fctr_aux);
     (void)hipFree(tr->uni);
-    (void)hipFree(tr->lml_stage);
-    (void)hipFree(tr->pose_stage);
-    (void)hipFree(tr->p3d_stage);
+    (void)hipFree(tr->stage_out.lml);
+    (void)hipFree(tr->stage_out.pose);
+    (void)hipFree(tr->stage_out.pos3d);
+    (void)hipFree(tr->stage_out.tf);
+    (void)hipFree(tr->stage_out.pose2d);
     if (tr->ev_pre) (void)hipEventDestroy(tr->ev_pre);
     if (tr->ev_t0) (void)hipEventDestroy(tr->ev_t0);
     if (tr->ev_t1) (void)hipEventDestroy(tr->ev_t1);
@@ -686,9 +757,11 @@ int pf_tracker_create(pf_tracker **out, const pf_tracker_config *cfg, pf_handle 
     if (!st) st = hip_ok(hipMemset(tr->rec, 0, sizeof(FrameRec)), "hipMemset");
     if (!st) st = hip_ok(talloc(&tr->fctr, 1), "hipMalloc");
     if (!st) st = hip_ok(talloc(&tr->fctr_aux, 1), "hipMalloc");
-    if (!st) st = hip_ok(talloc(&tr->lml_stage, 2), "hipMalloc");
-    if (!st) st = hip_ok(talloc(&tr->pose_stage, (size_t)2 * npose), "hipMalloc");
-    if (!st) st = hip_ok(talloc(&tr->p3d_stage, 30), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->stage_out.lml, 2), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->stage_out.pose, (size_t)2 * npose), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->stage_out.pos3d, 30), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->stage_out.tf, 24), "hipMalloc");
+    if (!st) st = hip_ok(talloc(&tr->stage_out.pose2d, 16), "hipMalloc");
     if (!st) st = hip_ok(hipEventCreateWithFlags(&tr->ev_pre, hipEventDisableTiming), "hipEventCreate");
     if (!st) st = hip_ok(hipEventCreate(&tr->ev_t0), "hipEventCreate");
     if (!st) st = hip_ok(hipEventCreate(&tr->ev_t1), "hipEventCreate");
@@ -715,12 +788,14 @@ int pf_tracker_num_proposals(pf_tracker *tr, int32_t *n_c)
     return PF_OK;
 }
 
-int pf_tracker_run_dev(pf_tracker *tr, int32_t T, const uint8_t *likelihood, const int32_t *roi, double *lml,
-                       double *pose, double *pos3d, int32_t flags)
+int pf_tracker_run_dev(pf_tracker *tr, int32_t T, const uint8_t *likelihood, const int32_t *roi,
+                       const pf_tracker_outputs *outputs, int32_t flags)
 {
     int st = tr_check(tr);
     if (st) return st;
-    if (T <= 0 || !likelihood || !roi || !lml) return set_error(PF_ERR_ARG, "bad run arguments");
+    if (T <= 0 || !likelihood || !roi) return set_error(PF_ERR_ARG, "bad run arguments");
+    pf_tracker_outputs out{};
+    if (outputs) out = *outputs;
     for (int a = 0; a < 2; ++a) {
         eng::Info in;
         st = eng::info(tr->arm[a], &in);
@@ -737,7 +812,7 @@ int pf_tracker_run_dev(pf_tracker *tr, int32_t T, const uint8_t *likelihood, con
     for (int a = 0; !st && a < 2; ++a) st = eng::set_frame(tr->arm[a], 0);
     if (!st) st = prepare_estimates(tr);
     if (st) return st;
-    auto frame = [&](int f) { return enqueue_frame(tr, f & 1, T, likelihood, roi, lml, pose, pos3d); };
+    auto frame = [&](int f) { return enqueue_frame(tr, f & 1, T, likelihood, roi, out); };
     st = frame(0);
     if (st) return st;
     const int pairs = graph ? (T - 1) / 2 : 0;
@@ -788,24 +863,26 @@ int pf_tracker_last_run_ms(pf_tracker *tr, double *ms)
     return PF_OK;
 }
 
-int pf_tracker_frame(pf_tracker *tr, const uint8_t *likelihood, const int32_t roi[4], double *lml, double *pose,
-                     double *pos3d)
+int pf_tracker_frame(pf_tracker *tr, const uint8_t *likelihood, const int32_t *roi, const pf_tracker_outputs *out)
 {
     int st = tr_check(tr);
     if (st) return st;
-    if (!likelihood || !roi) return set_error(PF_ERR_ARG, "null image or ROI");
+    pf_tracker_outputs o{};
+    if (out) o = *out;
+    if (!roi) {            // no face: pfPose.cpp:370-377
+        tr->uniform_frame = tr->frames_done;
+        if (o.pose2d) std::memset(o.pose2d, 0, sizeof(double) * 16);
+        return PF_OK;
+    }
+    if (!likelihood) return set_error(PF_ERR_ARG, "null image");
     if (roi[2] <= 0 || roi[3] <= 0) return set_error(PF_ERR_ARG, "empty face ROI");
     const size_t px = (size_t)tr->cfg.rows * tr->cfg.cols;
     TR_TRY(hipMemcpyAsync(tr->img_stage, likelihood, px, hipMemcpyHostToDevice, tr->s[0]));
     TR_TRY(hipMemcpyAsync(tr->roi_stage, roi, sizeof(int32_t) * 4, hipMemcpyHostToDevice, tr->s[0]));
     TR_TRY(hipStreamSynchronize(tr->s[0]));
-    st = pf_tracker_run_dev(tr, 1, tr->img_stage, tr->roi_stage, tr->lml_stage, tr->pose_stage, tr->p3d_stage, 0);
+    st = pf_tracker_run_dev(tr, 1, tr->img_stage, tr->roi_stage, &tr->stage_out, 0);
     if (st) return st;
-    if (lml) TR_TRY(hipMemcpyAsync(lml, tr->lml_stage, sizeof(double) * 2, hipMemcpyDeviceToHost, tr->s[0]));
-    if (pose)
-        TR_TRY(hipMemcpyAsync(pose, tr->pose_stage, sizeof(double) * 2 * tr->npose, hipMemcpyDeviceToHost, tr->s[0]));
-    if (pos3d) TR_TRY(hipMemcpyAsync(pos3d, tr->p3d_stage, sizeof(double) * 30, hipMemcpyDeviceToHost, tr->s[0]));
-    st = pf_tracker_synchronize(tr);
+    st = copy_stage_out(tr, o);
     if (st) return st;
     return status_of_arms(tr);
 }
@@ -847,7 +924,7 @@ int pf_tracker_get_last(pf_tracker *tr, uint8_t *blurred, double *proposals, dou
     return PF_OK;
 }
 
-int pf_tracker_pose(pf_tracker *tr, double *pose, double *pos3d)
+int pf_tracker_pose(pf_tracker *tr, const pf_tracker_outputs *out)
 {
     int st = tr_check(tr);
     if (st) return st;
@@ -861,12 +938,12 @@ int pf_tracker_pose(pf_tracker *tr, double *pose, double *pos3d)
     if (st) return st;
     hipLaunchKernelGGL(k_tr_set, dim3(1), dim3(64), 0, tr->s[0], tr->fctr_aux, 0);
     st = launch_check("k_tr_set");
-    if (!st) st = launch_pose(tr, tr->fctr_aux, tr->pose_stage, tr->p3d_stage);
+    if (!st) st = launch_pose(tr, tr->fctr_aux, tr->stage_out);
     if (st) return st;
-    if (pose)
-        TR_TRY(hipMemcpyAsync(pose, tr->pose_stage, sizeof(double) * 2 * tr->npose, hipMemcpyDeviceToHost, tr->s[0]));
-    if (pos3d) TR_TRY(hipMemcpyAsync(pos3d, tr->p3d_stage, sizeof(double) * 30, hipMemcpyDeviceToHost, tr->s[0]));
-    return pf_tracker_synchronize(tr);
+    pf_tracker_outputs o{};
+    if (out) o = *out;
+    o.lml = nullptr;
+    return copy_stage_out(tr, o);
 }
 
 int pf_tracker_blur(pf_tracker *tr, const uint8_t *src, uint8_t *dst)

@@ -83,29 +83,35 @@ class PoseTracker:
         check(self.lib.pf_tracker_reset(self.h))
 
     # ---------------------------------------------------------- per frame
+    def _host_outputs(self):
+        o = dict(lml=np.zeros(2), pose=np.zeros((2, self.npose)), pos3d=np.zeros((2, 3, 5)),
+                 tf=np.zeros((8, 3)), pose2d=np.zeros((8, 2)))
+        st = capi.TrackerOutputs(ptr(o["lml"]), ptr(o["pose"]), ptr(o["pos3d"]), ptr(o["tf"]), ptr(o["pose2d"]))
+        return o, st
+
     def callback(self, likelihood, roi):
         """One frame: likelihood (rows x cols uint8), roi = (x_offset,
-        y_offset, width, height) of the face.  Returns dict(lml (2,), pose
-        (2 x npose), pos3d (2 x 3 x 5), status)."""
+        y_offset, width, height) of the face, or None for a frame without a
+        face.  Returns dict(lml (2,), pose (2 x npose), pos3d (2 x 3 x 5),
+        tf (8 x 3, joints relative to Head), pose2d (8 x 2), status)."""
         img = np.ascontiguousarray(likelihood, np.uint8)
         if img.shape != (self.cfg.rows, self.cfg.cols):
             raise ValueError(f"likelihood image must be {self.cfg.rows} x {self.cfg.cols}")
-        r = np.ascontiguousarray(roi, np.int32)
-        lml = np.zeros(2)
-        pose = np.zeros((2, self.npose))
-        p3 = np.zeros((2, 3, 5))
-        st = check(self.lib.pf_tracker_frame(self.h, ptr(img), ptr(r), ptr(lml), ptr(pose), ptr(p3)))
-        return dict(lml=lml, pose=pose, pos3d=p3, status=st)
+        o, st_out = self._host_outputs()
+        r = None if roi is None else np.ascontiguousarray(roi, np.int32)
+        st = check(self.lib.pf_tracker_frame(self.h, ptr(img), ptr(r), C.byref(st_out)))
+        o["status"] = st
+        return o
 
     frame = callback
 
     def pose(self):
-        """e = pca_proj^T xbar + pca_mean and get3Dpose of both arms' current
-        tracks (pfPose.cpp:347-348, 93-127)."""
-        pose = np.zeros((2, self.npose))
-        p3 = np.zeros((2, 3, 5))
-        check(self.lib.pf_tracker_pose(self.h, ptr(pose), ptr(p3)))
-        return pose, p3
+        """Pose outputs of both arms' current tracks (pfPose.cpp:347-348,
+        93-208): dict(pose, pos3d, tf, pose2d)."""
+        o, st_out = self._host_outputs()
+        check(self.lib.pf_tracker_pose(self.h, C.byref(st_out)))
+        del o["lml"]
+        return o
 
     def get_last(self):
         rows, cols = self.cfg.rows, self.cfg.cols
@@ -124,9 +130,10 @@ class PoseTracker:
         return out
 
     # ------------------------------------------------ device-resident run
-    def run_dev(self, T, lik_ptr, roi_ptr, lml_ptr, pose_ptr=None, pos3d_ptr=None, graph=True):
-        check(self.lib.pf_tracker_run_dev(self.h, int(T), lik_ptr, roi_ptr, lml_ptr, pose_ptr, pos3d_ptr,
-                                          1 if graph else 0))
+    def run_dev(self, T, lik_ptr, roi_ptr, lml_ptr=None, pose_ptr=None, pos3d_ptr=None, tf_ptr=None,
+                pose2d_ptr=None, graph=True):
+        out = capi.TrackerOutputs(lml_ptr, pose_ptr, pos3d_ptr, tf_ptr, pose2d_ptr)
+        check(self.lib.pf_tracker_run_dev(self.h, int(T), lik_ptr, roi_ptr, C.byref(out), 1 if graph else 0))
 
     def synchronize(self):
         check(self.lib.pf_tracker_synchronize(self.h))

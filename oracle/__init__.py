@@ -99,6 +99,8 @@ def lib():
         L.orc_tracker_measurement.argtypes = [C.c_int, _ip, C.c_int, _dp, _ip, _dp]
         L.orc_rpy.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
         L.orc_get3Dpose.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_tf_rows.argtypes = [_dp, _dp, _dp]
+        L.orc_pose2d.argtypes = [_dp, _dp, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         _lib = L
@@ -426,6 +428,19 @@ def get3Dpose(e, fx=525.0, fy=525.0, cx=319.5, cy=239.5):
     return out.reshape(3, 5)
 
 
+def tf_rows(p1, p2):
+    """publishTFtree's joints relative to Head, 8 x 3 (extractTFdata order)."""
+    out = np.zeros(24)
+    lib().orc_tf_rows(_f64(p1).ravel(), _f64(p2).ravel(), out)
+    return out.reshape(8, 3)
+
+
+def pose2d(e1, e2):
+    out = np.zeros(16)
+    lib().orc_pose2d(_f64(e1), _f64(e2), out)
+    return out.reshape(8, 2)
+
+
 def project(Hp, M, xbar):
     """y = Hp xbar + M with the products of each row summed in index order
     (the order of orc_get_samples and of the tracker's kernels)."""
@@ -489,6 +504,8 @@ class PoseTrackerOracle:
         return res
 
     def pose(self, xbars):
+        """e, get3Dpose, TF rows and 2-D points of both arms' estimates."""
         fx, fy, cx, cy = self.camera
         es = [project(self.Hp[a], self.M[a], xbars[a]) for a in range(2)]
-        return es, [get3Dpose(e, fx, fy, cx, cy) for e in es]
+        p3 = [get3Dpose(e, fx, fy, cx, cy) for e in es]
+        return es, p3, tf_rows(p3[0], p3[1]), pose2d(es[0], es[1])

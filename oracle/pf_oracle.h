@@ -226,6 +226,10 @@ void orc_tracker_measurement(int N, const int *bins, int n_c, const double *prop
 /* rpy and get3Dpose (pfPose.cpp:84-127). */
 void orc_rpy(double roll, double pitch, double yaw, double R[9]);
 int orc_get3Dpose(const double *e, double fx, double fy, double cx, double cy, double out[15]);
+/* publishTFtree joints relative to Head (extractTFdata layout) and the
+ * publish2Dpos points (pfPose.cpp:129-208). */
+void orc_tf_rows(const double p1[15], const double p2[15], double out[24]);
+void orc_pose2d(const double *e1, const double *e2, double out[16]);
 
 /* ------------------------------------------------------------- misc */
 /* Number of OpenMP threads used by the particle loops (1 = sequential). */

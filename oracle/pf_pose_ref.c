@@ -275,3 +275,65 @@ int orc_get3Dpose(const double *e, double fx, double fy, double cx, double cy, d
     }
     return 0;
 }
+
+/* publishTFtree (pfPose.cpp:129-171) as extractTFdata records it
+ * (extractTFdata.cpp:87-103): the published child-in-parent offsets
+ * (identity rotations; a camera-frame difference v is published as
+ * (v.x, v.z, -v.y)), composed from Head down each chain as a translation
+ * sum.  p1, p2: get3Dpose of arm 1 (published as Right_*) and arm 2
+ * (Left_*), 3 x 5 row-major.  out: Head, Left_Hand, Right_Hand, Left_Elbow,
+ * Right_Elbow, Right_Shoulder, Left_Shoulder, Neck, each (x, y, z). */
+static void tf_offset(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[0] - b[0];
+    o[1] = a[2] - b[2];
+    o[2] = -a[1] + b[1];
+}
+
+void orc_tf_rows(const double p1[15], const double p2[15], double out[24])
+{
+    double J1[5][3], J2[5][3], head[3], neck[3];
+    for (int k = 0; k < 5; ++k)
+        for (int r = 0; r < 3; ++r) {
+            J1[k][r] = p1[r * 5 + k];
+            J2[k][r] = p2[r * 5 + k];
+        }
+    for (int r = 0; r < 3; ++r) {
+        neck[r] = (J1[4][r] + J2[4][r]) / 2.0;
+        head[r] = (J1[3][r] + J2[3][r]) / 2.0;
+    }
+    double neck_in_head[3], sh_in_neck[2][3], el_in_sh[2][3], ha_in_el[2][3];
+    tf_offset(neck, head, neck_in_head);
+    const double(*J[2])[3] = {J1, J2};
+    for (int a = 0; a < 2; ++a) {
+        tf_offset(J[a][2], neck, sh_in_neck[a]);
+        tf_offset(J[a][1], J[a][2], el_in_sh[a]);
+        tf_offset(J[a][0], J[a][1], ha_in_el[a]);
+    }
+    /* rows: 0 Head, 1 LHand, 2 RHand, 3 LElbow, 4 RElbow, 5 RShoulder, 6 LShoulder, 7 Neck */
+    const int arm_of[8] = {-1, 1, 0, 1, 0, 0, 1, -1};
+    const int depth_of[8] = {0, 3, 3, 2, 2, 1, 1, 0};
+    for (int j = 0; j < 8; ++j)
+        for (int c = 0; c < 3; ++c) {
+            double v = 0.0;
+            if (j == 7) v = neck_in_head[c];
+            else if (j > 0) {
+                const int a = arm_of[j];
+                v = neck_in_head[c] + sh_in_neck[a][c];
+                if (depth_of[j] >= 2) v += el_in_sh[a][c];
+                if (depth_of[j] >= 3) v += ha_in_el[a][c];
+            }
+            out[j * 3 + c] = v;
+        }
+}
+
+/* publish2Dpos (pfPose.cpp:173-208): hand 1, hand 2, head, neck (means of
+ * the two arms), elbow 1, elbow 2, shoulder 1, shoulder 2, each (x, y). */
+void orc_pose2d(const double *e1, const double *e2, double out[16])
+{
+    const double pts[16] = {e1[0], e1[1], e2[0], e2[1],
+                            0.5 * (e1[9] + e2[9]), 0.5 * (e1[10] + e2[10]),
+                            0.5 * (e1[12] + e2[12]), 0.5 * (e1[13] + e2[13]),
+                            e1[3], e1[4], e2[3], e2[4], e1[6], e1[7], e2[6], e2[7]};
+    memcpy(out, pts, sizeof(pts));
+}

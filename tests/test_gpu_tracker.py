@@ -136,13 +136,20 @@ def test_frames_match_oracle(gpu, frames):
             assert _rel(xa, r["xu"][o["anc"]]).max() <= 1e-4 and _rel(Pa, r["Pu"][o["anc"]]).max() <= 1e-4
             assert abs(o["lml"] - r["lml"]) <= 1e-4 * max(1.0, abs(r["lml"])), (g, a)
         # pose output from the updated tracks
-        es, p3 = orc.pose([a.estimate() for a in arms])
+        es, p3, tf, p2d = orc.pose([a.estimate() for a in arms])
         for a in range(2):
             assert np.allclose(out["pose"][a], es[a], rtol=1e-12, atol=1e-12), (g, a)
             assert np.allclose(out["pos3d"][a], p3[a], rtol=1e-9, atol=1e-12), (g, a)
             assert np.allclose(out["pos3d"][a], oracle.get3Dpose(out["pose"][a]), rtol=1e-9, atol=1e-12)
-    pose, p3 = tr.pose()
-    assert np.array_equal(pose, out["pose"]) and np.array_equal(p3, out["pos3d"])
+        # TF rows and 2-D points from the GPU's own pose: the oracle's literal
+        # chain composition (sums of published offsets) and point list
+        assert np.allclose(out["tf"], oracle.tf_rows(out["pos3d"][0], out["pos3d"][1]), rtol=0, atol=1e-12)
+        assert np.allclose(out["tf"], tf, rtol=0, atol=1e-9)
+        assert np.array_equal(out["pose2d"], oracle.pose2d(out["pose"][0], out["pose"][1]))
+        assert np.allclose(out["pose2d"], p2d, rtol=1e-12, atol=1e-12)
+    po = tr.pose()
+    for k in ("pose", "pos3d", "tf", "pose2d"):
+        assert np.array_equal(po[k], out[k]), k
     tr.close()
     for a in arms + twins:
         a.close()
@@ -160,16 +167,22 @@ def test_device_run_equals_frame_calls(gpu, graph):
     d_lml = DeviceArray.empty(2 * T, np.float64)
     d_pose = DeviceArray.empty(T * 2 * 22, np.float64)
     d_p3 = DeviceArray.empty(T * 2 * 15, np.float64)
-    tr_b.run_dev(T, d_img.ptr, d_roi.ptr, d_lml.ptr, d_pose.ptr, d_p3.ptr, graph=graph)
+    d_tf = DeviceArray.empty(T * 24, np.float64)
+    d_p2 = DeviceArray.empty(T * 16, np.float64)
+    tr_b.run_dev(T, d_img.ptr, d_roi.ptr, d_lml.ptr, d_pose.ptr, d_p3.ptr, d_tf.ptr, d_p2.ptr, graph=graph)
     tr_b.synchronize()
     assert tr_b.last_run_ms() > 0
     lml = d_lml.get().reshape(2, T)
     pose = d_pose.get().reshape(T, 2, 22)
     p3 = d_p3.get().reshape(T, 2, 3, 5)
+    tf = d_tf.get().reshape(T, 8, 3)
+    p2 = d_p2.get().reshape(T, 8, 2)
     for t in range(T):
         assert np.array_equal(lml[:, t], outs[t]["lml"]), t
         assert np.array_equal(pose[t], outs[t]["pose"]), t
         assert np.array_equal(p3[t], outs[t]["pos3d"]), t
+        assert np.array_equal(tf[t], outs[t]["tf"]), t
+        assert np.array_equal(p2[t], outs[t]["pose2d"]), t
     for a in range(2):
         xa, Pa = arms_a[a].get_tracks()
         xb, Pb = arms_b[a].get_tracks()
@@ -180,6 +193,35 @@ def test_device_run_equals_frame_calls(gpu, graph):
     tr_b.synchronize()
     for a in range(2):
         assert np.array_equal(arms_a[a].get_tracks()[0], arms_b[a].get_tracks()[0])
+    for t in (tr_a, tr_b):
+        t.close()
+    for a in arms_a + arms_b:
+        a.close()
+
+
+def test_frame_without_face(gpu):
+    """A frame without a face ROI updates nothing, reports zero 2-D points
+    and makes the next frame start again from uniform proposals."""
+    N = 400
+    imgs, rois = synthetic.tracker_sequence(3, seed=12)
+    tr_a, arms_a = _tracker(N)
+    tr_b, arms_b = _tracker(N)
+    for t in range(2):
+        oa, ob = tr_a.callback(imgs[t], rois[t]), tr_b.callback(imgs[t], rois[t])
+    before = [a.get_tracks()[0] for a in arms_a]
+    none = tr_a.callback(imgs[2], None)
+    assert np.all(none["pose2d"] == 0.0) and none["status"] == 0
+    assert all(np.array_equal(a.get_tracks()[0], b) for a, b in zip(arms_a, before))
+    oa = tr_a.callback(imgs[2], rois[2])
+    last = tr_a.get_last()
+    x0, y0, w, h = (int(v) for v in rois[2])
+    for a in range(2):       # uniform box below the face
+        px, py = last["props"][a]
+        assert px.min() >= max(x0 - 4 * w, 0) and px.max() < min(x0 + 5 * w, 640)
+        assert py.min() >= min(y0 + h, 480) and py.max() < min(y0 + 7 * h, 480)
+    # the same frame index and words as a tracker reset before that frame
+    ob = tr_b.callback(imgs[2], rois[2])
+    assert not np.array_equal(oa["pose"], ob["pose"])    # b used Gaussian proposals
     for t in (tr_a, tr_b):
         t.close()
     for a in arms_a + arms_b:

@@ -206,3 +206,27 @@ def test_project_order():
     rng = np.random.default_rng(6)
     Hp, M, x = rng.standard_normal((22, 12)), rng.standard_normal(22), rng.standard_normal(12)
     assert np.allclose(oracle.project(Hp, M, x), Hp @ x + M, rtol=1e-13, atol=1e-13)
+
+
+def test_tf_rows_telescope():
+    """The chain sums of publishTFtree's offsets equal the map (x, z, -y) of
+    each joint minus the mean head, and Neck / Head rows as published."""
+    rng = np.random.default_rng(7)
+    p1, p2 = rng.normal(size=(3, 5)), rng.normal(size=(3, 5))
+    tf = oracle.tf_rows(p1, p2)
+    head = 0.5 * (p1[:, 3] + p2[:, 3])
+    neck = 0.5 * (p1[:, 4] + p2[:, 4])
+    T = lambda v: np.array([v[0], v[2], -v[1]])
+    ref = [np.zeros(3), T(p2[:, 0] - head), T(p1[:, 0] - head), T(p2[:, 1] - head), T(p1[:, 1] - head),
+           T(p1[:, 2] - head), T(p2[:, 2] - head), T(neck - head)]
+    assert np.allclose(tf, np.array(ref), rtol=0, atol=1e-14)
+    assert np.array_equal(tf[0], np.zeros(3))
+
+
+def test_pose2d_points():
+    e1, e2 = np.arange(22.0), 100.0 + np.arange(22.0)
+    p = oracle.pose2d(e1, e2)
+    assert np.array_equal(p[0], e1[0:2]) and np.array_equal(p[1], e2[0:2])
+    assert np.array_equal(p[2], 0.5 * (e1[9:11] + e2[9:11])) and np.array_equal(p[3], 0.5 * (e1[12:14] + e2[12:14]))
+    assert np.array_equal(p[4], e1[3:5]) and np.array_equal(p[5], e2[3:5])
+    assert np.array_equal(p[6], e1[6:8]) and np.array_equal(p[7], e2[6:8])
