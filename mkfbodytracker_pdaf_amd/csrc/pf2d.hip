@@ -147,40 +147,37 @@ __device__ __forceinline__ double pf2d_weight_of(const float *lw, const double *
     return uniform ? 1.0 / (double)N : exp((double)lw[j] - *lsum);
 }
 
-// getEstimator (pf2D.cpp:79-87): sum_i w_i x_i, FP64 accumulation, fixed order.
+// getEstimator (pf2D.cpp:79-87): sum_i w_i x_i, FP64 accumulation in a fixed
+// order: block sums (block_sum256), then k_pf2d_estimate_final adds them.
 __global__ void __launch_bounds__(256) k_pf2d_estimate(const float *xs, const float *lw, const double *lsum,
-                                                       int uniform, int N, int ld, double *partial,
-                                                       FrameScratch *fs, double *out)
+                                                       int uniform, int N, int ld, double *partial)
 {
-    __shared__ double s_w[4][kD];
-    __shared__ int s_last;
+    __shared__ double s_v[kD][16][17];
+    __shared__ double s_run[kD][16];
+    __shared__ double s_sum[kD];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const double w = (j < N) ? pf2d_weight_of(lw, lsum, uniform, N, j) : 0.0;
+    double v[kD];
 #pragma unroll
-    for (int c = 0; c < kD; ++c) {
-        const double r = wave_sum((j < N) ? w * (double)xs[(size_t)c * ld + j] : 0.0);
-        if (lane == 0) s_w[wid][c] = r;
+    for (int c = 0; c < kD; ++c) v[c] = (j < N) ? w * (double)xs[(size_t)c * ld + j] : 0.0;
+    block_sum256<kD>(v, s_v, s_run, s_sum);
+    if (threadIdx.x < kD) partial[(size_t)blockIdx.x * kD + threadIdx.x] = s_sum[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(256) k_pf2d_estimate_final(const double *partial, int nb, double *out)
+{
+    __shared__ double s_v[kD][16][17];
+    __shared__ double s_run[kD][16];
+    __shared__ double s_sum[kD];
+    double v[kD];
+#pragma unroll
+    for (int c = 0; c < kD; ++c) v[c] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < kD; ++c) v[c] += partial[(size_t)b * kD + c];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int c = 0; c < kD; ++c) {
-            double r = 0.0;
-            for (int q = 0; q < (int)(blockDim.x >> 6); ++q) r += s_w[q][c];
-            partial[(size_t)blockIdx.x * kD + c] = r;
-        }
-        __threadfence();
-        s_last = (atomicAdd(&fs->done_est, 1u) == gridDim.x - 1);
-        if (s_last) __threadfence();
-    }
-    __syncthreads();
-    if (!s_last) return;
-    if (threadIdx.x < kD) {
-        double r = 0.0;
-        for (int b = 0; b < (int)gridDim.x; ++b) r += ((volatile double *)partial)[(size_t)b * kD + threadIdx.x];
-        out[threadIdx.x] = r;
-    }
-    if (threadIdx.x == 0) fs->done_est = 0u;
+    block_sum256<kD>(v, s_v, s_run, s_sum);
+    if (threadIdx.x < kD) out[threadIdx.x] = s_sum[threadIdx.x];
 }
 
 // Particles (row-major double) and weights out.
@@ -462,8 +459,10 @@ int pf2d_estimate(pf2d_handle *h, double *out)
 {
     if (!h || !out) return set_error(PF_ERR_ARG, "null argument");
     HIP_TRY2(hipSetDevice(h->device));
-    hipLaunchKernelGGL(k_pf2d_estimate, dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->xA, h->lw, h->lsum,
-                       (int)h->uniform, h->N, h->ld, h->partial, h->fs, h->out);
+    const int nb = (h->N + 255) / 256;
+    hipLaunchKernelGGL(k_pf2d_estimate, dim3(nb), dim3(256), 0, h->stream, h->xA, h->lw, h->lsum, (int)h->uniform,
+                       h->N, h->ld, h->partial);
+    hipLaunchKernelGGL(k_pf2d_estimate_final, dim3(1), dim3(256), 0, h->stream, (const double *)h->partial, nb, h->out);
     HIP_TRY2(hipGetLastError());
     HIP_TRY2(hipMemcpyAsync(out, h->out, sizeof(double) * kD, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY2(hipStreamSynchronize(h->stream));

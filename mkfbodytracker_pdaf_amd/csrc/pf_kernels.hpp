@@ -37,7 +37,7 @@ enum AncMode { kAncArray = 0, kAncFused = 1 };
 
 struct FrameScratch {
     unsigned long long maxkey[2];  // per frame parity: ordered key of the largest finite weight (0 = none)
-    unsigned int done_est;         // completion counter of k_estimate blocks
+    unsigned int spare;
     unsigned int status;           // bit 0: a degenerate (all-zero) resampling happened
     uint32_t U_pending;            // uniform of the pending resampling
     uint32_t pad;
