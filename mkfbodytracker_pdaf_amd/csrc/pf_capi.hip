@@ -442,8 +442,8 @@ int estimate(pf_handle *h)
     return check_launch("k_estimate");
 }
 
-int resample_linear(pf_handle *h, const double *w, int L, int N, int parity, const int *frame,
-                    const uint32_t *uni, int ustride, int uoff, int *out)
+int weights_linear(pf_handle *h, const double *w, int L, int parity, const int *frame, const uint32_t *uni,
+                   int ustride, int uoff)
 {
     if (L > h->Lcap) return set_error(PF_ERR_STATE, "resampler scratch holds %d weights, %d asked", h->Lcap, L);
     WeightArgs wa{};
@@ -461,12 +461,27 @@ int resample_linear(pf_handle *h, const double *w, int L, int N, int parity, con
     wa.uoff = uoff;
     const int nb = (L + kWeightsBlock - 1) / kWeightsBlock;
     hipLaunchKernelGGL((k_weights<kLinF64>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, wa);
-    int st = check_launch("k_weights");
-    if (st) return st;
-    return launch_ancestors(h, L, N, out, parity & 1, 1, nullptr);
+    return check_launch("k_weights");
 }
 
-int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, double *lml)
+Pending pending(pf_handle *h, int L) { return pending_of(h, L); }
+
+FrameScratch *scratch(pf_handle *h) { return h->fs; }
+
+template <int D>
+struct AncEstimateL {
+    static void run(pf_handle *h, const AncArgs &an)
+    {
+        const int nb = (h->N + 255) / 256;
+        const size_t lds = (size_t)anc_estimate_lds_bytes(an.pend.nb, D);
+        hipLaunchKernelGGL((k_ancestors_estimate<D>), dim3(nb), dim3(256), lds, h->stream, an, (const float *)h->xB,
+                           h->ld, h->est_partial);
+        hipLaunchKernelGGL((k_estimate_final<D>), dim3(1), dim3(256), 0, h->stream, (const double *)h->est_partial, nb,
+                           h->N, (const double *)h->Tinv, h->est_out);
+    }
+};
+
+int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, double *lml, bool with_estimate)
 {
     const int p = parity & 1;
     StepArgs a = step_args(h, kModeRef, uni, Z, 0, h->N, nullptr, nullptr, nullptr, p, kAncArray, nullptr);
@@ -474,7 +489,21 @@ int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, doub
     if (st) return st;
     st = launch_weights(h, uni, p);
     if (st) return st;
-    st = launch_ancestors(h, h->N, h->N, h->anc, p, 0, lml, h->frame + p);
+    if (with_estimate) {
+        AncArgs an{};
+        an.pend = pending_of(h, h->N);
+        an.fs = h->fs;
+        an.anc = h->anc;
+        an.lml = lml;
+        an.lml_frame = h->frame + p;
+        an.N = h->N;
+        an.parity = p;
+        an.kind_linear = 0;
+        st = dispatch_d<AncEstimateL>(h->D, h, an);      // reads the new state, h->xB before the swap
+        if (!st) st = check_launch("k_ancestors_estimate");
+    } else {
+        st = launch_ancestors(h, h->N, h->N, h->anc, p, 0, lml, h->frame + p);
+    }
     if (st) return st;
     swap_state(h);
     return PF_OK;

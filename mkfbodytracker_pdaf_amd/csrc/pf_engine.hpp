@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #include "../../include/pf_rbpf.h"
+#include "pf_kernels.hpp"
 
 namespace mkfbt {
 namespace eng {
@@ -26,22 +27,28 @@ int reserve_resample(pf_handle *h, int L);
 // Device address of the estimate written by estimate(): D doubles, model basis.
 const double *estimate_ptr(pf_handle *h);
 // Address of the handle's largest-weight slot `parity` (an ordered 64-bit key,
-// raised with atomicMax by the producer of the weights resample_linear reads).
+// raised with atomicMax by the producer of the weights weights_linear reads).
 unsigned long long *maxkey_ptr(pf_handle *h, int parity);
 // Sets both frame counters of the handle to f.
 int set_frame(pf_handle *h, int f);
 // getEstimator (pf2DRao.cpp:23-31) of the current tracks into estimate_ptr().
 int estimate(pf_handle *h);
-// resample(w, N) (pf2DRao.cpp:175-210) of L linear weights already on the
-// device whose maximum key is in maxkey slot `parity`; the uniform is
-// uni[*frame * ustride + uoff].  Writes N indices to out.
-int resample_linear(pf_handle *h, const double *w, int L, int N, int parity, const int *frame,
-                    const uint32_t *uni, int ustride, int uoff, int *out);
 // ParticleFilter::update (pf2DRao.cpp:125-158) with one measurement column per
 // particle: Z (m x N floats, leading dimension N), uniforms uni[2t], uni[2t+1]
 // for t = the handle's frame counter `parity`; the frame's log mean weight goes
-// to lml[t].  Ancestors are materialised (estimate() may follow).
-int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, double *lml);
+// to lml[t].  Ancestors are materialised; with_estimate also computes
+// getEstimator of the resampled tracks into estimate_ptr() (fused with the
+// ancestors), otherwise estimate() may follow.
+int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, double *lml, bool with_estimate);
+// resample(w, N) (pf2DRao.cpp:175-210), first half, on L linear weights already
+// on the device (the maximum key in slot `parity`): fixed-point weights and block scans, the
+// uniform uni[*frame * ustride + uoff] recorded as pending.  The caller
+// resolves the ancestors itself (pending(), scratch(), resolve_ancestor).
+int weights_linear(pf_handle *h, const double *w, int L, int parity, const int *frame, const uint32_t *uni,
+                   int ustride, int uoff);
+// The pending resampling of L weights (block scans) and the frame scratch.
+Pending pending(pf_handle *h, int L);
+FrameScratch *scratch(pf_handle *h);
 // Reads and clears the handle's status word (synchronises the handle's stream).
 int take_status(pf_handle *h, int *degenerate);
 

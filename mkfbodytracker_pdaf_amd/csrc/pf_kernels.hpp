@@ -1044,6 +1044,38 @@ __global__ void __launch_bounds__(256) k_estimate_final(const double *partial, i
     }
 }
 
+// k_ancestors, then the block sums of getEstimator over the resampled state
+// (x[anc_j], summed as k_estimate does; the resolver's LDS is reused for the
+// reduction).  Followed by k_estimate_final.  Dynamic LDS:
+// anc_estimate_lds_bytes(nb, D).
+template <int D>
+__global__ void __launch_bounds__(256) k_ancestors_estimate(AncArgs a, const float *x, int ld, double *partial)
+{
+    extern __shared__ unsigned long long wsp[];
+    const int j0 = blockIdx.x * blockDim.x;
+    const int j = j0 + threadIdx.x;
+    double es = 0.0;
+    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0 && a.lml, es);
+    if (j < a.N) a.anc[j] = src;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        finish_resampling(a.fs, a.parity, es, a.N, a.kind_linear,
+                          (a.lml && a.lml_frame) ? a.lml + *a.lml_frame : a.lml);
+    double v[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) v[i] = (j < a.N) ? (double)x[(size_t)i * ld + src] : 0.0;
+    __syncthreads();                     // every lane is done with the resolver's LDS
+    double *ws = reinterpret_cast<double *>(wsp);
+    block_sum256<D>(v, reinterpret_cast<double (*)[16][17]>(ws), reinterpret_cast<double (*)[16]>(ws + D * 16 * 17),
+                    ws + D * 16 * 17 + D * 16);
+    if (threadIdx.x < D) partial[(size_t)blockIdx.x * D + threadIdx.x] = ws[D * 16 * 17 + D * 16 + threadIdx.x];
+}
+
+__host__ __device__ constexpr int anc_estimate_lds_bytes(int nb, int D)
+{
+    return 8 * resolve_lds_words(nb) > 8 * (D * 16 * 17 + D * 16 + D) ? 8 * resolve_lds_words(nb)
+                                                                        : 8 * (D * 16 * 17 + D * 16 + D);
+}
+
 // Both estimate kernels on one stream.
 template <int D>
 inline void launch_estimate(hipStream_t st, const float *x, const int *anc, int N, int ld, double *partial,

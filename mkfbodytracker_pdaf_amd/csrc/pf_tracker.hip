@@ -8,12 +8,14 @@
 //   k_propose     2 proposals per lane for one arm: Philox words -> position,
 //                 association weight against the blurred image and both arms'
 //                 proposal densities; block maxima for the resampler
-//   (resampler)   the filter library's systematic resampling of the n_c
-//                 weights into N proposal indices (per arm, own stream)
-//   k_build_meas  the 6 x N measurement from the resampled indices; the
-//                 proposal positions are recomputed from their Philox words
-//                 rather than stored (nothing but the weights reaches HBM)
-//   (filter)      update + ancestors + estimate of each arm (own stream)
+//   (weights)     the filter library's fixed-point weights and block scans of
+//                 the n_c proposal weights (per arm, own stream)
+//   k_prop_select the systematic resampling of the proposals into N indices
+//                 and the 6 x N measurement; the chosen proposals are
+//                 recomputed from their Philox words rather than stored
+//                 (nothing but the weights reaches HBM)
+//   (filter)      update, weights, ancestors fused with the estimate's block
+//                 sums, the estimate (own stream)
 //   k_pose        e = pca_proj^T xbar + pca_mean and get3Dpose for both arms
 #include <hip/hip_runtime.h>
 
@@ -26,6 +28,7 @@
 #include "pf_common.hpp"
 #include "pf_engine.hpp"
 #include "pf_internal.hpp"
+#include "pf_kernels.hpp"
 #include "pf_rng.hpp"
 
 using namespace mkfbt;
@@ -46,7 +49,7 @@ constexpr int kBlurW = 32, kBlurH = 16, kBlurThreads = 128;
 constexpr int kProposeBlock = 256;
 constexpr int kMaxProposals = 1 << 20;    // resampler scan limit
 
-// Frame record written by k_propose: what k_build_meas and the inspection
+// Frame record written by k_propose: what k_prop_select and the inspection
 // call need to recompute a proposal.
 struct FrameRec {
     double y[4];     // projected hand mean of arm 0 (u, v), arm 1 (u, v)
@@ -236,15 +239,25 @@ __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev 
     }
 }
 
-// Measurement of arm `arm` (pfPose.cpp:303-320), one lane per particle.
-__global__ void __launch_bounds__(256) k_build_meas(PropParams p, int arm, const int *bins, const FrameRec *rec,
-                                                    float *Z, int N)
+// resample(weights, N) of arm `arm`'s proposals (pfPose.cpp:300-301) and its
+// measurement (pfPose.cpp:303-320), one lane per particle: the systematic
+// resampler's ancestor resolution over the n_c proposal weights (pending on
+// the arm handle after its weights kernel), the chosen proposal recomputed
+// from its Philox words, the 6 rows written as FP32.  Dynamic LDS:
+// 8 resolve_lds_words(nb).
+__global__ void __launch_bounds__(256) k_prop_select(PropParams p, int arm, Pending pend, FrameScratch *fs, int parity,
+                                                     const FrameRec *rec, int *bins, float *Z, int N)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ unsigned long long wsp[];
+    const int j0 = blockIdx.x * blockDim.x, j = j0 + threadIdx.x;
+    double es = 0.0;
+    const int c = resolve_ancestor(pend, N, fs->U_pending, j0, wsp, false, es);
+    if (blockIdx.x == 0 && threadIdx.x == 0) finish_resampling(fs, parity, 0.0, N, 1, nullptr);
     if (j >= N) return;
+    bins[j] = c;
     const int *roi = rec->roi;
     double px, py;
-    proposal(p, arm, rec->g, rec->uniform != 0, roi, rec->y[2 * arm], rec->y[2 * arm + 1], bins[j], px, py);
+    proposal(p, arm, rec->g, rec->uniform != 0, roi, rec->y[2 * arm], rec->y[2 * arm + 1], c, px, py);
     const double fx = (double)roi[0] + (double)roi[2] / 2.0;
     Z[j] = (float)fx;
     Z[(size_t)N + j] = (float)((double)roi[1] + 0.5 * (double)roi[3]);
@@ -519,16 +532,16 @@ int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const i
     const uint32_t *u_prop = tr->uni + 4 * (size_t)tr->uni_cap;
     for (int a = 0; a < 2; ++a) {
         pf_handle *h = tr->arm[a];
-        st = eng::resample_linear(h, tr->w[a], tr->n_c, tr->N, parity, tr->fctr, u_prop, 2, a, tr->bins[a]);
+        st = eng::weights_linear(h, tr->w[a], tr->n_c, parity, tr->fctr, u_prop, 2, a);
         if (st) return st;
-        hipLaunchKernelGGL(k_build_meas, dim3((tr->N + 255) / 256), dim3(256), 0, tr->s[a], p, a,
-                           (const int *)tr->bins[a], (const FrameRec *)tr->rec, tr->Z[a], tr->N);
-        st = launch_check("k_build_meas");
+        const Pending pend = eng::pending(h, tr->n_c);
+        hipLaunchKernelGGL(k_prop_select, dim3((tr->N + 255) / 256), dim3(256), 8 * (size_t)resolve_lds_words(pend.nb),
+                           tr->s[a], p, a, pend, eng::scratch(h), parity, (const FrameRec *)tr->rec, tr->bins[a],
+                           tr->Z[a], tr->N);
+        st = launch_check("k_prop_select");
         if (st) return st;
         st = eng::step_ref(h, tr->Z[a], tr->uni + 2 * (size_t)tr->uni_cap * a, parity,
-                           out.lml ? out.lml + (size_t)a * T : nullptr);
-        if (st) return st;
-        st = eng::estimate(h);
+                           out.lml ? out.lml + (size_t)a * T : nullptr, true);
         if (st) return st;
     }
     TR_TRY(hipEventRecord(tr->ev_join[parity], tr->s[1]));
