@@ -13,8 +13,10 @@
 #include "../../include/mkfbt/my_gmm.hpp"
 #include "../../include/mkfbt/pf2D.hpp"
 #include "../../include/mkfbt/pf2DRao.hpp"
+#include "../../include/mkfbt/pfPose.hpp"
 #include "../../include/pf2d.h"
 #include "../../include/pf_rbpf.h"
+#include "../../include/pf_tracker.h"
 #include "pf_internal.hpp"
 
 namespace mkfbt {
@@ -307,4 +309,74 @@ Mat ParticleFilter::getEstimator()
 }
 
 }  // namespace pf2d
+
+// ------------------------------------------------------------- PFTracker
+namespace {
+
+Mat read_yml(const std::string &path, const char *key)
+{
+    int32_t r = 0, c = 0;
+    ok(pf_yml_read_matrix(path.c_str(), key, &r, &c, nullptr, 0));
+    Mat m(r, c);
+    ok(pf_yml_read_matrix(path.c_str(), key, &r, &c, m.ptr(), (int64_t)m.data.size()));
+    return m;
+}
+
+}  // namespace
+
+PFTracker::PFTracker(const std::string &left_yml, const std::string &right_yml, int numParticles, uint64_t seed,
+                     int device, const pf_tracker_config *cfg)
+{
+    const std::string paths[2] = {left_yml, right_yml};
+    for (int a = 0; a < 2; ++a) {
+        Mat means = read_yml(paths[a], "means"), covs = read_yml(paths[a], "covs");
+        Mat weights = read_yml(paths[a], "weights"), gam = read_yml(paths[a], "gamma");
+        Mat proj = read_yml(paths[a], "pca_proj"), pmean = read_yml(paths[a], "pca_mean");
+        const int K = means.rows, d = means.cols;
+        if (covs.rows != K * d || covs.cols != d || (int)gam.data.size() != K || proj.rows != d)
+            throw PfException(PF_ERR_ARG, "PFTracker: inconsistent model in " + paths[a]);
+        auto pf = std::make_unique<pfrao::ParticleFilter>(numParticles, seed + (uint64_t)a, device);
+        for (int i = 0; i < K; ++i) {                                 // pfPose.cpp:61-65
+            Mat s(d, d, covs.ptr() + (size_t)d * d * i);
+            pf->gmm.loadGaussian(means.row(i), s, proj, pmean, weights.data[i], gam.data[i]);
+        }
+        pf->gmm.resetTracker(pf->resample(pf->gmm.weight, numParticles));   // pfPose.cpp:68-71
+        (a ? h2_pca : h1_pca) = proj;
+        (a ? m2_pca : m1_pca) = pmean;
+        (a ? pf2_ : pf1_) = std::move(pf);
+    }
+    npose_ = h1_pca.cols;
+    if (h2_pca.cols != npose_) throw PfException(PF_ERR_ARG, "PFTracker: the arms' pose sizes differ");
+    pf_tracker_config c;
+    pf_tracker_config_default(&c);
+    if (cfg) c = *cfg;
+    else c.seed = seed;
+    ok(pf_tracker_create(&tr_, &c, pf1_->handle(), pf2_->handle(), npose_, h1_pca.ptr(), m1_pca.ptr(),
+                         h2_pca.ptr(), m2_pca.ptr()));
+}
+
+PFTracker::~PFTracker()
+{
+    if (tr_) pf_tracker_destroy(tr_);
+}
+
+PoseFrame PFTracker::run(const uint8_t *likelihood, const int32_t *roi, bool update)
+{
+    PoseFrame f;
+    std::vector<double> pose((size_t)2 * npose_, 0.0), p3(30, 0.0);
+    f.tf = Mat::zeros(8, 3);
+    f.pose2d = Mat::zeros(8, 2);
+    pf_tracker_outputs o = {f.lml, pose.data(), p3.data(), f.tf.ptr(), f.pose2d.ptr()};
+    f.status = update ? ok(pf_tracker_frame(tr_, likelihood, roi, &o)) : ok(pf_tracker_pose(tr_, &o));
+    f.e1 = Mat(1, npose_, pose.data());
+    f.e2 = Mat(1, npose_, pose.data() + npose_);
+    f.p3D1 = Mat(3, 5, p3.data());
+    f.p3D2 = Mat(3, 5, p3.data() + 15);
+    return f;
+}
+
+PoseFrame PFTracker::callback(const uint8_t *likelihood, const int32_t *roi) { return run(likelihood, roi, true); }
+
+PoseFrame PFTracker::pose() { return run(nullptr, nullptr, false); }
+
 }  // namespace mkfbt

@@ -2,7 +2,8 @@
 // (src/pfPose.cpp:34-71 and 300-348): load the shipped left-arm GMM/PCA model,
 // loadGaussian x K, resample(gmm.weight) + resetTracker, then per-frame update
 // with a 6 x N measurement matrix, getEstimator, getSamples, getSampleProb;
-// plus KF_model and the pf2D filter.  Exits non-zero on any failed check.
+// plus KF_model, the pf2D filter, and the PFTracker core (both arms, the whole
+// per-frame pipeline).  Exits non-zero on any failed check.
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -10,6 +11,7 @@
 
 #include "mkfbt/pf2D.hpp"
 #include "mkfbt/pf2DRao.hpp"
+#include "mkfbt/pfPose.hpp"
 #include "pf_rbpf.h"
 
 using mkfbt::Mat;
@@ -112,6 +114,42 @@ int main(int argc, char **argv)
     bool threw = false;
     try { pf.update(Mat(6, N + 1)); } catch (const mkfbt::PfException &) { threw = true; }
     CHECK(threw);
+
+    // PFTracker core (pfPose.cpp:7-74, 332-377): a likelihood image with blobs
+    // at the two mean hand positions, the face ROI on the mean head
+    {
+        mkfbt::PFTracker trk(dir + "/data13D_PCA_100000_15_12.yml", dir + "/data23D_PCA_100000_15_12.yml", 500, 3);
+        std::vector<uint8_t> img((size_t)480 * 640, 0);
+        const double hx[2] = {388.0, 264.0}, hy[2] = {281.0, 268.0};
+        for (int y = 0; y < 480; ++y)
+            for (int x = 0; x < 640; ++x) {
+                double v = 0.0;
+                for (int a = 0; a < 2; ++a)
+                    v += 230.0 * std::exp(-((x - hx[a]) * (x - hx[a]) + (y - hy[a]) * (y - hy[a])) / 98.0);
+                img[(size_t)y * 640 + x] = (uint8_t)std::lround(v > 255.0 ? 255.0 : v);
+            }
+        const int32_t roi[4] = {304, 50, 40, 48};
+        mkfbt::PoseFrame f;
+        for (int t = 0; t < 4; ++t) {
+            f = trk.callback(img.data(), roi);
+            CHECK(f.status == 0 && std::isfinite(f.lml[0]) && std::isfinite(f.lml[1]));
+        }
+        CHECK(f.e1.cols == 22 && f.p3D1.rows == 3 && f.p3D1.cols == 5 && f.tf.rows == 8);
+        CHECK(f.tf(0, 0) == 0.0 && f.tf(0, 1) == 0.0 && f.tf(0, 2) == 0.0);   // Head relative to itself
+        for (double v : f.e1.data) CHECK(std::isfinite(v));
+        for (double v : f.p3D2.data) CHECK(std::isfinite(v));
+        // the tracked hands stay near the blobs (image coordinates of e1, e2)
+        CHECK(std::fabs(f.pose2d(0, 0) - hx[0]) < 40.0 && std::fabs(f.pose2d(0, 1) - hy[0]) < 40.0);
+        CHECK(std::fabs(f.pose2d(1, 0) - hx[1]) < 40.0 && std::fabs(f.pose2d(1, 1) - hy[1]) < 40.0);
+        mkfbt::PoseFrame g = trk.pose();
+        CHECK(g.e1.data == f.e1.data && g.tf.data == f.tf.data);
+        mkfbt::PoseFrame none = trk.callback(img.data(), nullptr);           // no face this frame
+        for (double v : none.pose2d.data) CHECK(v == 0.0);
+        bool bad = false;
+        const int32_t empty_roi[4] = {304, 50, 0, 48};
+        try { trk.callback(img.data(), empty_roi); } catch (const mkfbt::PfException &) { bad = true; }
+        CHECK(bad);
+    }
 
     std::printf(failures ? "FAILED (%d)\n" : "OK\n", failures);
     return failures ? 1 : 0;
