@@ -47,6 +47,7 @@ constexpr uint32_t kStreamFrame = 24;     // 8 words per frame
 constexpr int kMaxRadius = 15;
 constexpr int kBlurW = 32, kBlurH = 16, kBlurThreads = 128;
 constexpr int kProposeBlock = 256;
+constexpr int kProposeBlocksPerCU = 4;   // per arm: 2 x 4 blocks of 4 waves = 8 waves per SIMD
 constexpr int kMaxProposals = 1 << 20;    // resampler scan limit
 
 // Frame record written by k_propose: what k_prop_select and the inspection
@@ -147,14 +148,17 @@ __device__ inline double project_row(const double *hp, const double *pm, const d
     return s + pm[r];
 }
 
-// Proposal c of arm `arm` in frame g: getSamples (pf2DRao.cpp:85-103) or the
-// uniform first-frame box (pfPose.cpp:222-235).
-__device__ inline void proposal(const PropParams &p, int arm, long long g, bool uniform, const int *roi, double y0,
-                                double y1, long long c, double &px, double &py)
+// Philox block of proposals 2b and 2b + 1 of arm `arm` in frame g.
+__device__ inline void proposal_block(const PropParams &p, int arm, long long g, long long b, uint32_t wd[4])
 {
-    uint32_t wd[4];
-    philox_stream_block(p.seed, kStreamProposal + arm, (uint64_t)g * (uint64_t)(p.n_c / 2) + (uint64_t)(c >> 1), wd);
-    const uint32_t w0 = wd[2 * (c & 1)], w1 = wd[2 * (c & 1) + 1];
+    philox_stream_block(p.seed, kStreamProposal + arm, (uint64_t)g * (uint64_t)(p.n_c / 2) + (uint64_t)b, wd);
+}
+
+// A proposal from its two words: getSamples (pf2DRao.cpp:85-103) or the
+// uniform first-frame box (pfPose.cpp:222-235).
+__device__ inline void proposal_at(const PropParams &p, bool uniform, const int *roi, double y0, double y1,
+                                   uint32_t w0, uint32_t w1, double &px, double &py)
+{
     if (uniform) {
         const long long xo = roi[0], yo = roi[1], wi = roi[2], he = roi[3];
         const long long x_lo = xo - 4 * wi, x_hi = xo + 5 * wi, y_lo = yo + he, y_hi = yo + 7 * he;
@@ -171,6 +175,15 @@ __device__ inline void proposal(const PropParams &p, int arm, long long g, bool 
     }
 }
 
+// Proposal c of arm `arm` in frame g (words 2c, 2c + 1 of the arm's stream).
+__device__ inline void proposal(const PropParams &p, int arm, long long g, bool uniform, const int *roi, double y0,
+                                double y1, long long c, double &px, double &py)
+{
+    uint32_t wd[4];
+    proposal_block(p, arm, g, c >> 1, wd);
+    proposal_at(p, uniform, roi, y0, y1, wd[2 * (c & 1)], wd[2 * (c & 1) + 1], px, py);
+}
+
 // N((px, py); y, var I) (pf2DRao.cpp:69-83,105-122): getSampleProb takes
 // spread x width as the variance, while getSamples draws with it as the
 // standard deviation (pf2DRao.cpp:90 vs :114).
@@ -182,9 +195,39 @@ __device__ inline double density(double px, double py, double y0, double y1, dou
     return inv2piv * det_exp(q * nhiv);
 }
 
+// Hand rows 0, 1 of both arms' projections y = Hp xbar + M, summed as
+// project_row does; the operands (Hp rows 0, 1, xbar, M[0..1] of each arm,
+// 3 D + 2 doubles per arm) are first staged in LDS by one load per thread so
+// that the block pays one memory latency.  Every thread of the block calls it.
+__device__ inline void hand_projections(const ArmDev &a0, const ArmDev &a1, double *stage, double *y)
+{
+#pragma clang fp contract(off)
+    const int n0 = 3 * a0.D + 2, n1 = 3 * a1.D + 2;
+    for (int e = threadIdx.x; e < n0 + n1; e += blockDim.x) {
+        const ArmDev &A = e < n0 ? a0 : a1;
+        const int i = e < n0 ? e : e - n0;
+        const int D = A.D;
+        stage[e] = i < 2 * D ? A.hp[i] : (i < 3 * D ? A.xbar[i - 2 * D] : A.pm[i - 3 * D]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int arm = threadIdx.x >> 1, r = threadIdx.x & 1;
+        const int D = arm ? a1.D : a0.D;
+        const double *st = stage + (arm ? n0 : 0);
+        double v = 0.0;
+        for (int k = 0; k < D; ++k) v = v + st[r * D + k] * st[2 * D + k];
+        y[threadIdx.x] = v + st[3 * D + r];
+    }
+    __syncthreads();
+}
+
+// Proposals of both arms (grid.y = arm), two per lane and iteration (one
+// Philox block); a grid-stride loop over the n_c / 2 pairs, so the frame
+// prologue and the block maximum are paid once per block.
 __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev a0, ArmDev a1, const uint8_t *img,
                                                           FrameRec *rec)
 {
+    __shared__ double s_stage[2 * (3 * 24 + 2)];
     __shared__ double s_y[4];
     __shared__ double s_max[kProposeBlock / 64];
     const int t = *p.frame;
@@ -192,11 +235,7 @@ __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev 
     const bool uniform = (g == p.uniform_frame);
     const int *roi = p.roi + 4 * (size_t)t;
     const int arm = blockIdx.y;
-    if (threadIdx.x < 4) {
-        const ArmDev &A = (threadIdx.x >> 1) ? a1 : a0;
-        s_y[threadIdx.x] = project_row(A.hp, A.pm, A.xbar, A.D, threadIdx.x & 1);
-    }
-    __syncthreads();
+    hand_projections(a0, a1, s_stage, s_y);
     if (blockIdx.x == 0 && arm == 0 && threadIdx.x < 4) {
         rec->y[threadIdx.x] = s_y[threadIdx.x];
         rec->roi[threadIdx.x] = roi[threadIdx.x];
@@ -209,12 +248,17 @@ __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev 
     const double yt0 = s_y[2 * (1 - arm)], yt1 = s_y[2 * (1 - arm) + 1];
     const double var = p.spread * (double)roi[2];
     const double inv2piv = 1.0 / (6.283185307179586 * var), nhiv = -0.5 / var;
-    const long long k = (long long)blockIdx.x * kProposeBlock + threadIdx.x;
-    double wv[2] = {0.0, 0.0};
-    if (2 * k < p.n_c) {
+    double *w = arm ? a1.w : a0.w;
+    const long long pairs = p.n_c / 2, stride = (long long)gridDim.x * kProposeBlock;
+    double m = 0.0;
+    for (long long k = (long long)blockIdx.x * kProposeBlock + threadIdx.x; k < pairs; k += stride) {
+        uint32_t wd[4];
+        proposal_block(p, arm, g, k, wd);     // one Philox block serves both proposals
+        double wv[2] = {0.0, 0.0};
+#pragma unroll
         for (int i = 0; i < 2; ++i) {
             double px, py;
-            proposal(p, arm, g, uniform, roi, yo0, yo1, 2 * k + i, px, py);
+            proposal_at(p, uniform, roi, yo0, yo1, wd[2 * i], wd[2 * i + 1], px, py);
             // pfPose.cpp:251-266 (arm 1) and :274-289 (arm 2)
             if (py > 0.0 && py < (double)p.rows && px > 0.0 && px < (double)p.cols) {
                 const unsigned v = img[(size_t)(int)py * p.cols + (int)px];
@@ -225,16 +269,15 @@ __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev 
                 }
             }
         }
-        double2 *dst = reinterpret_cast<double2 *>((arm ? a1.w : a0.w) + 2 * k);
-        *dst = make_double2(wv[0], wv[1]);
+        *reinterpret_cast<double2 *>(w + 2 * k) = make_double2(wv[0], wv[1]);
+        m = fmax(m, fmax(wv[0], wv[1]));
     }
-    double m = fmax(wv[0], wv[1]);
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double b = s_max[0];
-        for (int w = 1; w < kProposeBlock / 64; ++w) b = fmax(b, s_max[w]);
+        for (int i = 1; i < kProposeBlock / 64; ++i) b = fmax(b, s_max[i]);
         if (b > 0.0 && isfinite(b)) atomicMax(arm ? a1.maxkey : a0.maxkey, order_key(b));
     }
 }
@@ -411,7 +454,7 @@ struct pf_tracker {
     pf_tracker_config cfg{};
     pf_handle *arm[2] = {nullptr, nullptr};
     hipStream_t s[2] = {nullptr, nullptr};
-    int N = 0, D[2] = {0, 0}, npose = 0, n_c = 0, device = 0, radius = 0;
+    int N = 0, D[2] = {0, 0}, npose = 0, n_c = 0, device = 0, radius = 0, cus = 1;
     double *hp[2] = {nullptr, nullptr}, *pm[2] = {nullptr, nullptr};
     float *taps = nullptr;
     uint8_t *img_stage = nullptr, *blur = nullptr;
@@ -523,7 +566,8 @@ int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const i
     const PropParams p = prop_params(tr, roi, tr->fctr);
     const ArmDev a0 = arm_dev(tr, 0, parity), a1 = arm_dev(tr, 1, parity);
     const int pairs = tr->n_c / 2;
-    hipLaunchKernelGGL(k_propose, dim3((pairs + kProposeBlock - 1) / kProposeBlock, 2), dim3(kProposeBlock), 0,
+    const int pblocks = std::min((pairs + kProposeBlock - 1) / kProposeBlock, kProposeBlocksPerCU * tr->cus);
+    hipLaunchKernelGGL(k_propose, dim3(pblocks, 2), dim3(kProposeBlock), 0,
                        tr->s[0], p, a0, a1, (const uint8_t *)tr->blur, tr->rec);
     st = launch_check("k_propose");
     if (st) return st;
@@ -731,6 +775,10 @@ int pf_tracker_create(pf_tracker **out, const pf_tracker_config *cfg, pf_handle 
     };
     int st = tr_check(tr);
     if (st) return fail(st);
+    st = hip_ok(hipDeviceGetAttribute(&tr->cus, hipDeviceAttributeMultiprocessorCount, tr->device),
+                "hipDeviceGetAttribute");
+    if (st) return fail(st);
+    tr->cus = std::max(tr->cus, 1);
     // Hp = pca_proj^T (npose x D), pca_mean as given
     for (int a = 0; a < 2; ++a) {
         const double *P = a ? pca_proj2 : pca_proj1;
