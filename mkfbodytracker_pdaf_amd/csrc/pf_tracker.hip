@@ -47,7 +47,6 @@ constexpr uint32_t kStreamFrame = 24;     // 8 words per frame
 constexpr int kMaxRadius = 15;
 constexpr int kBlurW = 32, kBlurH = 16, kBlurThreads = 128;
 constexpr int kProposeBlock = 256;
-constexpr int kProposeBlocksPerCU = 4;   // per arm: 2 x 4 blocks of 4 waves = 8 waves per SIMD
 constexpr int kMaxProposals = 1 << 20;    // resampler scan limit
 
 // Frame record written by k_propose: what k_prop_select and the inspection
@@ -293,14 +292,19 @@ __global__ void __launch_bounds__(256) k_prop_select(PropParams p, int arm, Pend
 {
     extern __shared__ unsigned long long wsp[];
     const int j0 = blockIdx.x * blockDim.x, j = j0 + threadIdx.x;
+    // the frame record is loaded before the resolution so that its latency
+    // overlaps the block-sum loads
+    const int roi[4] = {rec->roi[0], rec->roi[1], rec->roi[2], rec->roi[3]};
+    const long long g = rec->g;
+    const bool uniform = rec->uniform != 0;
+    const double y0 = rec->y[2 * arm], y1 = rec->y[2 * arm + 1];
     double es = 0.0;
     const int c = resolve_ancestor(pend, N, fs->U_pending, j0, wsp, false, es);
     if (blockIdx.x == 0 && threadIdx.x == 0) finish_resampling(fs, parity, 0.0, N, 1, nullptr);
     if (j >= N) return;
     bins[j] = c;
-    const int *roi = rec->roi;
     double px, py;
-    proposal(p, arm, rec->g, rec->uniform != 0, roi, rec->y[2 * arm], rec->y[2 * arm + 1], c, px, py);
+    proposal(p, arm, g, uniform, roi, y0, y1, c, px, py);
     const double fx = (double)roi[0] + (double)roi[2] / 2.0;
     Z[j] = (float)fx;
     Z[(size_t)N + j] = (float)((double)roi[1] + 0.5 * (double)roi[3]);
@@ -454,7 +458,8 @@ struct pf_tracker {
     pf_tracker_config cfg{};
     pf_handle *arm[2] = {nullptr, nullptr};
     hipStream_t s[2] = {nullptr, nullptr};
-    int N = 0, D[2] = {0, 0}, npose = 0, n_c = 0, device = 0, radius = 0, cus = 1;
+    int N = 0, D[2] = {0, 0}, npose = 0, n_c = 0, device = 0, radius = 0;
+    int prop_blocks = 1;              // k_propose blocks per arm: one resident round over the device
     double *hp[2] = {nullptr, nullptr}, *pm[2] = {nullptr, nullptr};
     float *taps = nullptr;
     uint8_t *img_stage = nullptr, *blur = nullptr;
@@ -566,7 +571,7 @@ int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const i
     const PropParams p = prop_params(tr, roi, tr->fctr);
     const ArmDev a0 = arm_dev(tr, 0, parity), a1 = arm_dev(tr, 1, parity);
     const int pairs = tr->n_c / 2;
-    const int pblocks = std::min((pairs + kProposeBlock - 1) / kProposeBlock, kProposeBlocksPerCU * tr->cus);
+    const int pblocks = std::min((pairs + kProposeBlock - 1) / kProposeBlock, tr->prop_blocks);
     hipLaunchKernelGGL(k_propose, dim3(pblocks, 2), dim3(kProposeBlock), 0,
                        tr->s[0], p, a0, a1, (const uint8_t *)tr->blur, tr->rec);
     st = launch_check("k_propose");
@@ -775,10 +780,18 @@ int pf_tracker_create(pf_tracker **out, const pf_tracker_config *cfg, pf_handle 
     };
     int st = tr_check(tr);
     if (st) return fail(st);
-    st = hip_ok(hipDeviceGetAttribute(&tr->cus, hipDeviceAttributeMultiprocessorCount, tr->device),
-                "hipDeviceGetAttribute");
-    if (st) return fail(st);
-    tr->cus = std::max(tr->cus, 1);
+    {
+        // k_propose's grid-stride loop: as many blocks as are resident at once
+        // (both arms together), so no block waits for a slot
+        int cus = 0, per_cu = 0;
+        st = hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, tr->device),
+                    "hipDeviceGetAttribute");
+        if (!st)
+            st = hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_propose, kProposeBlock, 0),
+                        "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        if (st) return fail(st);
+        tr->prop_blocks = std::max(1, (cus * per_cu) / 2);
+    }
     // Hp = pca_proj^T (npose x D), pca_mean as given
     for (int a = 0; a < 2; ++a) {
         const double *P = a ? pca_proj2 : pca_proj1;
