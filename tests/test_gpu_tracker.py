@@ -92,15 +92,15 @@ def frames():
     return synthetic.tracker_sequence(4, seed=5)
 
 
-def test_frames_match_oracle(gpu, frames):
-    N = 1000
-    imgs, rois = frames
-    tr, arms = _tracker(N)
-    twins = [_arm(p, N, a)[0] for a, p in enumerate((LEFT, RIGHT))]
-    (P1, M1), (P2, M2) = _pca(LEFT), _pca(RIGHT)
-    arms_o = [_oracle_model(LEFT)[0], _oracle_model(RIGHT)[0]]
-    orc = oracle.PoseTrackerOracle(arms_o, [P1.T, P2.T], [M1, M2], N, seed=SEED)
-    for g in range(3):
+def _against_oracle(tr, arms, twins, arms_o, pcas, N, imgs, rois, frames=3, twin_exact=True):
+    """Runs frames through tr and checks every stage against the oracle (the
+    module docstring's list); twins are handles with the same models, started
+    from the arms' tracks through get_tracks / set_tracks.  That round trip
+    goes through the FP64 basis change of the selection basis and back, which
+    can move an FP32 covariance entry by one unit in the last place for some
+    models; twin_exact=False then compares the twin within 1e-6."""
+    orc = oracle.PoseTrackerOracle(arms_o, [P.T for P, _ in pcas], [M for _, M in pcas], N, seed=SEED)
+    for g in range(frames):
         img, roi = imgs[g], rois[g]
         xbars = [a.estimate() for a in arms]
         before = [a.get_tracks() for a in arms]
@@ -124,8 +124,12 @@ def test_frames_match_oracle(gpu, frames):
             st, o = twins[a].step(last["Z"][a].astype(np.float64), int(words[2 * a]), int(words[2 * a + 1]))
             xa, Pa = arms[a].get_tracks()
             xt, Pt = twins[a].get_tracks()
-            assert np.array_equal(xa, xt) and np.array_equal(Pa, Pt), (g, a)
-            assert out["lml"][a] == o["lml"], (g, a)
+            if twin_exact:
+                assert np.array_equal(xa, xt) and np.array_equal(Pa, Pt), (g, a)
+                assert out["lml"][a] == o["lml"], (g, a)
+            else:
+                assert _rel(xt, xa).max() <= 1e-6 and _rel(Pt, Pa).max() <= 1e-6, (g, a)
+                assert abs(out["lml"][a] - o["lml"]) <= 1e-9 * max(1.0, abs(o["lml"])), (g, a)
             # ... whose step the oracle pins
             r = arms_o[a].update_ref(before[a][0], before[a][1], last["Z"][a].astype(np.float64),
                                      int(words[2 * a]), int(words[2 * a + 1]))
@@ -150,9 +154,71 @@ def test_frames_match_oracle(gpu, frames):
     po = tr.pose()
     for k in ("pose", "pos3d", "tf", "pose2d"):
         assert np.array_equal(po[k], out[k]), k
+
+
+def test_frames_match_oracle(gpu, frames):
+    N = 1000
+    imgs, rois = frames
+    tr, arms = _tracker(N)
+    twins = [_arm(p, N, a)[0] for a, p in enumerate((LEFT, RIGHT))]
+    arms_o = [_oracle_model(LEFT)[0], _oracle_model(RIGHT)[0]]
+    _against_oracle(tr, arms, twins, arms_o, [_pca(LEFT), _pca(RIGHT)], N, imgs, rois)
     tr.close()
     for a in arms + twins:
         a.close()
+
+
+def _resized_arm(path, D, seed):
+    """The shipped arm model with state dimension D: its first D coordinates
+    (a marginal of the mixture) for D < 12, or padded with D - 12 coordinates
+    of small independent noise and small loadings for D > 12."""
+    means = capi.yml_read_matrix(path, "means")
+    K, d = means.shape
+    covs = capi.yml_read_matrix(path, "covs").reshape(K, d, d)
+    w = capi.yml_read_matrix(path, "weights").ravel()
+    g = capi.yml_read_matrix(path, "gamma").ravel()
+    P, M = _pca(path)
+    if D <= d:
+        return means[:, :D].copy(), covs[:, :D, :D].copy(), w, g, P[:D].copy(), M
+    rng = np.random.default_rng(seed)
+    e = D - d
+    mu = np.concatenate([means, 0.1 * rng.standard_normal((K, e))], axis=1)
+    S = np.zeros((K, D, D))
+    S[:, :d, :d] = covs
+    S[:, d:, d:] = 0.05 * np.eye(e)
+    Pd = np.concatenate([P, 0.02 * rng.standard_normal((e, P.shape[1]))], axis=0)
+    return mu, S, w, g, Pd, M
+
+
+def test_arms_of_other_state_dimensions(gpu, frames):
+    """Arms with d = 8 and d = 16 (and 24 and 12): the hand projections of
+    both arms are staged together in k_propose, the filter step, ancestors
+    and estimate kernels run at each arm's own d."""
+    N = 600
+    imgs, rois = frames
+    sel = np.array([9, 10, 0, 1, 12, 13], np.int32)
+    for dims in ((8, 16), (24, 12)):
+        models = [_resized_arm(p, D, 40 + D) for p, D in zip((LEFT, RIGHT), dims)]
+
+        def handle(a):
+            mu, S, w, g, P, M = models[a]
+            rb = Rbpf(N)
+            rb.load_model(mu, S.reshape(-1, mu.shape[1]), w, g, P, M)
+            bins, _ = rb.resample(w, N, int(capi.philox_u32(SEED, 32 + a, 0, 1)[0]))
+            rb.reset(bins)
+            return rb
+
+        arms, twins = [handle(0), handle(1)], [handle(0), handle(1)]
+        arms_o = []
+        for mu, S, w, g, P, M in models:
+            kf = oracle.load_gaussian(mu[0], S[0], P, M, g[0], sel, 100.0 * np.eye(6))
+            arms_o.append(oracle.Model(w, g, mu, S, kf["H"], kf["BH"], kf["R"]))
+        pcas = [(m[4], m[5]) for m in models]
+        tr = PoseTracker(arms[0], arms[1], pcas[0][0], pcas[0][1], pcas[1][0], pcas[1][1], seed=SEED)
+        _against_oracle(tr, arms, twins, arms_o, pcas, N, imgs, rois, frames=2, twin_exact=False)
+        tr.close()
+        for a in arms + twins:
+            a.close()
 
 
 @pytest.mark.parametrize("graph", [True, False])
