@@ -426,7 +426,6 @@ int reserve_resample(pf_handle *h, int L) { return ensure_resample_scratch(h, L)
 
 const double *estimate_ptr(pf_handle *h) { return h->est_out; }
 
-unsigned long long *maxkey_ptr(pf_handle *h, int parity) { return &h->fs->maxkey[parity & 1]; }
 
 int set_frame(pf_handle *h, int f)
 {
@@ -440,28 +439,6 @@ int estimate(pf_handle *h)
     int st = dispatch_d<EstimateL>(h->D, h);
     if (st) return st;
     return check_launch("k_estimate");
-}
-
-int weights_linear(pf_handle *h, const double *w, int L, int parity, const int *frame, const uint32_t *uni,
-                   int ustride, int uoff)
-{
-    if (L > h->Lcap) return set_error(PF_ERR_STATE, "resampler scratch holds %d weights, %d asked", h->Lcap, L);
-    WeightArgs wa{};
-    wa.w64 = w;
-    wa.cloc = h->cloc;
-    wa.bsum = h->bsum;
-    wa.bexp = h->bexp;
-    wa.fs = h->fs;
-    wa.frame = frame;
-    wa.frame_next = nullptr;
-    wa.uni = uni;
-    wa.L = L;
-    wa.parity = parity & 1;
-    wa.ustride = ustride;
-    wa.uoff = uoff;
-    const int nb = (L + kWeightsBlock - 1) / kWeightsBlock;
-    hipLaunchKernelGGL((k_weights<kLinF64>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, wa);
-    return check_launch("k_weights");
 }
 
 Pending pending(pf_handle *h, int L) { return pending_of(h, L); }

@@ -26,9 +26,6 @@ int info(pf_handle *h, Info *out);
 int reserve_resample(pf_handle *h, int L);
 // Device address of the estimate written by estimate(): D doubles, model basis.
 const double *estimate_ptr(pf_handle *h);
-// Address of the handle's largest-weight slot `parity` (an ordered 64-bit key,
-// raised with atomicMax by the producer of the weights weights_linear reads).
-unsigned long long *maxkey_ptr(pf_handle *h, int parity);
 // Sets both frame counters of the handle to f.
 int set_frame(pf_handle *h, int f);
 // getEstimator (pf2DRao.cpp:23-31) of the current tracks into estimate_ptr().
@@ -40,13 +37,10 @@ int estimate(pf_handle *h);
 // getEstimator of the resampled tracks into estimate_ptr() (fused with the
 // ancestors), otherwise estimate() may follow.
 int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, double *lml, bool with_estimate);
-// resample(w, N) (pf2DRao.cpp:175-210), first half, on L linear weights already
-// on the device (the maximum key in slot `parity`): fixed-point weights and block scans, the
-// uniform uni[*frame * ustride + uoff] recorded as pending.  The caller
-// resolves the ancestors itself (pending(), scratch(), resolve_ancestor).
-int weights_linear(pf_handle *h, const double *w, int L, int parity, const int *frame, const uint32_t *uni,
-                   int ustride, int uoff);
-// The pending resampling of L weights (block scans) and the frame scratch.
+// The resampler scratch of the handle viewed as a pending resampling of L
+// weights (block-local scans cloc, block sums bsum, kWeightsBlock inputs per
+// block; a caller's own kernel fills them and resolves the ancestors with
+// resolve_prefix / resolve_search), and the frame scratch (U_pending, status).
 Pending pending(pf_handle *h, int L);
 FrameScratch *scratch(pf_handle *h);
 // Reads and clears the handle's status word (synchronises the handle's stream).

@@ -187,6 +187,12 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
     // 2. the input block of each output: largest b with pre[b] * S <= t_j
     const int j = j0 + tid;
     const int jc = min(j, N - 1);
+    if (total == 0ull) {
+        // every weight zero: the uniform resampling of q_i = 1 (C_i = i + 1,
+        // total L), whose a_j = floor((j 2^32 + U) L / (N 2^32)) in closed form
+        const u128 x = (u128)((((unsigned long long)(uint32_t)jc) << 32) | U) * (u128)(unsigned)pd.L;
+        return (int)((unsigned long long)(x >> 32) / (unsigned long long)N);
+    }
     const u128 S = (u128)(unsigned long long)N << 32;
     const u128 tj = sys_target((uint32_t)jc, U, total);
     int lo = 0, hi = nb - 1;

@@ -5,11 +5,12 @@
 //
 // Kernels, per frame:
 //   k_blur        separable Gaussian of the likelihood image, LDS-tiled
-//   k_propose     2 proposals per lane for one arm: Philox words -> position,
-//                 association weight against the blurred image and both arms'
-//                 proposal densities; block maxima for the resampler
-//   (weights)     the filter library's fixed-point weights and block scans of
-//                 the n_c proposal weights (per arm, own stream)
+//   k_propose     2 proposals per lane for both arms: Philox words ->
+//                 position, association weight against the blurred image and
+//                 both arms' proposal densities, and the resampler's
+//                 fixed-point weights and block scans (the weights' bound,
+//                 not their maximum, sets the fixed-point scale, so no second
+//                 pass over the weights is needed)
 //   k_prop_select the systematic resampling of the proposals into N indices
 //                 and the 6 x N measurement; the chosen proposals are
 //                 recomputed from their Philox words rather than stored
@@ -47,6 +48,7 @@ constexpr uint32_t kStreamFrame = 24;     // 8 words per frame
 constexpr int kMaxRadius = 15;
 constexpr int kBlurW = 32, kBlurH = 16, kBlurThreads = 128;
 constexpr int kProposeBlock = 256;
+constexpr int kMaxArmD = 24;              // k_propose stages 3 D + 2 projection operands per arm
 constexpr int kMaxProposals = 1 << 20;    // resampler scan limit
 
 // Frame record written by k_propose: what k_prop_select and the inspection
@@ -63,8 +65,10 @@ struct ArmDev {
     const double *xbar;       // estimate, model basis (D)
     const double *hp;         // npose x D
     const double *pm;         // npose
-    double *w;                // n_c association weights
-    unsigned long long *maxkey;
+    double *w;                // n_c association weights (kept for inspection)
+    unsigned long long *cloc; // n_c block-local inclusive scans of the fixed-point weights
+    unsigned long long *bsum; // their block sums (kWeightsBlock inputs per block)
+    FrameScratch *fs;         // the arm handle's resampler scratch (pending uniform, status)
     int D;
 };
 
@@ -134,6 +138,7 @@ struct PropParams {
     long long uniform_frame;  // global index of the uniform-proposal frame (-1: none)
     const int *roi;           // frames x 4
     const int *frame;         // t
+    const uint32_t *u_prop;   // frames x 2: the proposal resampling uniform of each arm
     int n_c, rows, cols;
     double spread, pa, zfloor;
 };
@@ -220,21 +225,29 @@ __device__ inline void hand_projections(const ArmDev &a0, const ArmDev &a1, doub
     __syncthreads();
 }
 
-// Proposals of both arms (grid.y = arm), two per lane and iteration (one
-// Philox block); a grid-stride loop over the n_c / 2 pairs, so the frame
-// prologue and the block maximum are paid once per block.
+// Proposals of both arms (grid.y = arm), two per lane and chunk of 256
+// pairs; a block walks the chunks c = blockIdx.x, + gridDim.x, ... (one
+// resident round of blocks), so the frame prologue is paid once per block.
+// Chunk c holds proposals 512 c .. 512 c + 511, i.e. resampler input blocks
+// 2 c (lanes 0-127) and 2 c + 1 (lanes 128-255), whose block-local inclusive
+// scans and sums of q = fixed32(w z) are written here: with z = z_floor
+// (1 - 2 Pa) > 0 every weight is at most 1 / z, so w z is in [0, 1] and the
+// maximum is not needed (resample(weights, N), pfPose.cpp:300-301).
 __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev a0, ArmDev a1, const uint8_t *img,
                                                           FrameRec *rec)
 {
-    __shared__ double s_stage[2 * (3 * 24 + 2)];
+    static_assert(kProposeBlock == 256 && kWeightsBlock == 256, "a chunk of 256 pairs is two resampler blocks");
+    __shared__ double s_stage[2 * (3 * kMaxArmD + 2)];
     __shared__ double s_y[4];
-    __shared__ double s_max[kProposeBlock / 64];
+    __shared__ unsigned long long s_wave[kProposeBlock / 64];
     const int t = *p.frame;
     const long long g = p.g0 + t;
     const bool uniform = (g == p.uniform_frame);
     const int *roi = p.roi + 4 * (size_t)t;
     const int arm = blockIdx.y;
+    const ArmDev &A = arm ? a1 : a0;
     hand_projections(a0, a1, s_stage, s_y);
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.fs->U_pending = p.u_prop[2 * (size_t)t + arm];
     if (blockIdx.x == 0 && arm == 0 && threadIdx.x < 4) {
         rec->y[threadIdx.x] = s_y[threadIdx.x];
         rec->roi[threadIdx.x] = roi[threadIdx.x];
@@ -247,47 +260,62 @@ __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev 
     const double yt0 = s_y[2 * (1 - arm)], yt1 = s_y[2 * (1 - arm) + 1];
     const double var = p.spread * (double)roi[2];
     const double inv2piv = 1.0 / (6.283185307179586 * var), nhiv = -0.5 / var;
-    double *w = arm ? a1.w : a0.w;
-    const long long pairs = p.n_c / 2, stride = (long long)gridDim.x * kProposeBlock;
-    double m = 0.0;
-    for (long long k = (long long)blockIdx.x * kProposeBlock + threadIdx.x; k < pairs; k += stride) {
-        uint32_t wd[4];
-        proposal_block(p, arm, g, k, wd);     // one Philox block serves both proposals
+    const long long pairs = p.n_c / 2;
+    const int nb = (p.n_c + kWeightsBlock - 1) / kWeightsBlock;
+    const int nchunks = (int)((pairs + kProposeBlock - 1) / kProposeBlock);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const long long k = (long long)c * kProposeBlock + threadIdx.x;
         double wv[2] = {0.0, 0.0};
+        unsigned long long q[2] = {0ull, 0ull};
+        if (k < pairs) {
+            uint32_t wd[4];
+            proposal_block(p, arm, g, k, wd);     // one Philox block serves both proposals
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            double px, py;
-            proposal_at(p, uniform, roi, yo0, yo1, wd[2 * i], wd[2 * i + 1], px, py);
-            // pfPose.cpp:251-266 (arm 1) and :274-289 (arm 2)
-            if (py > 0.0 && py < (double)p.rows && px > 0.0 && px < (double)p.cols) {
-                const unsigned v = img[(size_t)(int)py * p.cols + (int)px];
-                if (v != 0u) {     // (v / 255) / Z with one division
-                    const double po = density(px, py, yo0, yo1, nhiv, inv2piv);
-                    const double pt = density(px, py, yt0, yt1, nhiv, inv2piv);
-                    wv[i] = (double)v / (255.0 * fma(p.pa, po, fma(p.pa, pt, p.zfloor)));
+            for (int i = 0; i < 2; ++i) {
+                double px, py;
+                proposal_at(p, uniform, roi, yo0, yo1, wd[2 * i], wd[2 * i + 1], px, py);
+                // pfPose.cpp:251-266 (arm 1) and :274-289 (arm 2)
+                if (py > 0.0 && py < (double)p.rows && px > 0.0 && px < (double)p.cols) {
+                    const unsigned v = img[(size_t)(int)py * p.cols + (int)px];
+                    if (v != 0u) {     // (v / 255) / Z with one division
+                        const double po = density(px, py, yo0, yo1, nhiv, inv2piv);
+                        const double pt = density(px, py, yt0, yt1, nhiv, inv2piv);
+                        wv[i] = (double)v / (255.0 * fma(p.pa, po, fma(p.pa, pt, p.zfloor)));
+                    }
                 }
+                q[i] = (isfinite(wv[i]) && wv[i] > 0.0) ? fixed32(wv[i] * p.zfloor) : 0ull;
             }
+            *reinterpret_cast<double2 *>(A.w + 2 * k) = make_double2(wv[0], wv[1]);
         }
-        *reinterpret_cast<double2 *>(w + 2 * k) = make_double2(wv[0], wv[1]);
-        m = fmax(m, fmax(wv[0], wv[1]));
-    }
-    m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double b = s_max[0];
-        for (int i = 1; i < kProposeBlock / 64; ++i) b = fmax(b, s_max[i]);
-        if (b > 0.0 && isfinite(b)) atomicMax(arm ? a1.maxkey : a0.maxkey, order_key(b));
+        // inclusive scan of the pair sums within each 128-lane half
+        unsigned long long sc = q[0] + q[1];
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned long long n = __shfl_up(sc, o, 64);
+            if (lane >= o) sc += n;
+        }
+        if (lane == 63) s_wave[wid] = sc;
+        __syncthreads();
+        if (wid & 1) sc += s_wave[wid - 1];
+        if (k < pairs) {
+            A.cloc[2 * k] = sc - q[1];
+            A.cloc[2 * k + 1] = sc;
+        }
+        const int rb = 2 * c + (wid >> 1);
+        if ((threadIdx.x & 127) == 127 && rb < nb) A.bsum[rb] = sc;
+        __syncthreads();                     // s_wave is reused by the next chunk
     }
 }
 
 // resample(weights, N) of arm `arm`'s proposals (pfPose.cpp:300-301) and its
 // measurement (pfPose.cpp:303-320), one lane per particle: the systematic
-// resampler's ancestor resolution over the n_c proposal weights (pending on
-// the arm handle after its weights kernel), the chosen proposal recomputed
-// from its Philox words, the 6 rows written as FP32.  Dynamic LDS:
-// 8 resolve_lds_words(nb).
-__global__ void __launch_bounds__(256) k_prop_select(PropParams p, int arm, Pending pend, FrameScratch *fs, int parity,
+// resampler's ancestor resolution over the n_c proposal weights (scanned by
+// k_propose), the chosen proposal recomputed from its Philox words, the 6
+// rows written as FP32.  All weights zero (a blank image): uniform indices
+// and the degenerate status bit, as the filter's resampler does.  Dynamic
+// LDS: 8 resolve_lds_words(nb).
+__global__ void __launch_bounds__(256) k_prop_select(PropParams p, int arm, Pending pend, FrameScratch *fs,
                                                      const FrameRec *rec, int *bins, float *Z, int N)
 {
     extern __shared__ unsigned long long wsp[];
@@ -299,8 +327,9 @@ __global__ void __launch_bounds__(256) k_prop_select(PropParams p, int arm, Pend
     const bool uniform = rec->uniform != 0;
     const double y0 = rec->y[2 * arm], y1 = rec->y[2 * arm + 1];
     double es = 0.0;
-    const int c = resolve_ancestor(pend, N, fs->U_pending, j0, wsp, false, es);
-    if (blockIdx.x == 0 && threadIdx.x == 0) finish_resampling(fs, parity, 0.0, N, 1, nullptr);
+    const unsigned long long total = resolve_prefix(pend, wsp, false, es);
+    const int c = resolve_search(pend, N, fs->U_pending, total, j0, wsp);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && total == 0ull) fs->status |= 1u;
     if (j >= N) return;
     bins[j] = c;
     double px, py;
@@ -488,14 +517,17 @@ int tr_check(pf_tracker *tr)
     return PF_OK;
 }
 
-ArmDev arm_dev(pf_tracker *tr, int a, int parity)
+ArmDev arm_dev(pf_tracker *tr, int a)
 {
     ArmDev d{};
     d.xbar = eng::estimate_ptr(tr->arm[a]);
     d.hp = tr->hp[a];
     d.pm = tr->pm[a];
     d.w = tr->w[a];
-    d.maxkey = eng::maxkey_ptr(tr->arm[a], parity);
+    const Pending pend = eng::pending(tr->arm[a], tr->n_c);
+    d.cloc = const_cast<unsigned long long *>(pend.cloc);
+    d.bsum = const_cast<unsigned long long *>(pend.bsum);
+    d.fs = eng::scratch(tr->arm[a]);
     d.D = tr->D[a];
     return d;
 }
@@ -508,6 +540,7 @@ PropParams prop_params(pf_tracker *tr, const int *roi, const int *frame)
     p.uniform_frame = tr->uniform_frame;
     p.roi = roi;
     p.frame = frame;
+    p.u_prop = tr->uni + 4 * (size_t)tr->uni_cap;
     p.n_c = tr->n_c;
     p.rows = tr->cfg.rows;
     p.cols = tr->cfg.cols;
@@ -546,8 +579,8 @@ int launch_blur(pf_tracker *tr, const uint8_t *src, const int *frame, uint8_t *d
 int launch_pose(pf_tracker *tr, int *frame, const pf_tracker_outputs &out)
 {
     PoseArgs pa{};
-    pa.arm[0] = arm_dev(tr, 0, 0);
-    pa.arm[1] = arm_dev(tr, 1, 0);
+    pa.arm[0] = arm_dev(tr, 0);
+    pa.arm[1] = arm_dev(tr, 1);
     pa.frame = frame;
     pa.pose = out.pose;
     pa.pos3d = out.pos3d;
@@ -569,7 +602,7 @@ int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const i
     int st = launch_blur(tr, lik, tr->fctr, tr->blur);
     if (st) return st;
     const PropParams p = prop_params(tr, roi, tr->fctr);
-    const ArmDev a0 = arm_dev(tr, 0, parity), a1 = arm_dev(tr, 1, parity);
+    const ArmDev a0 = arm_dev(tr, 0), a1 = arm_dev(tr, 1);
     const int pairs = tr->n_c / 2;
     const int pblocks = std::min((pairs + kProposeBlock - 1) / kProposeBlock, tr->prop_blocks);
     hipLaunchKernelGGL(k_propose, dim3(pblocks, 2), dim3(kProposeBlock), 0,
@@ -578,15 +611,12 @@ int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const i
     if (st) return st;
     TR_TRY(hipEventRecord(tr->ev_fork[parity], tr->s[0]));
     TR_TRY(hipStreamWaitEvent(tr->s[1], tr->ev_fork[parity], 0));
-    const uint32_t *u_prop = tr->uni + 4 * (size_t)tr->uni_cap;
     for (int a = 0; a < 2; ++a) {
         pf_handle *h = tr->arm[a];
-        st = eng::weights_linear(h, tr->w[a], tr->n_c, parity, tr->fctr, u_prop, 2, a);
-        if (st) return st;
         const Pending pend = eng::pending(h, tr->n_c);
         hipLaunchKernelGGL(k_prop_select, dim3((tr->N + 255) / 256), dim3(256), 8 * (size_t)resolve_lds_words(pend.nb),
-                           tr->s[a], p, a, pend, eng::scratch(h), parity, (const FrameRec *)tr->rec, tr->bins[a],
-                           tr->Z[a], tr->N);
+                           tr->s[a], p, a, pend, eng::scratch(h), (const FrameRec *)tr->rec, tr->bins[a], tr->Z[a],
+                           tr->N);
         st = launch_check("k_prop_select");
         if (st) return st;
         st = eng::step_ref(h, tr->Z[a], tr->uni + 2 * (size_t)tr->uni_cap * a, parity,
@@ -740,6 +770,7 @@ int pf_tracker_create(pf_tracker **out, const pf_tracker_config *cfg, pf_handle 
         if (st) return st;
         if (!in[a].model) return set_error(PF_ERR_STATE, "arm %d has no model", a + 1);
         if (in[a].m != 6) return set_error(PF_ERR_ARG, "arm %d measures %d rows; the tracker's measurement has 6", a + 1, in[a].m);
+        if (in[a].D > kMaxArmD) return set_error(PF_ERR_UNSUPPORTED, "arm %d state dimension %d > %d", a + 1, in[a].D, kMaxArmD);
     }
     if (in[0].N != in[1].N) return set_error(PF_ERR_ARG, "arms with different particle counts (%d, %d)", in[0].N, in[1].N);
     if (in[0].device != in[1].device) return set_error(PF_ERR_ARG, "arms on different devices");

@@ -64,6 +64,7 @@ def lib():
         L.orc_det_exp.restype = C.c_double
         L.orc_resample_linear.argtypes = [_dp, C.c_int, C.c_int, C.c_uint32, _ip]
         L.orc_resample_log.argtypes = [_dp, C.c_int, C.c_int, C.c_uint32, _ip]
+        L.orc_resample_scaled.argtypes = [_dp, C.c_int, C.c_int, C.c_uint32, C.c_double, _ip]
         L.orc_resample_wheel.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_void_p, _ip]
         L.orc_kf_predict.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_kf_update.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -137,6 +138,15 @@ def resample_linear(w, N: int, U: int):
     w = _f64(w)
     out = np.zeros(N, np.int32)
     st = lib().orc_resample_linear(w, w.size, N, int(U), out)
+    return out, st
+
+
+def resample_scaled(w, N: int, U: int, scale: float):
+    """Systematic resampling of linear weights bounded by 1/scale:
+    q_i = floor(2^32 w_i scale) instead of the maximum-relative q."""
+    w = _f64(w)
+    out = np.zeros(N, np.int32)
+    st = lib().orc_resample_scaled(w, w.size, N, int(U), float(scale), out)
     return out, st
 
 
@@ -487,7 +497,7 @@ class PoseTrackerOracle:
             w = tracker_weights(props[a], p_own, p_oth, blurred, self.pa, self.z_floor)
             out["w"].append(w)
             wr = w if weights_from is None else weights_from[a]
-            bins, _ = resample_linear(wr, self.N, int(words[4 + a]))
+            bins, _ = resample_scaled(wr, self.N, int(words[4 + a]), self.z_floor * (1.0 - 2.0 * self.pa))
             out["bins"].append(bins)
             out["Z"].append(tracker_measurement(bins, props[a], roi))
         return out

@@ -134,6 +134,20 @@ int orc_resample_linear(const double *w, int L, int N, uint32_t U, int *out)
     return st;
 }
 
+/* Linear weights with a known bound 1/scale (w scale <= 1): q_i =
+ * floor(2^32 w_i scale), no maximum needed.  The pose tracker's proposal
+ * weights L/Z (pfPose.cpp:300-301) are bounded by 1/(z_floor (1 - 2 Pa)). */
+int orc_resample_scaled(const double *w, int L, int N, uint32_t U, double scale, int *out)
+{
+    if (L <= 0 || N <= 0 || L > (1 << 28) || N > (1 << 28) || !(scale > 0.0)) return -1;
+    uint64_t *q = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)L);
+    for (int i = 0; i < L; ++i)
+        q[i] = (isfinite(w[i]) && w[i] > 0.0) ? fixed32(w[i] * scale) : 0;
+    int st = resample_fixed(q, L, N, U, out);
+    free(q);
+    return st;
+}
+
 int orc_resample_log(const double *lw, int L, int N, uint32_t U, int *out)
 {
     if (L <= 0 || N <= 0 || L > (1 << 28) || N > (1 << 28)) return -1;

@@ -304,7 +304,7 @@ def test_blank_image_falls_back_to_uniform(gpu):
     for a in range(2):
         assert np.all(last["w"][a] == 0.0)
         U = int(oracle.philox_u32(SEED, 24, 0, 6)[4 + a])
-        ref, st = oracle.resample_linear(np.zeros(n_c), N, U)
+        ref, st = oracle.resample_scaled(np.zeros(n_c), N, U, 1e-4 * 0.9)
         assert st == 1 and np.array_equal(last["bins"][a], ref)
     tr.close()
     for a in arms:

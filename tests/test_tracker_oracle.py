@@ -10,6 +10,10 @@ pinned against library routines and closed forms:
 * proposals against their definition from the Philox words;
 * association weights and measurements against hand computations of
   pfPose.cpp:244-320;
+* the proposals' bound-scaled resampling against an exact integer
+  restatement (bisection over big-integer prefix sums), against the
+  maximum-relative resampler where the two scales coincide, and its
+  all-zero fallback against the uniform closed form;
 * get3Dpose against re-projection through K [R | t] and rpy against
   scipy's Rotation.
 """
@@ -153,6 +157,52 @@ def test_association_weights_by_hand():
     assert w[4] == 0.0                      # x = 0 is outside (strict inequality)
     assert w[5] == pytest.approx(1.0 / z5, rel=1e-15)
     assert w[6] == 0.0                      # x = cols
+
+
+def _scaled_brute(w, N, U, scale):
+    """a_j = min{i : C_i N 2^32 > (j 2^32 + U) C_L} over C = prefix sums of
+    q_i = floor(2^32 w_i scale) (the FP64 product, then exact integers)."""
+    import bisect
+    x = np.asarray(w, np.float64) * scale
+    q = [int(math.floor(Fraction(float(v)) * 2 ** 32)) if v > 0 else 0 for v in x]
+    C, acc = [], 0
+    for v in q:
+        acc += v
+        C.append(acc * N * 2 ** 32)
+    return np.array([bisect.bisect_right(C, (j * 2 ** 32 + U) * acc) for j in range(N)])
+
+
+def test_scaled_resampling_exact():
+    rng = np.random.default_rng(7)
+    for trial in range(60):
+        L, N = int(rng.integers(1, 80)), int(rng.integers(1, 120))
+        zf = float(rng.uniform(1e-5, 1e-3))
+        w = rng.random(L) / zf                 # within the bound 1 / zf
+        w[rng.random(L) < 0.3] = 0.0
+        if not w.any():
+            w[0] = 1.0
+        U = int(rng.integers(0, 2 ** 32))
+        a, st = oracle.resample_scaled(w, N, U, zf)
+        assert st == 0 and np.array_equal(a, _scaled_brute(w, N, U, zf)), trial
+
+
+def test_scaled_resampling_equals_max_relative_when_scales_coincide():
+    rng = np.random.default_rng(8)
+    for trial in range(20):
+        L, N = int(rng.integers(2, 50)), int(rng.integers(1, 100))
+        w = rng.random(L)
+        w[int(rng.integers(0, L))] = 64.0       # max w = 1 / scale exactly: w / max == w * scale
+        U = int(rng.integers(0, 2 ** 32))
+        a, _ = oracle.resample_scaled(w, N, U, 1.0 / 64.0)
+        b, _ = oracle.resample_linear(w, N, U)
+        assert np.array_equal(a, b)
+
+
+def test_scaled_resampling_all_zero_is_uniform():
+    L, N, U = 37, 11, 3_000_000_000
+    a, st = oracle.resample_scaled(np.zeros(L), N, U, 9e-5)
+    expect = [((j * 2 ** 32 + U) * L) // (N * 2 ** 32) for j in range(N)]
+    assert st == 1 and list(a) == expect
 
 
 def test_measurement_rows():
