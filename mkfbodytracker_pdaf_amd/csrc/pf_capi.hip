@@ -269,7 +269,7 @@ Pending pending_of(const pf_handle *h, int L)
 // Fixed-point weights and block scans of the step's FP32 log-weights.
 // parity selects which of the two frame counters is read (the other
 // receives t + 1) and which maximum slot holds the frame's maximum.
-int launch_weights(pf_handle *h, const uint32_t *uni, int parity)
+WeightArgs weight_args(pf_handle *h, const uint32_t *uni, int parity)
 {
     WeightArgs w{};
     w.lw32 = h->lw;
@@ -284,6 +284,12 @@ int launch_weights(pf_handle *h, const uint32_t *uni, int parity)
     w.parity = parity;
     w.ustride = 2;
     w.uoff = 1;
+    return w;
+}
+
+int launch_weights(pf_handle *h, const uint32_t *uni, int parity)
+{
+    const WeightArgs w = weight_args(h, uni, parity);
     const int nb = (h->N + kWeightsBlock - 1) / kWeightsBlock;
     hipLaunchKernelGGL((k_weights<kLogF32>), dim3(nb), dim3(kWeightsBlock), 0, h->stream, w);
     return check_launch("k_weights");
@@ -458,6 +464,22 @@ struct AncEstimateL {
     }
 };
 
+// The frame's resampling of the filter's own weights (k_weights of the
+// current frame), its log mean weight to lml[*frame].
+AncArgs frame_anc_args(pf_handle *h, int p, double *lml)
+{
+    AncArgs an{};
+    an.pend = pending_of(h, h->N);
+    an.fs = h->fs;
+    an.anc = h->anc;
+    an.lml = lml;
+    an.lml_frame = h->frame + p;
+    an.N = h->N;
+    an.parity = p;
+    an.kind_linear = 0;
+    return an;
+}
+
 int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, double *lml, bool with_estimate)
 {
     const int p = parity & 1;
@@ -467,15 +489,7 @@ int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, doub
     st = launch_weights(h, uni, p);
     if (st) return st;
     if (with_estimate) {
-        AncArgs an{};
-        an.pend = pending_of(h, h->N);
-        an.fs = h->fs;
-        an.anc = h->anc;
-        an.lml = lml;
-        an.lml_frame = h->frame + p;
-        an.N = h->N;
-        an.parity = p;
-        an.kind_linear = 0;
+        const AncArgs an = frame_anc_args(h, p, lml);
         st = dispatch_d<AncEstimateL>(h->D, h, an);      // reads the new state, h->xB before the swap
         if (!st) st = check_launch("k_ancestors_estimate");
     } else {
@@ -483,6 +497,72 @@ int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, doub
     }
     if (st) return st;
     swap_state(h);
+    return PF_OK;
+}
+
+// The pair launches are compiled for the selection basis with 3 joints (the
+// tracker's 6-row measurement) at every state dimension.
+bool pair_compatible(pf_handle *h0, pf_handle *h1)
+{
+    return h0 && h1 && h0 != h1 && h0->model && h1->model && h0->D == h1->D && h0->N == h1->N &&
+           h0->cfg.device == h1->cfg.device && h0->sel_mode && h1->sel_mode && h0->J == 3 && h1->J == 3 &&
+           h0->m == h1->m && h0->table_floats == h1->table_floats && h0->cfg.block_size == h1->cfg.block_size &&
+           (h0->D == 8 || h0->D == 12 || h0->D == 16 || h0->D == 24);
+}
+
+template <int D>
+struct PairL {
+    static void run(pf_handle *h0, pf_handle *h1, const StepArgs &a0, const StepArgs &a1, const WeightArgs &w0,
+                    const WeightArgs &w1, const AncEstArgs &e0, const AncEstArgs &e1, const EstFinalArgs &f0,
+                    const EstFinalArgs &f1, int block)
+    {
+        hipStream_t s = h0->stream;
+        const dim3 sg((h0->N + block - 1) / block, 2);
+        hipLaunchKernelGGL((k_step_pair<D, kModeRef, 3>), sg, dim3(block),
+                           (size_t)step_lds_bytes(a0.stage_floats, 0), s, a0, a1);
+        const dim3 wg((h0->N + kWeightsBlock - 1) / kWeightsBlock, 2);
+        hipLaunchKernelGGL((k_weights_pair<kLogF32>), wg, dim3(kWeightsBlock), 0, s, w0, w1);
+        const int nb = (h0->N + 255) / 256;
+        hipLaunchKernelGGL((k_ancestors_estimate_pair<D>), dim3(nb, 2), dim3(256),
+                           (size_t)anc_estimate_lds_bytes(e0.an.pend.nb, D), s, e0, e1);
+        hipLaunchKernelGGL((k_estimate_final_pair<D>), dim3(2), dim3(256), 0, s, f0, f1);
+    }
+};
+
+int step_ref_pair(pf_handle *h0, pf_handle *h1, const float *Z0, const float *Z1, const uint32_t *uni0,
+                  const uint32_t *uni1, int parity, double *lml0, double *lml1)
+{
+    if (!pair_compatible(h0, h1)) return set_error(PF_ERR_UNSUPPORTED, "the two filters cannot share launches");
+    const int p = parity & 1;
+    pf_handle *h[2] = {h0, h1};
+    const float *Z[2] = {Z0, Z1};
+    const uint32_t *uni[2] = {uni0, uni1};
+    double *lml[2] = {lml0, lml1};
+    StepArgs a[2];
+    WeightArgs w[2];
+    AncEstArgs e[2];
+    EstFinalArgs f[2];
+    for (int i = 0; i < 2; ++i) {
+        a[i] = step_args(h[i], kModeRef, uni[i], Z[i], 0, h[i]->N, nullptr, nullptr, nullptr, p, kAncArray, nullptr);
+        w[i] = weight_args(h[i], uni[i], p);
+        e[i].an = frame_anc_args(h[i], p, lml[i]);
+        e[i].x = h[i]->xB;                      // the updated state, before the swap
+        e[i].partial = h[i]->est_partial;
+        e[i].ld = h[i]->ld;
+        f[i].partial = h[i]->est_partial;
+        f[i].Ti = h[i]->Tinv;
+        f[i].out = h[i]->est_out;
+        f[i].nb = (h[i]->N + 255) / 256;
+        f[i].N = h[i]->N;
+    }
+    if (a[0].stage_floats != a[1].stage_floats)
+        return set_error(PF_ERR_UNSUPPORTED, "the two filters stage different model images");
+    const int block = h0->cfg.block_size > 0 ? h0->cfg.block_size : 256;
+    int st = dispatch_d<PairL>(h0->D, h0, h1, a[0], a[1], w[0], w[1], e[0], e[1], f[0], f[1], block);
+    if (!st) st = check_launch("pair step");
+    if (st) return st;
+    swap_state(h0);
+    swap_state(h1);
     return PF_OK;
 }
 

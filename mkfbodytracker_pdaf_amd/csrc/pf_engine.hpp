@@ -43,6 +43,15 @@ int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, doub
 // resolve_prefix / resolve_search), and the frame scratch (U_pending, status).
 Pending pending(pf_handle *h, int L);
 FrameScratch *scratch(pf_handle *h);
+// Whether step_ref_pair can advance h0 and h1 together (same state
+// dimension, particle count, device, selection basis with 3 joints, model
+// image size and block size).
+bool pair_compatible(pf_handle *h0, pf_handle *h1);
+// step_ref(h0, ...) and step_ref(h1, ...) with with_estimate, in one launch
+// per stage on h0's stream (grid.y = filter); the frame counter `parity` of
+// each handle is read.  The caller orders h1's stream around it.
+int step_ref_pair(pf_handle *h0, pf_handle *h1, const float *Z0, const float *Z1, const uint32_t *uni0,
+                  const uint32_t *uni1, int parity, double *lml0, double *lml1);
 // Reads and clears the handle's status word (synchronises the handle's stream).
 int take_status(pf_handle *h, int *degenerate);
 

@@ -29,6 +29,7 @@ namespace mkfbt {
 
 constexpr int kMaxJoints = 8;
 constexpr int kWeightsBlock = 256;     // input block of the resampler scans
+constexpr int kStageBlocks = 12;       // input blocks the ancestor search stages in LDS
 constexpr int kMaxScanBlocks = 4096;   // => at most 2^20 resampled weights
 constexpr int kMaxLdsBytes = 64 * 1024;  // per-workgroup LDS budget
 
@@ -50,6 +51,8 @@ struct Pending {
     const unsigned long long *__restrict__ bsum;
     const double *__restrict__ bexp;
     int L, nb;
+    int bshift = 8;                              // log2 of the inputs per block (kWeightsBlock)
+    int stage = kStageBlocks * kWeightsBlock;    // LDS elements of the staged search
 };
 
 struct StepArgs {
@@ -102,10 +105,12 @@ __device__ __forceinline__ T wave_sum(T v)
 // The prefix sum is held as block-local scans (cloc) plus block sums (bsum);
 // resolve_ancestor() finds a_i for one block of outputs.
 
-// LDS workspace of resolve_ancestor: nb + 1 prefix words, kStageBlocks
-// staged input blocks, wave totals and a few broadcast slots.
-constexpr int kStageBlocks = 12;
-__host__ __device__ constexpr int resolve_lds_words(int nb) { return nb + 1 + kStageBlocks * kWeightsBlock + 8; }
+// LDS workspace of resolve_ancestor: nb + 1 prefix words, the staged input
+// elements (Pending::stage), wave totals and a few broadcast slots.
+__host__ __device__ constexpr int resolve_lds_words(int nb, int stage = kStageBlocks * kWeightsBlock)
+{
+    return nb + 1 + stage + 8;
+}
 
 // Phase 1 of the ancestor resolution: exclusive prefix of the block sums in
 // LDS (pre[0..nb], pre[nb] = total) and, if want_es, the FP64 sum of the
@@ -118,7 +123,7 @@ __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, 
     const int lane = tid & 63, wid = tid >> 6, nwaves = (nt + 63) >> 6;
     unsigned long long *pre = w;
     unsigned long long *stg = w + nb + 1;
-    unsigned long long *misc = stg + kStageBlocks * kWeightsBlock;
+    unsigned long long *misc = stg + pd.stage;
     const int per = (nb + nt - 1) / nt;
     const int c0 = min(nb, tid * per), c1 = min(nb, c0 + per);
     // loads in groups of 8 so that their latencies overlap
@@ -183,7 +188,8 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
     const int tid = threadIdx.x, nt = blockDim.x, nb = pd.nb;
     unsigned long long *pre = w;
     unsigned long long *stg = w + nb + 1;
-    unsigned long long *misc = stg + kStageBlocks * kWeightsBlock;
+    unsigned long long *misc = stg + pd.stage;
+    const int bs = 1 << pd.bshift;
     // 2. the input block of each output: largest b with pre[b] * S <= t_j
     const int j = j0 + tid;
     const int jc = min(j, N - 1);
@@ -208,14 +214,14 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
     if (j == jlast) misc[5] = (unsigned long long)bj;
     __syncthreads();
     const int blo = (int)misc[4], bhi = (int)misc[5];
-    // 3. a span of at most kStageBlocks input blocks is staged in LDS
-    //    (C = pre + cloc, one memory latency) and searched there; wider spans
-    //    (strongly concentrated weights) search the input block in global
-    //    memory, one dependent probe per level.
+    // 3. a span of input blocks that fits the stage (pd.stage elements) is
+    //    staged in LDS (C = pre + cloc, one memory latency) and searched
+    //    there; wider spans (strongly concentrated weights) search the input
+    //    block in global memory, one dependent probe per level.
     int a;
-    if (bhi - blo < kStageBlocks) {
-        const int base = blo * kWeightsBlock;
-        const int n = min(pd.L, (bhi + 1) * kWeightsBlock) - base;
+    if (((bhi - blo + 1) << pd.bshift) <= pd.stage) {
+        const int base = blo << pd.bshift;
+        const int n = min(pd.L, (bhi + 1) << pd.bshift) - base;
         for (int k0 = tid; k0 < n; k0 += 4 * nt) {      // 4 loads in flight per lane
             unsigned long long c4[4];
 #pragma unroll
@@ -223,11 +229,11 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int k = k0 + u * nt;
-                if (k < n) stg[k] = pre[blo + k / kWeightsBlock] + c4[u];
+                if (k < n) stg[k] = pre[blo + (k >> pd.bshift)] + c4[u];
             }
         }
         __syncthreads();
-        int l = (bj - blo) * kWeightsBlock, h = min(n, (bj - blo + 1) * kWeightsBlock) - 1;
+        int l = (bj - blo) * bs, h = min(n, (bj - blo + 1) * bs) - 1;
         while (l < h) {
             const int mid = (l + h) >> 1;
             if ((u128)stg[mid] * S > tj) h = mid;
@@ -236,7 +242,7 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
         a = base + l;
     } else {
         const unsigned long long pb = pre[bj];
-        int l = bj * kWeightsBlock, h = min(pd.L, (bj + 1) * kWeightsBlock) - 1;
+        int l = bj * bs, h = min(pd.L, (bj + 1) * bs) - 1;
         while (l < h) {
             const int mid = (l + h) >> 1;
             if ((u128)(pb + pd.cloc[mid]) * S > tj) h = mid;
@@ -657,7 +663,7 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
 }
 
 template <int D, int MODE, int SELJ>
-__global__ void __launch_bounds__(256) k_step(StepArgs a)
+__device__ __forceinline__ void step_body(const StepArgs &a)
 {
     constexpr int NP = npacked(D);
     extern __shared__ float4 sm4[];
@@ -726,6 +732,20 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
     }
 }
 
+template <int D, int MODE, int SELJ>
+__global__ void __launch_bounds__(256) k_step(StepArgs a)
+{
+    step_body<D, MODE, SELJ>(a);
+}
+
+// Two filters of the same shape in one launch (grid.y = filter): the pose
+// tracker advances its two arms together.
+template <int D, int MODE, int SELJ>
+__global__ void __launch_bounds__(256) k_step_pair(StepArgs a0, StepArgs a1)
+{
+    step_body<D, MODE, SELJ>(blockIdx.y ? a1 : a0);
+}
+
 // Weights of the resampler: q_i and per-block inclusive scans.
 enum WeightKind { kLogF32 = 0, kLogF64 = 1, kLinF64 = 2 };
 
@@ -743,7 +763,7 @@ struct WeightArgs {
 };
 
 template <int KIND>
-__global__ void __launch_bounds__(kWeightsBlock) k_weights(WeightArgs a)
+__device__ __forceinline__ void weights_body(const WeightArgs &a)
 {
     __shared__ unsigned long long s_wave[4];
     __shared__ double s_dwave[4];
@@ -791,6 +811,18 @@ __global__ void __launch_bounds__(kWeightsBlock) k_weights(WeightArgs a)
         a.fs->U_pending = a.uni[(size_t)t * a.ustride + a.uoff];
         if (a.frame_next) *a.frame_next = t + 1;
     }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kWeightsBlock) k_weights(WeightArgs a)
+{
+    weights_body<KIND>(a);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kWeightsBlock) k_weights_pair(WeightArgs a0, WeightArgs a1)
+{
+    weights_body<KIND>(blockIdx.y ? a1 : a0);
 }
 
 // Materialises the pending resampling: anc[j] for every output j, and the
@@ -1029,8 +1061,8 @@ __global__ void __launch_bounds__(256) k_estimate(const float *x, const int *anc
 
 // Sum of the nb block partials, / N, back to the model basis (xbar = Ti xbar').
 template <int D>
-__global__ void __launch_bounds__(256) k_estimate_final(const double *partial, int nb, int N, const double *Ti,
-                                                        double *out)
+__device__ __forceinline__ void estimate_final_body(const double *partial, int nb, int N, const double *Ti,
+                                                    double *out)
 {
     __shared__ double s_v[D][16][17];
     __shared__ double s_run[D][16];
@@ -1050,12 +1082,34 @@ __global__ void __launch_bounds__(256) k_estimate_final(const double *partial, i
     }
 }
 
+template <int D>
+__global__ void __launch_bounds__(256) k_estimate_final(const double *partial, int nb, int N, const double *Ti,
+                                                        double *out)
+{
+    estimate_final_body<D>(partial, nb, N, Ti, out);
+}
+
+// The estimates of two filters (block b = filter b).
+struct EstFinalArgs {
+    const double *partial;
+    const double *Ti;
+    double *out;
+    int nb, N;
+};
+
+template <int D>
+__global__ void __launch_bounds__(256) k_estimate_final_pair(EstFinalArgs e0, EstFinalArgs e1)
+{
+    const EstFinalArgs &e = blockIdx.x ? e1 : e0;
+    estimate_final_body<D>(e.partial, e.nb, e.N, e.Ti, e.out);
+}
+
 // k_ancestors, then the block sums of getEstimator over the resampled state
 // (x[anc_j], summed as k_estimate does; the resolver's LDS is reused for the
 // reduction).  Followed by k_estimate_final.  Dynamic LDS:
 // anc_estimate_lds_bytes(nb, D).
 template <int D>
-__global__ void __launch_bounds__(256) k_ancestors_estimate(AncArgs a, const float *x, int ld, double *partial)
+__device__ __forceinline__ void ancestors_estimate_body(const AncArgs &a, const float *x, int ld, double *partial)
 {
     extern __shared__ unsigned long long wsp[];
     const int j0 = blockIdx.x * blockDim.x;
@@ -1074,6 +1128,26 @@ __global__ void __launch_bounds__(256) k_ancestors_estimate(AncArgs a, const flo
     block_sum256<D>(v, reinterpret_cast<double (*)[16][17]>(ws), reinterpret_cast<double (*)[16]>(ws + D * 16 * 17),
                     ws + D * 16 * 17 + D * 16);
     if (threadIdx.x < D) partial[(size_t)blockIdx.x * D + threadIdx.x] = ws[D * 16 * 17 + D * 16 + threadIdx.x];
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_ancestors_estimate(AncArgs a, const float *x, int ld, double *partial)
+{
+    ancestors_estimate_body<D>(a, x, ld, partial);
+}
+
+struct AncEstArgs {
+    AncArgs an;
+    const float *x;     // the resampled filter's updated state (x rows, leading dimension ld)
+    double *partial;    // block sums of the estimate
+    int ld;
+};
+
+template <int D>
+__global__ void __launch_bounds__(256) k_ancestors_estimate_pair(AncEstArgs e0, AncEstArgs e1)
+{
+    const AncEstArgs &e = blockIdx.y ? e1 : e0;
+    ancestors_estimate_body<D>(e.an, e.x, e.ld, e.partial);
 }
 
 __host__ __device__ constexpr int anc_estimate_lds_bytes(int nb, int D)

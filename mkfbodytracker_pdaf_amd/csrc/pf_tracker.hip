@@ -49,6 +49,7 @@ constexpr int kMaxRadius = 15;
 constexpr int kBlurW = 32, kBlurH = 16, kBlurThreads = 128;
 constexpr int kProposeBlock = 256;
 constexpr int kMaxArmD = 24;              // k_propose stages 3 D + 2 projection operands per arm
+constexpr int kPropShift = 10;            // the proposal resampler's input blocks: 1024 proposals
 constexpr int kMaxProposals = 1 << 20;    // resampler scan limit
 
 // Frame record written by k_propose: what k_prop_select and the inspection
@@ -225,21 +226,23 @@ __device__ inline void hand_projections(const ArmDev &a0, const ArmDev &a1, doub
     __syncthreads();
 }
 
-// Proposals of both arms (grid.y = arm), two per lane and chunk of 256
-// pairs; a block walks the chunks c = blockIdx.x, + gridDim.x, ... (one
-// resident round of blocks), so the frame prologue is paid once per block.
-// Chunk c holds proposals 512 c .. 512 c + 511, i.e. resampler input blocks
-// 2 c (lanes 0-127) and 2 c + 1 (lanes 128-255), whose block-local inclusive
-// scans and sums of q = fixed32(w z) are written here: with z = z_floor
-// (1 - 2 Pa) > 0 every weight is at most 1 / z, so w z is in [0, 1] and the
-// maximum is not needed (resample(weights, N), pfPose.cpp:300-301).
+// Proposals of both arms (grid.y = arm).  A chunk of 512 proposal pairs
+// (1024 proposals) is one input block of the proposal resampler; a block
+// walks the chunks c = blockIdx.x, + gridDim.x, ... (one resident round of
+// blocks, so the frame prologue is paid once per block), each lane taking
+// pairs 512 c + tid and 512 c + 256 + tid (one Philox block per pair).  It
+// writes the weights and the block-local inclusive scans and block sums of
+// q = fixed32(w z): with z = z_floor (1 - 2 Pa) > 0 every weight is at most
+// 1 / z, so w z is in [0, 1] and the maximum is not needed
+// (resample(weights, N), pfPose.cpp:300-301).
 __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev a0, ArmDev a1, const uint8_t *img,
                                                           FrameRec *rec)
 {
-    static_assert(kProposeBlock == 256 && kWeightsBlock == 256, "a chunk of 256 pairs is two resampler blocks");
+    static_assert(kProposeBlock == 256 && (1 << kPropShift) == 4 * kProposeBlock,
+                  "a chunk is two pairs per lane, one resampler block");
     __shared__ double s_stage[2 * (3 * kMaxArmD + 2)];
     __shared__ double s_y[4];
-    __shared__ unsigned long long s_wave[kProposeBlock / 64];
+    __shared__ unsigned long long s_wave[2][kProposeBlock / 64];
     const int t = *p.frame;
     const long long g = p.g0 + t;
     const bool uniform = (g == p.uniform_frame);
@@ -261,62 +264,86 @@ __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev 
     const double var = p.spread * (double)roi[2];
     const double inv2piv = 1.0 / (6.283185307179586 * var), nhiv = -0.5 / var;
     const long long pairs = p.n_c / 2;
-    const int nb = (p.n_c + kWeightsBlock - 1) / kWeightsBlock;
-    const int nchunks = (int)((pairs + kProposeBlock - 1) / kProposeBlock);
+    const int nchunks = (p.n_c + (1 << kPropShift) - 1) >> kPropShift;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const long long k = (long long)c * kProposeBlock + threadIdx.x;
-        double wv[2] = {0.0, 0.0};
-        unsigned long long q[2] = {0ull, 0ull};
-        if (k < pairs) {
-            uint32_t wd[4];
-            proposal_block(p, arm, g, k, wd);     // one Philox block serves both proposals
+        unsigned long long q[4] = {0ull, 0ull, 0ull, 0ull};
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                double px, py;
-                proposal_at(p, uniform, roi, yo0, yo1, wd[2 * i], wd[2 * i + 1], px, py);
-                // pfPose.cpp:251-266 (arm 1) and :274-289 (arm 2)
-                if (py > 0.0 && py < (double)p.rows && px > 0.0 && px < (double)p.cols) {
-                    const unsigned v = img[(size_t)(int)py * p.cols + (int)px];
-                    if (v != 0u) {     // (v / 255) / Z with one division
-                        const double po = density(px, py, yo0, yo1, nhiv, inv2piv);
-                        const double pt = density(px, py, yt0, yt1, nhiv, inv2piv);
-                        wv[i] = (double)v / (255.0 * fma(p.pa, po, fma(p.pa, pt, p.zfloor)));
+        for (int h = 0; h < 2; ++h) {
+            const long long k = ((long long)c << (kPropShift - 1)) + h * kProposeBlock + threadIdx.x;
+            if (k < pairs) {
+                uint32_t wd[4];
+                proposal_block(p, arm, g, k, wd);     // one Philox block serves both proposals
+                double wv[2] = {0.0, 0.0};
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    double px, py;
+                    proposal_at(p, uniform, roi, yo0, yo1, wd[2 * i], wd[2 * i + 1], px, py);
+                    // pfPose.cpp:251-266 (arm 1) and :274-289 (arm 2)
+                    if (py > 0.0 && py < (double)p.rows && px > 0.0 && px < (double)p.cols) {
+                        const unsigned v = img[(size_t)(int)py * p.cols + (int)px];
+                        if (v != 0u) {     // (v / 255) / Z with one division
+                            const double po = density(px, py, yo0, yo1, nhiv, inv2piv);
+                            const double pt = density(px, py, yt0, yt1, nhiv, inv2piv);
+                            wv[i] = (double)v / (255.0 * fma(p.pa, po, fma(p.pa, pt, p.zfloor)));
+                        }
                     }
+                    q[2 * h + i] = (isfinite(wv[i]) && wv[i] > 0.0) ? fixed32(wv[i] * p.zfloor) : 0ull;
                 }
-                q[i] = (isfinite(wv[i]) && wv[i] > 0.0) ? fixed32(wv[i] * p.zfloor) : 0ull;
+                *reinterpret_cast<double2 *>(A.w + 2 * k) = make_double2(wv[0], wv[1]);
             }
-            *reinterpret_cast<double2 *>(A.w + 2 * k) = make_double2(wv[0], wv[1]);
         }
-        // inclusive scan of the pair sums within each 128-lane half
-        unsigned long long sc = q[0] + q[1];
+        // block-local inclusive scan over the chunk: pairs of the first half
+        // (lane order), then those of the second half
+        unsigned long long s1 = q[0] + q[1], s2 = q[2] + q[3];
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
-            const unsigned long long n = __shfl_up(sc, o, 64);
-            if (lane >= o) sc += n;
+            const unsigned long long n1 = __shfl_up(s1, o, 64), n2 = __shfl_up(s2, o, 64);
+            if (lane >= o) {
+                s1 += n1;
+                s2 += n2;
+            }
         }
-        if (lane == 63) s_wave[wid] = sc;
+        if (lane == 63) {
+            s_wave[0][wid] = s1;
+            s_wave[1][wid] = s2;
+        }
         __syncthreads();
-        if (wid & 1) sc += s_wave[wid - 1];
-        if (k < pairs) {
-            A.cloc[2 * k] = sc - q[1];
-            A.cloc[2 * k + 1] = sc;
+        unsigned long long first = 0;
+#pragma unroll
+        for (int w = 0; w < kProposeBlock / 64; ++w) {
+            first += s_wave[0][w];
+            if (w < wid) {
+                s1 += s_wave[0][w];
+                s2 += s_wave[1][w];
+            }
         }
-        const int rb = 2 * c + (wid >> 1);
-        if ((threadIdx.x & 127) == 127 && rb < nb) A.bsum[rb] = sc;
+        s2 += first;
+        const long long k1 = ((long long)c << (kPropShift - 1)) + threadIdx.x, k2 = k1 + kProposeBlock;
+        if (k1 < pairs) {
+            A.cloc[2 * k1] = s1 - q[1];
+            A.cloc[2 * k1 + 1] = s1;
+        }
+        if (k2 < pairs) {
+            A.cloc[2 * k2] = s2 - q[3];
+            A.cloc[2 * k2 + 1] = s2;
+        }
+        if (threadIdx.x == kProposeBlock - 1) A.bsum[c] = s2;
         __syncthreads();                     // s_wave is reused by the next chunk
     }
 }
 
-// resample(weights, N) of arm `arm`'s proposals (pfPose.cpp:300-301) and its
-// measurement (pfPose.cpp:303-320), one lane per particle: the systematic
-// resampler's ancestor resolution over the n_c proposal weights (scanned by
-// k_propose), the chosen proposal recomputed from its Philox words, the 6
-// rows written as FP32.  All weights zero (a blank image): uniform indices
-// and the degenerate status bit, as the filter's resampler does.  Dynamic
-// LDS: 8 resolve_lds_words(nb).
-__global__ void __launch_bounds__(256) k_prop_select(PropParams p, int arm, Pending pend, FrameScratch *fs,
-                                                     const FrameRec *rec, int *bins, float *Z, int N)
+// One arm's proposal selection: its resampler scans, frame scratch, and the
+// chosen indices and measurement rows it writes.
+struct SelectArgs {
+    Pending pend;
+    FrameScratch *fs;
+    int *bins;
+    float *Z;
+};
+
+__device__ __forceinline__ void prop_select_body(const PropParams &p, int arm, const SelectArgs &sa,
+                                                 const FrameRec *rec, int N)
 {
     extern __shared__ unsigned long long wsp[];
     const int j0 = blockIdx.x * blockDim.x, j = j0 + threadIdx.x;
@@ -327,13 +354,14 @@ __global__ void __launch_bounds__(256) k_prop_select(PropParams p, int arm, Pend
     const bool uniform = rec->uniform != 0;
     const double y0 = rec->y[2 * arm], y1 = rec->y[2 * arm + 1];
     double es = 0.0;
-    const unsigned long long total = resolve_prefix(pend, wsp, false, es);
-    const int c = resolve_search(pend, N, fs->U_pending, total, j0, wsp);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && total == 0ull) fs->status |= 1u;
+    const unsigned long long total = resolve_prefix(sa.pend, wsp, false, es);
+    const int c = resolve_search(sa.pend, N, sa.fs->U_pending, total, j0, wsp);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && total == 0ull) sa.fs->status |= 1u;
     if (j >= N) return;
-    bins[j] = c;
+    sa.bins[j] = c;
     double px, py;
     proposal(p, arm, g, uniform, roi, y0, y1, c, px, py);
+    float *Z = sa.Z;
     const double fx = (double)roi[0] + (double)roi[2] / 2.0;
     Z[j] = (float)fx;
     Z[(size_t)N + j] = (float)((double)roi[1] + 0.5 * (double)roi[3]);
@@ -341,6 +369,18 @@ __global__ void __launch_bounds__(256) k_prop_select(PropParams p, int arm, Pend
     Z[3 * (size_t)N + j] = (float)py;
     Z[4 * (size_t)N + j] = (float)fx;
     Z[5 * (size_t)N + j] = (float)((double)roi[1] + 1.65 * (double)roi[3]);
+}
+
+__global__ void __launch_bounds__(256) k_prop_select(PropParams p, int arm, SelectArgs sa, const FrameRec *rec, int N)
+{
+    prop_select_body(p, arm, sa, rec, N);
+}
+
+// Both arms in one launch (grid.y = arm).
+__global__ void __launch_bounds__(256) k_prop_select_pair(PropParams p, SelectArgs s0, SelectArgs s1,
+                                                          const FrameRec *rec, int N)
+{
+    prop_select_body(p, (int)blockIdx.y, blockIdx.y ? s1 : s0, rec, N);
 }
 
 // All proposals of the recorded frame (inspection only): out 2 x 2 x n_c.
@@ -517,6 +557,16 @@ int tr_check(pf_tracker *tr)
     return PF_OK;
 }
 
+// The arm handle's resampler scratch as the pending resampling of its n_c
+// proposal weights, in k_propose's input blocks of 1024.
+Pending proposal_pending(pf_tracker *tr, int a)
+{
+    Pending pend = eng::pending(tr->arm[a], tr->n_c);
+    pend.bshift = kPropShift;
+    pend.nb = (tr->n_c + (1 << kPropShift) - 1) >> kPropShift;
+    return pend;
+}
+
 ArmDev arm_dev(pf_tracker *tr, int a)
 {
     ArmDev d{};
@@ -524,7 +574,7 @@ ArmDev arm_dev(pf_tracker *tr, int a)
     d.hp = tr->hp[a];
     d.pm = tr->pm[a];
     d.w = tr->w[a];
-    const Pending pend = eng::pending(tr->arm[a], tr->n_c);
+    const Pending pend = proposal_pending(tr, a);
     d.cloc = const_cast<unsigned long long *>(pend.cloc);
     d.bsum = const_cast<unsigned long long *>(pend.bsum);
     d.fs = eng::scratch(tr->arm[a]);
@@ -609,18 +659,36 @@ int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const i
                        tr->s[0], p, a0, a1, (const uint8_t *)tr->blur, tr->rec);
     st = launch_check("k_propose");
     if (st) return st;
+    SelectArgs sa[2];
+    for (int a = 0; a < 2; ++a) {
+        sa[a].pend = proposal_pending(tr, a);
+        sa[a].fs = eng::scratch(tr->arm[a]);
+        sa[a].bins = tr->bins[a];
+        sa[a].Z = tr->Z[a];
+    }
+    const size_t sel_lds = 8 * (size_t)resolve_lds_words(sa[0].pend.nb);
+    const uint32_t *u_arm[2] = {tr->uni, tr->uni + 2 * (size_t)tr->uni_cap};
+    double *lml[2] = {out.lml, out.lml ? out.lml + T : nullptr};
+    if (eng::pair_compatible(tr->arm[0], tr->arm[1])) {
+        // both arms advance together on stream 0, one launch per stage
+        hipLaunchKernelGGL(k_prop_select_pair, dim3((tr->N + 255) / 256, 2), dim3(256), sel_lds, tr->s[0], p, sa[0],
+                           sa[1], (const FrameRec *)tr->rec, tr->N);
+        st = launch_check("k_prop_select_pair");
+        if (!st) st = eng::step_ref_pair(tr->arm[0], tr->arm[1], tr->Z[0], tr->Z[1], u_arm[0], u_arm[1], parity,
+                                         lml[0], lml[1]);
+        if (st) return st;
+        return launch_pose(tr, tr->fctr, out);
+    }
+    // otherwise each arm on its own stream, forked here and joined before the pose
     TR_TRY(hipEventRecord(tr->ev_fork[parity], tr->s[0]));
     TR_TRY(hipStreamWaitEvent(tr->s[1], tr->ev_fork[parity], 0));
     for (int a = 0; a < 2; ++a) {
         pf_handle *h = tr->arm[a];
-        const Pending pend = eng::pending(h, tr->n_c);
-        hipLaunchKernelGGL(k_prop_select, dim3((tr->N + 255) / 256), dim3(256), 8 * (size_t)resolve_lds_words(pend.nb),
-                           tr->s[a], p, a, pend, eng::scratch(h), (const FrameRec *)tr->rec, tr->bins[a], tr->Z[a],
-                           tr->N);
+        hipLaunchKernelGGL(k_prop_select, dim3((tr->N + 255) / 256), dim3(256), sel_lds, tr->s[a], p, a, sa[a],
+                           (const FrameRec *)tr->rec, tr->N);
         st = launch_check("k_prop_select");
         if (st) return st;
-        st = eng::step_ref(h, tr->Z[a], tr->uni + 2 * (size_t)tr->uni_cap * a, parity,
-                           out.lml ? out.lml + (size_t)a * T : nullptr, true);
+        st = eng::step_ref(h, tr->Z[a], u_arm[a], parity, lml[a], true);
         if (st) return st;
     }
     TR_TRY(hipEventRecord(tr->ev_join[parity], tr->s[1]));
@@ -941,6 +1009,9 @@ int pf_tracker_run_dev(pf_tracker *tr, int32_t T, const uint8_t *likelihood, con
         if (st) return st;
     }
     TR_TRY(hipEventRecord(tr->ev_t1, tr->s[0]));
+    // arm 2's stream orders its later work after the run (the frames may
+    // have advanced arm 2 on stream 0)
+    TR_TRY(hipStreamWaitEvent(tr->s[1], tr->ev_t1, 0));
     tr->frames_done += T;
     tr->ran = true;
     return PF_OK;
