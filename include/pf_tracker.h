@@ -21,12 +21,17 @@
  *
  * The tracker borrows two filter handles (include/pf_rbpf.h) with loaded
  * models and initialised tracks (pf_reset with the bins of pfPose.cpp:68-71);
- * it runs arm 1's work on arm 1's stream and arm 2's on arm 2's, joined by
- * events, and owns its own buffers.  Calls on a tracker and on its two
+ * when the two arms have the same shape (state dimension, particle count,
+ * model image, selection basis) both advance in one launch per stage on
+ * arm 1's stream, otherwise arm 1's work runs on arm 1's stream and arm 2's
+ * on arm 2's, joined by events; after a run arm 2's stream is ordered after
+ * arm 1's.  The tracker owns its own buffers.  Calls on a tracker and on its two
  * handles are serialised by the caller.  Status codes are those of
  * pf_rbpf.h; PF_WARN_DEGENERATE reports that some resampling of the call
  * fell back to uniform weights (every proposal weight zero, the reference's
- * NaN weights branch).
+ * NaN weights branch).  The proposal weights are resampled as fixed-point
+ * values floor(2^32 w z) with z = z_floor (1 - 2 p_assoc), the reciprocal of
+ * their bound (pf_tracker_create requires 0 <= p_assoc < 1/2, z_floor > 0).
  *
  * Randomness: the proposal positions and the per-frame uniforms are words
  * of the seeded Philox4x32-10 stream of pf_philox_u32.  For global frame g
