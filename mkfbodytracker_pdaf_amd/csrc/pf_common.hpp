@@ -8,6 +8,22 @@
 
 namespace mkfbt {
 
+// fma(a, b, c) for the deterministic polynomials.  On the device it is the
+// three-address v_fma_f64: the compiler's own FP64 FMA selection ties the
+// addend to the result register, so a loop-invariant coefficient held in a
+// register is copied before every Horner step; this form reads it in place.
+// Either way one correctly rounded fused multiply-add, as on the host.
+__host__ __device__ inline double dfma(double a, double b, double c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#else
+    return fma(a, b, c);
+#endif
+}
+
 typedef unsigned __int128 u128;
 
 // Packed upper-triangular index of (i, j), i <= j, row-major, n x n.
@@ -30,18 +46,18 @@ __host__ __device__ inline double det_exp(double x)
     double r = fma(-n, 6.93147180369123816490e-01, x);
     r = fma(-n, 1.90821492927058770002e-10, r);
     double p = 1.0 / 479001600.0;
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
-    p = fma(p, r, 0.5);
-    p = fma(p, r, 1.0);
-    p = fma(p, r, 1.0);
+    p = dfma(p, r, 1.0 / 39916800.0);
+    p = dfma(p, r, 1.0 / 3628800.0);
+    p = dfma(p, r, 1.0 / 362880.0);
+    p = dfma(p, r, 1.0 / 40320.0);
+    p = dfma(p, r, 1.0 / 5040.0);
+    p = dfma(p, r, 1.0 / 720.0);
+    p = dfma(p, r, 1.0 / 120.0);
+    p = dfma(p, r, 1.0 / 24.0);
+    p = dfma(p, r, 1.0 / 6.0);
+    p = dfma(p, r, 0.5);
+    p = dfma(p, r, 1.0);
+    p = dfma(p, r, 1.0);
     return ldexp(p, (int)n);
 }
 

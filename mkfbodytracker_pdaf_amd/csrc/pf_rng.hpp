@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pf_common.hpp"
+
 namespace mkfbt {
 
 __host__ __device__ inline void philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
@@ -60,17 +62,17 @@ __host__ __device__ inline double det_log(double x)
     const double s = f / (2.0 + f);
     const double z = s * s;
     double p = 1.0 / 25.0;
-    p = fma(p, z, 1.0 / 23.0);
-    p = fma(p, z, 1.0 / 21.0);
-    p = fma(p, z, 1.0 / 19.0);
-    p = fma(p, z, 1.0 / 17.0);
-    p = fma(p, z, 1.0 / 15.0);
-    p = fma(p, z, 1.0 / 13.0);
-    p = fma(p, z, 1.0 / 11.0);
-    p = fma(p, z, 1.0 / 9.0);
-    p = fma(p, z, 1.0 / 7.0);
-    p = fma(p, z, 1.0 / 5.0);
-    p = fma(p, z, 1.0 / 3.0);
+    p = dfma(p, z, 1.0 / 23.0);
+    p = dfma(p, z, 1.0 / 21.0);
+    p = dfma(p, z, 1.0 / 19.0);
+    p = dfma(p, z, 1.0 / 17.0);
+    p = dfma(p, z, 1.0 / 15.0);
+    p = dfma(p, z, 1.0 / 13.0);
+    p = dfma(p, z, 1.0 / 11.0);
+    p = dfma(p, z, 1.0 / 9.0);
+    p = dfma(p, z, 1.0 / 7.0);
+    p = dfma(p, z, 1.0 / 5.0);
+    p = dfma(p, z, 1.0 / 3.0);
     const double s2 = s + s;
     const double lm = fma(s2 * z, p, s2);
     const double de = (double)e;
@@ -91,24 +93,24 @@ __host__ __device__ inline void det_sincos_turn(uint32_t w, double &sn, double &
     const double x = (double)g * (1.5707963267948966 / 1073741824.0);
     const double z = x * x;
     double ps = -1.0 / 121645100408832000.0;
-    ps = fma(ps, z, 1.0 / 355687428096000.0);
-    ps = fma(ps, z, -1.0 / 1307674368000.0);
-    ps = fma(ps, z, 1.0 / 6227020800.0);
-    ps = fma(ps, z, -1.0 / 39916800.0);
-    ps = fma(ps, z, 1.0 / 362880.0);
-    ps = fma(ps, z, -1.0 / 5040.0);
-    ps = fma(ps, z, 1.0 / 120.0);
-    ps = fma(ps, z, -1.0 / 6.0);
+    ps = dfma(ps, z, 1.0 / 355687428096000.0);
+    ps = dfma(ps, z, -1.0 / 1307674368000.0);
+    ps = dfma(ps, z, 1.0 / 6227020800.0);
+    ps = dfma(ps, z, -1.0 / 39916800.0);
+    ps = dfma(ps, z, 1.0 / 362880.0);
+    ps = dfma(ps, z, -1.0 / 5040.0);
+    ps = dfma(ps, z, 1.0 / 120.0);
+    ps = dfma(ps, z, -1.0 / 6.0);
     const double s0 = fma(x * z, ps, x);
     double pc = -1.0 / 6402373705728000.0;
-    pc = fma(pc, z, 1.0 / 20922789888000.0);
-    pc = fma(pc, z, -1.0 / 87178291200.0);
-    pc = fma(pc, z, 1.0 / 479001600.0);
-    pc = fma(pc, z, -1.0 / 3628800.0);
-    pc = fma(pc, z, 1.0 / 40320.0);
-    pc = fma(pc, z, -1.0 / 720.0);
-    pc = fma(pc, z, 1.0 / 24.0);
-    pc = fma(pc, z, -0.5);
+    pc = dfma(pc, z, 1.0 / 20922789888000.0);
+    pc = dfma(pc, z, -1.0 / 87178291200.0);
+    pc = dfma(pc, z, 1.0 / 479001600.0);
+    pc = dfma(pc, z, -1.0 / 3628800.0);
+    pc = dfma(pc, z, 1.0 / 40320.0);
+    pc = dfma(pc, z, -1.0 / 720.0);
+    pc = dfma(pc, z, 1.0 / 24.0);
+    pc = dfma(pc, z, -0.5);
     const double c0 = fma(z, pc, 1.0);
     const double a = flip ? c0 : s0;   // sin of the angle within the quadrant
     const double b = flip ? s0 : c0;   // cos
