@@ -105,30 +105,52 @@ __global__ void __launch_bounds__(kBlurThreads) k_blur(BlurArgs a)
     const long long f = a.frame ? (long long)*a.frame : 0;
     const uint8_t *src = a.src + f * a.frame_stride;
     if ((int)threadIdx.x < NT) s_tap[threadIdx.x] = a.taps[threadIdx.x];
-    for (int e = threadIdx.x; e < IH * IW; e += blockDim.x) {
+    // all of a thread's source loads are issued before its first LDS store,
+    // so the tile costs one memory latency rather than one per element
+    constexpr int kPer = (IH * IW + kBlurThreads - 1) / kBlurThreads;
+    uint8_t v[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int e = (int)threadIdx.x + r * kBlurThreads;
         const int iy = e / IW, ix = e - iy * IW;
         const int gy = reflect101(y0 + iy - R, a.rows), gx = reflect101(x0 + ix - R, a.cols);
-        s_in[e] = src[(size_t)gy * a.cols + gx];
+        v[r] = e < IH * IW ? src[(size_t)gy * a.cols + gx] : (uint8_t)0;
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < IH * kBlurW; e += blockDim.x) {
-        const int iy = e / kBlurW, ox = e - iy * kBlurW;
-        const uint8_t *p = s_in + iy * IW + ox;
-        float acc = 0.f;
 #pragma unroll
-        for (int k = 0; k < NT; ++k) acc = fmaf(s_tap[k], (float)p[k], acc);
-        s_row[e] = acc;
+    for (int r = 0; r < kPer; ++r) {
+        const int e = (int)threadIdx.x + r * kBlurThreads;
+        if (e < IH * IW) s_in[e] = v[r];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < kBlurH * kBlurW; e += blockDim.x) {
+    // both passes unrolled over a thread's outputs, so their independent fma
+    // chains interleave
+    constexpr int kRow = (IH * kBlurW + kBlurThreads - 1) / kBlurThreads;
+#pragma unroll
+    for (int r = 0; r < kRow; ++r) {
+        const int e = (int)threadIdx.x + r * kBlurThreads;
+        if (e < IH * kBlurW) {
+            const int iy = e / kBlurW, ox = e - iy * kBlurW;
+            const uint8_t *p = s_in + iy * IW + ox;
+            float acc = 0.f;
+#pragma unroll
+            for (int k = 0; k < NT; ++k) acc = fmaf(s_tap[k], (float)p[k], acc);
+            s_row[e] = acc;
+        }
+    }
+    __syncthreads();
+    constexpr int kCol = (kBlurH * kBlurW + kBlurThreads - 1) / kBlurThreads;
+#pragma unroll
+    for (int r = 0; r < kCol; ++r) {
+        const int e = (int)threadIdx.x + r * kBlurThreads;
         const int oy = e / kBlurW, ox = e - oy * kBlurW;
         const int gy = y0 + oy, gx = x0 + ox;
-        if (gy >= a.rows || gx >= a.cols) continue;
-        const float *p = s_row + oy * kBlurW + ox;
-        float acc = 0.f;
+        if (e < kBlurH * kBlurW && gy < a.rows && gx < a.cols) {
+            const float *p = s_row + oy * kBlurW + ox;
+            float acc = 0.f;
 #pragma unroll
-        for (int k = 0; k < NT; ++k) acc = fmaf(s_tap[k], p[k * kBlurW], acc);
-        a.dst[(size_t)gy * a.cols + gx] = (uint8_t)fminf(fmaxf(rintf(acc), 0.f), 255.f);
+            for (int k = 0; k < NT; ++k) acc = fmaf(s_tap[k], p[k * kBlurW], acc);
+            a.dst[(size_t)gy * a.cols + gx] = (uint8_t)fminf(fmaxf(rintf(acc), 0.f), 255.f);
+        }
     }
 }
 
