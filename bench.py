@@ -49,6 +49,14 @@ def step_bytes_per_particle(d: int) -> int:
     return 8 * (d + np_) + 4 + 4
 
 
+def frame_bytes_per_particle(d: int) -> int:
+    """Algorithmic HBM bytes per particle of the single-sequence step that
+    also writes its frame's resampler weights: the fused step, plus the
+    block-local scan of the fixed-point weight written (8) and read back by
+    the next frame's ancestor search (8)."""
+    return step_bytes_per_particle(d) + 16
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,7 +144,7 @@ def main():
     rb.synchronize()
     step_ms_total, frame_ms_total, launches = rb.last_kernel_time()
     step_avg_s = step_ms_total / launches / 1e3
-    bpp = step_bytes_per_particle(mdl.d)
+    bpp = frame_bytes_per_particle(mdl.d)
     achieved = bpp * N / step_avg_s / 1e9
 
     if args.calib_copy_mb:

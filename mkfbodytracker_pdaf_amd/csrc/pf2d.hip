@@ -50,8 +50,7 @@ __host__ __device__ inline void col_bounds(int c, int side, float &lo, float &hi
 //   + log N2(x[6:8]; z_0, 15 I) + log N2(x[0:2]; z_1, 15 I).
 // GMM table per component: [log(pi_k c_k), mu_k (8), Sigma_k^-1 (64)].
 __global__ void __launch_bounds__(256) k_pf2d_weight(const float *xs, int N, int ld, const float *gmm, int K,
-                                                     float z00, float z01, float z10, float z11, float *lw,
-                                                     FrameScratch *fs)
+                                                     float z00, float z01, float z10, float z11, float *lw)
 {
     constexpr int CS = 1 + kD + kD * kD;
     extern __shared__ float sg[];
@@ -92,9 +91,6 @@ __global__ void __launch_bounds__(256) k_pf2d_weight(const float *xs, int N, int
         out = lprior + 2.f * lnorm - (fmaf(u0, u0, u1 * u1) + fmaf(v0, v0, v1 * v1)) / 30.f;
         lw[j] = out;
     }
-    double v = isfinite(out) ? (double)out : -INFINITY;
-    v = wave_max(v);
-    if ((threadIdx.x & 63) == 0 && isfinite(v)) atomicMax(&fs->maxkey[0], order_key(v));
 }
 
 // Resampling gather or re-initialisation, then predict (pf2D.cpp:90-102):
@@ -227,6 +223,7 @@ struct pf2d_handle {
     unsigned long long *cloc = nullptr, *bsum = nullptr;
     double *bexp = nullptr, *lml = nullptr, *lsum = nullptr, *partial = nullptr, *out = nullptr;
     double *nbuf = nullptr, *rbuf = nullptr, *pbuf = nullptr, *wbuf = nullptr;
+    int *bpow = nullptr;
 };
 
 extern "C" {
@@ -237,7 +234,7 @@ int pf2d_destroy(pf2d_handle *h)
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->xA, h->xB, h->lw, h->table, h->anc, h->frame, h->uni, h->fs, h->cloc, h->bsum,
-                    h->bexp, h->lml, h->lsum, h->partial, h->out, h->nbuf, h->rbuf, h->pbuf, h->wbuf};
+                    h->bexp, h->bpow, h->lml, h->lsum, h->partial, h->out, h->nbuf, h->rbuf, h->pbuf, h->wbuf};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -273,6 +270,7 @@ int pf2d_create(pf2d_handle **out, int32_t n_particles, int32_t n_dims, int32_t 
     e = e ? e : dalloc2(&h->cloc, (size_t)h->N);
     e = e ? e : dalloc2(&h->bsum, (size_t)nb);
     e = e ? e : dalloc2(&h->bexp, (size_t)nb);
+    e = e ? e : dalloc2(&h->bpow, (size_t)nb);
     e = e ? e : dalloc2(&h->lml, 1);
     e = e ? e : dalloc2(&h->lsum, 1);
     e = e ? e : dalloc2(&h->partial, (size_t)((h->N + 255) / 256) * kD);
@@ -403,13 +401,14 @@ int pf2d_update(pf2d_handle *h, const double *meas, uint32_t u, const double *no
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
     const size_t lds = sizeof(float) * (size_t)h->K * (1 + kD + kD * kD);
     hipLaunchKernelGGL(k_pf2d_weight, dim3((N + 255) / 256), dim3(256), lds, h->stream, h->xA, N, h->ld, h->table,
-                       h->K, (float)meas[0], (float)meas[1], (float)meas[2], (float)meas[3], h->lw, h->fs);
+                       h->K, (float)meas[0], (float)meas[1], (float)meas[2], (float)meas[3], h->lw);
     HIP_TRY2(hipGetLastError());
     WeightArgs w{};
     w.lw32 = h->lw;
     w.cloc = h->cloc;
     w.bsum = h->bsum;
     w.bexp = h->bexp;
+    w.bpow = h->bpow;
     w.fs = h->fs;
     w.frame = h->frame;
     w.frame_next = h->frame + 1;
@@ -424,6 +423,7 @@ int pf2d_update(pf2d_handle *h, const double *meas, uint32_t u, const double *no
     an.pend.cloc = h->cloc;
     an.pend.bsum = h->bsum;
     an.pend.bexp = h->bexp;
+    an.pend.bpow = h->bpow;
     an.pend.L = N;
     an.pend.nb = nb;
     an.fs = h->fs;

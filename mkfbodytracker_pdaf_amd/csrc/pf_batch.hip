@@ -41,11 +41,13 @@ __device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc
     pd.cloc = a.cloc + sd.pofs;
     pd.bsum = a.bsum + sd.bofs;
     pd.bexp = a.bexp + sd.bofs;
+    pd.bpow = a.bpow + sd.bofs;
     pd.L = sd.N;
     pd.nb = sd.nb;
     double es = 0.0;
     const bool first = (jt0 == 0);
-    const unsigned long long total = resolve_prefix(pd, wsp, first, es);
+    int E;
+    const unsigned long long total = resolve_prefix(pd, wsp, first, es, E);
     const uint32_t U = a.Upend[s];
 #pragma unroll
     for (int r = 0; r < PPL; ++r) {
@@ -59,12 +61,10 @@ __device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc
         }
     }
     if (first && threadIdx.x == 0) {
-        unsigned long long *mk = a.maxkey + (size_t)stats_parity * a.S + s;
-        const unsigned long long key = *mk;
-        if (key == 0ull) a.status[s] |= 1u;
-        if (lml_slot) *lml_slot = (key == 0ull) ? -INFINITY : key_value(key) + log(es) - log((double)sd.N);
-        *mk = 0ull;
+        if (total == 0ull) a.status[s] |= 1u;
+        if (lml_slot) *lml_slot = (total == 0ull) ? -INFINITY : pow_ref(E) + log(es) - log((double)sd.N);
     }
+    (void)stats_parity;
 }
 
 // PERSIST: a grid of blocks_per_cu x CUs walks the active tiles; otherwise
@@ -130,7 +130,6 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
         int *comp_out = a.rec_comp ? a.rec_comp + (size_t)t * a.ld + sd.pofs : nullptr;
         const float *comp_global = seq_comp_global(a, sd);
         const uint32_t U1 = a.uni[(size_t)(sd.tofs + t) * 2];
-        float lmax = -INFINITY;
         const __amdgpu_buffer_rsrc_t wx = soa_rsrc(xout, rowb * D), wP = soa_rsrc(Pout, rowb * NP);
 #pragma unroll
         for (int r = 0; r < PPL; ++r) {
@@ -152,62 +151,26 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
                     for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[r][p]);
                 }
                 a.lw[sd.pofs + j] = lwj;
-                if (isfinite(lwj)) lmax = fmaxf(lmax, lwj);
             }
         }
-        double mv = (double)lmax;
-        mv = wave_max(mv);
-        if ((threadIdx.x & 63) == 0 && isfinite(mv))
-            atomicMax(a.maxkey + (size_t)a.parity * a.S + s, order_key(mv));
     }
 }
 
-// Fixed-point weights and block scans, per sequence (kWeightsBlock blocks).
+// Fixed-point weights and block scans, per sequence (kWeightsBlock blocks):
+// the block-scaled log-weights of the resampler (block_weights_log).
 __global__ void __launch_bounds__(kWeightsBlock) k_batch_weights(BatchArgs a)
 {
-    __shared__ unsigned long long s_wave[4];
-    __shared__ double s_dwave[4];
     const int t = *a.frame;
     const int nwb = a.n_wblocks[t];
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.frame_next = t + 1;
     for (int wb = blockIdx.x; wb < nwb; wb += gridDim.x) {
         const int s = a.wb_seq[wb], b = a.wb_b[wb];
         const SeqDesc sd = a.seqs[s];
-        const unsigned long long key = a.maxkey[(size_t)a.parity * a.S + s];
-        const bool degenerate = (key == 0ull);
-        const double ref = degenerate ? 0.0 : key_value(key);
         const int i = b * kWeightsBlock + (int)threadIdx.x;
-        unsigned long long q = 0;
-        double v = 0.0;
-        if (i < sd.N) {
-            if (degenerate) {
-                q = 1;
-            } else {
-                const double x = (double)a.lw[sd.pofs + i];
-                v = isfinite(x) ? det_exp(x - ref) : 0.0;
-                q = qlog(x, ref);
-            }
-        }
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-        unsigned long long sc = q;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned long long n = __shfl_up(sc, o, 64);
-            if (lane >= o) sc += n;
-        }
-        const double vs = wave_sum(v);
-        __syncthreads();
-        if (lane == 63) s_wave[wid] = sc;
-        if (lane == 0) s_dwave[wid] = vs;
-        __syncthreads();
-        unsigned long long off = 0;
-        for (int w = 0; w < wid; ++w) off += s_wave[w];
-        sc += off;
-        if (i < sd.N) a.cloc[sd.pofs + i] = sc;
-        if (threadIdx.x == kWeightsBlock - 1) {
-            a.bsum[sd.bofs + b] = sc;
-            a.bexp[sd.bofs + b] = (s_dwave[0] + s_dwave[1]) + (s_dwave[2] + s_dwave[3]);
-        }
+        const bool valid = i < sd.N;
+        const double x = valid ? (double)a.lw[sd.pofs + i] : -INFINITY;
+        block_weights_log(x, valid, a.cloc + sd.pofs + (valid ? i : 0), a.bsum + sd.bofs + b, a.bexp + sd.bofs + b,
+                          a.bpow + sd.bofs + b);
         if (b == 0 && threadIdx.x == 0) a.Upend[s] = a.uni[(size_t)(sd.tofs + t) * 2 + 1];
     }
 }
@@ -357,6 +320,7 @@ BatchArgs batch_args(pf_batch *b, const int32_t *count, const float *cand, const
     a.cloc = b->cloc;
     a.bsum = b->bsum;
     a.bexp = b->bexp;
+    a.bpow = b->bpow;
     a.tables = b->tables;
     a.seqs = b->seqs;
     a.tile_seq = b->tile_seq;
@@ -539,7 +503,7 @@ int pf_batch_destroy(pf_batch *b)
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     void *ptrs[] = {b->bx[0], b->bx[1], b->bP[0], b->bP[1], b->lw, b->tables, b->mus, b->sigs, b->Tis, b->Ts,
-                    b->bexp, b->partial, b->est, b->gbuf, b->anc, b->frame, b->maxkey, b->cloc, b->bsum,
+                    b->bexp, b->bpow, b->partial, b->est, b->gbuf, b->anc, b->frame, b->maxkey, b->cloc, b->bsum,
                     b->status, b->Upend, b->seqs, b->tile_seq, b->tile_j0, b->n_tiles, b->wb_seq, b->wb_b,
                     b->n_wblocks, b->fs, b->rec_comp, b->rec_anc, b->rec_bins, b->ck_x, b->ck_P, b->seg_x[0],
                     b->seg_x[1], b->seg_P[0], b->seg_P[1], b->adj_x[0], b->adj_x[1], b->adj_P[0], b->adj_P[1], b->first, b->tacc, b->gsum,
@@ -628,6 +592,7 @@ int pf_batch_create(pf_batch **out, int32_t S, const int32_t *n_particles, const
     e = e ? e : dallocb(&b->cloc, (size_t)b->ld);
     e = e ? e : dallocb(&b->bsum, (size_t)bofs);
     e = e ? e : dallocb(&b->bexp, (size_t)bofs);
+    e = e ? e : dallocb(&b->bpow, (size_t)bofs);
     e = e ? e : dallocb(&b->maxkey, 2 * (size_t)S);
     e = e ? e : dallocb(&b->status, (size_t)S);
     e = e ? e : dallocb(&b->Upend, (size_t)S);

@@ -39,6 +39,7 @@ struct BatchArgs {
     uint32_t *Upend;              // S
     unsigned long long *cloc, *bsum;
     double *bexp;
+    int *bpow;
     const float *tables;
     const SeqDesc *seqs;
     const int *tile_seq, *tile_j0, *n_tiles;   // tiles of active sequences first
@@ -137,6 +138,7 @@ struct pf_batch {
     float *bx[2] = {nullptr, nullptr}, *bP[2] = {nullptr, nullptr};
     float *lw = nullptr, *tables = nullptr, *mus = nullptr, *sigs = nullptr;
     double *Tis = nullptr, *Ts = nullptr, *bexp = nullptr, *partial = nullptr, *est = nullptr, *gbuf = nullptr;
+    int *bpow = nullptr;
     size_t gbuf_bytes = 0;
     int *anc = nullptr, *frame = nullptr;
     unsigned long long *maxkey = nullptr, *cloc = nullptr, *bsum = nullptr;

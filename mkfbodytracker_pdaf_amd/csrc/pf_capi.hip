@@ -71,10 +71,15 @@ struct pf_handle {
     int *frame = nullptr;
     uint32_t *uni = nullptr;   // 2 words for the host-driven step
     double *lml1 = nullptr;
-    // resampler scratch
+    // resampler scratch (set 0), and set 1 of the device-resident runs, whose
+    // step kernel writes frame t's weights while frame t - 1's are resolved
     unsigned long long *cloc = nullptr, *bsum = nullptr;
     double *bexp = nullptr;
+    int *bpow = nullptr;
     int Lcap = 0;
+    unsigned long long *cloc1 = nullptr, *bsum1 = nullptr;
+    double *bexp1 = nullptr;
+    int *bpow1 = nullptr;
     // staging for host-driven calls
     float *zbuf = nullptr;
     int *cntbuf = nullptr;
@@ -111,12 +116,26 @@ int ensure_resample_scratch(pf_handle *h, int L)
     (void)hipFree(h->cloc);
     (void)hipFree(h->bsum);
     (void)hipFree(h->bexp);
-    h->cloc = nullptr; h->bsum = nullptr; h->bexp = nullptr;
+    (void)hipFree(h->bpow);
+    h->cloc = nullptr; h->bsum = nullptr; h->bexp = nullptr; h->bpow = nullptr;
     const int nb = (L + kWeightsBlock - 1) / kWeightsBlock;
     HIP_TRY(dalloc(&h->cloc, (size_t)L));
     HIP_TRY(dalloc(&h->bsum, (size_t)nb));
     HIP_TRY(dalloc(&h->bexp, (size_t)nb));
+    HIP_TRY(dalloc(&h->bpow, (size_t)nb));
     h->Lcap = L;
+    return PF_OK;
+}
+
+// Set 1 of the resampler scratch, sized for the handle's N.
+int ensure_run_scratch(pf_handle *h)
+{
+    if (h->cloc1) return PF_OK;
+    const int nb = (h->N + kWeightsBlock - 1) / kWeightsBlock;
+    HIP_TRY(dalloc(&h->cloc1, (size_t)h->N));
+    HIP_TRY(dalloc(&h->bsum1, (size_t)nb));
+    HIP_TRY(dalloc(&h->bexp1, (size_t)nb));
+    HIP_TRY(dalloc(&h->bpow1, (size_t)nb));
     return PF_OK;
 }
 
@@ -188,9 +207,8 @@ int launch_step_d24(pf_handle *h, const StepArgs &a, int mode, int block)
     return set_error(PF_ERR_UNSUPPORTED, "d = 24 is compiled for 3 or 5 joints with a full-rank H");
 }
 
-int launch_step(pf_handle *h, const StepArgs &a, int mode)
+int launch_step_block(pf_handle *h, const StepArgs &a, int mode, int block)
 {
-    int block = h->cfg.block_size > 0 ? h->cfg.block_size : 256;
     switch (h->D) {
     case 8: return launch_step_d<8>(h, a, mode, block);
     case 12: return launch_step_d<12>(h, a, mode, block);
@@ -198,6 +216,11 @@ int launch_step(pf_handle *h, const StepArgs &a, int mode)
     case 24: return launch_step_d24(h, a, mode, block);
     default: return set_error(PF_ERR_UNSUPPORTED, "state dimension %d not compiled", h->D);
     }
+}
+
+int launch_step(pf_handle *h, const StepArgs &a, int mode)
+{
+    return launch_step_block(h, a, mode, h->cfg.block_size > 0 ? h->cfg.block_size : 256);
 }
 
 template <template <int> class F, typename... Args>
@@ -255,12 +278,15 @@ struct EstimateL {
     }
 };
 
-Pending pending_of(const pf_handle *h, int L)
+// The pending resampling held in scratch set `set` (0, or 1 of the
+// device-resident runs): of L log-weights (block exponents) or linear weights.
+Pending pending_of(const pf_handle *h, int L, bool log_weights = true, int set = 0)
 {
     Pending p{};
-    p.cloc = h->cloc;
-    p.bsum = h->bsum;
-    p.bexp = h->bexp;
+    p.cloc = set ? h->cloc1 : h->cloc;
+    p.bsum = set ? h->bsum1 : h->bsum;
+    p.bexp = set ? h->bexp1 : h->bexp;
+    p.bpow = log_weights ? (set ? h->bpow1 : h->bpow) : nullptr;
     p.L = L;
     p.nb = (L + kWeightsBlock - 1) / kWeightsBlock;
     return p;
@@ -276,6 +302,7 @@ WeightArgs weight_args(pf_handle *h, const uint32_t *uni, int parity)
     w.cloc = h->cloc;
     w.bsum = h->bsum;
     w.bexp = h->bexp;
+    w.bpow = h->bpow;
     w.fs = h->fs;
     w.frame = h->frame + parity;
     w.frame_next = h->frame + (1 - parity);
@@ -298,10 +325,10 @@ int launch_weights(pf_handle *h, const uint32_t *uni, int parity)
 // Materialise the pending resampling of L weights into N ancestors.  The log
 // mean weight goes to lml, or to lml[*lml_frame] when a frame counter is given.
 int launch_ancestors(pf_handle *h, int L, int N, int *out, int parity, int kind_linear, double *lml,
-                     const int *lml_frame = nullptr)
+                     const int *lml_frame = nullptr, int set = 0)
 {
     AncArgs an{};
-    an.pend = pending_of(h, L);
+    an.pend = pending_of(h, L, !kind_linear, set);
     an.fs = h->fs;
     an.anc = out;
     an.lml = lml;
@@ -447,7 +474,7 @@ int estimate(pf_handle *h)
     return check_launch("k_estimate");
 }
 
-Pending pending(pf_handle *h, int L) { return pending_of(h, L); }
+Pending pending(pf_handle *h, int L) { return pending_of(h, L, false); }
 
 FrameScratch *scratch(pf_handle *h) { return h->fs; }
 
@@ -650,8 +677,8 @@ int pf_destroy(pf_handle *h)
     (void)hipSetDevice(h->cfg.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->xA, h->PA, h->xB, h->PB, h->anc, h->comp, h->bins, h->lw, h->table,
-                    h->mu, h->sig, h->fs, h->frame, h->uni, h->lml1, h->cloc, h->bsum, h->bexp,
-                    h->zbuf, h->cntbuf, h->candbuf, h->est_partial, h->est_out, h->gbuf, h->ibuf,
+                    h->mu, h->sig, h->fs, h->frame, h->uni, h->lml1, h->cloc, h->bsum, h->bexp, h->bpow,
+                    h->cloc1, h->bsum1, h->bexp1, h->bpow1, h->zbuf, h->cntbuf, h->candbuf, h->est_partial, h->est_out, h->gbuf, h->ibuf,
                     h->Tdev, h->Tinv};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -864,15 +891,14 @@ static int resample_generic(pf_handle *h, const double *w, int L, int N, uint32_
     HIP_TRY(hipMemcpyAsync(duni, uu, sizeof(uu), hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
     const int nbm = (L + 255) / 256;
-    if (kind == kLogF64)
-        hipLaunchKernelGGL((k_blockmax64<kLogF64>), dim3(nbm), dim3(256), 0, h->stream, h->gbuf, L, h->fs);
-    else
+    if (kind == kLinF64)      // linear weights are scaled by their maximum (log-weights per block)
         hipLaunchKernelGGL((k_blockmax64<kLinF64>), dim3(nbm), dim3(256), 0, h->stream, h->gbuf, L, h->fs);
     WeightArgs wa{};
     wa.w64 = h->gbuf;
     wa.cloc = h->cloc;
     wa.bsum = h->bsum;
     wa.bexp = h->bexp;
+    wa.bpow = h->bpow;
     wa.fs = h->fs;
     wa.frame = h->frame;
     wa.frame_next = nullptr;
@@ -990,22 +1016,40 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
     }
     // Frame f reads frame counter f & 1 and writes the other one.  Frame 0
     // reads the materialised ancestor array; later frames resolve their
-    // ancestors from the previous frame's weights inside the step kernel.
+    // ancestors from the previous frame's weights inside the step kernel,
+    // which also writes its own frame's resampler weights (scratch set f & 1;
+    // frame f resolves from set (f - 1) & 1), so a frame is one launch.
     // The fused ancestor resolution needs its LDS workspace next to the model
-    // image; if both do not fit one workgroup, ancestors are materialised by
-    // a separate k_ancestors launch each frame.
+    // image; if both do not fit one workgroup, the weights and ancestors run
+    // as separate launches each frame.
     const int nbp = (h->N + kWeightsBlock - 1) / kWeightsBlock;
     const int stage = h->table_floats + (mode == kModeCand ? cand_lds_floats(h->J, h->cap) : 0);
     const bool fused = step_lds_bytes(stage, nbp) <= kMaxLdsBytes;
+    if (fused) {
+        st = ensure_run_scratch(h);
+        if (st) return st;
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, &h->fs->pow_max[0], kPowEmpty);
+        st = check_launch("k_set_int");
+        if (st) return st;
+    }
     auto one_frame = [&](int f) -> int {
         const int parity = f & 1;
         const int amode = (f == 0 || !fused) ? kAncArray : kAncFused;
         StepArgs a = step_args(h, mode, uniforms, Z, zstride, h->N, count, cand, nullptr, parity, amode, lml);
+        if (fused) {
+            a.pend = pending_of(h, h->N, true, parity ^ 1);
+            a.w_cloc = parity ? h->cloc1 : h->cloc;
+            a.w_bsum = parity ? h->bsum1 : h->bsum;
+            a.w_bexp = parity ? h->bexp1 : h->bexp;
+            a.w_bpow = parity ? h->bpow1 : h->bpow;
+            a.frame_next = h->frame + (1 - parity);
+        }
         if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f], h->stream));
-        int s = launch_step(h, a, mode);
+        // the fused step's blocks are the resampler's blocks
+        int s = fused ? launch_step_block(h, a, mode, kWeightsBlock) : launch_step(h, a, mode);
         if (s) return s;
         if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f + 1], h->stream));
-        s = launch_weights(h, uniforms, parity);
+        if (!fused) s = launch_weights(h, uniforms, parity);
         if (s) return s;
         if (!fused && f + 1 < T) {
             // counter f & 1 still holds this frame's index (k_weights wrote
@@ -1049,7 +1093,7 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
         }
     }
     // Materialise the last frame's ancestors and its log mean weight.
-    st = launch_ancestors(h, h->N, h->N, h->anc, (T - 1) & 1, 0, lml + (T - 1));
+    st = launch_ancestors(h, h->N, h->N, h->anc, (T - 1) & 1, 0, lml + (T - 1), nullptr, fused ? (T - 1) & 1 : 0);
     if (st) return st;
     if (timing) {
         HIP_TRY(hipEventSynchronize(h->ev[2 * T + 1]));

@@ -76,6 +76,41 @@ __host__ __device__ inline uint64_t qlog(double lw, double lmax)
     return (isfinite(lw) && isfinite(lmax)) ? fixed32(det_exp(lw - lmax)) : 0;
 }
 
+// Block-scaled log-weights (the resampler of log-weights).  A block of
+// kWeightsBlock consecutive log-weights whose largest finite value is m is
+// scaled by 2^-e with e = block_pow(m) = ceil(m log2 e), so that
+// q_i = floor(2^32 exp(lw_i - e ln 2)) <= 2^32 and the block's largest q is
+// at least 2^31; a block without a finite log-weight has e = kPowEmpty and
+// q = 0.  Blocks are aligned by right shifts of E - e, E the largest e, so
+// every prefix sum stays an exact integer and no pass over the weights needs
+// the global maximum first.
+constexpr int kPowEmpty = -2147483647 - 1;
+constexpr int kPowLimit = 1 << 30;
+
+__host__ __device__ inline int block_pow(double m)
+{
+    const double c = ceil(m * 1.4426950408889634);
+    return c > (double)kPowLimit ? kPowLimit : (c < -(double)kPowLimit ? -kPowLimit : (int)c);
+}
+
+__host__ __device__ inline double pow_ref(int e) { return (double)e * 0.6931471805599453; }
+
+__host__ __device__ inline uint64_t qlog_pow(double lw, int e)
+{
+    return (isfinite(lw) && e != kPowEmpty) ? fixed32(det_exp(lw - pow_ref(e))) : 0;
+}
+
+// Right shift aligning a block of exponent e to the largest exponent E (64:
+// the block contributes nothing).
+__host__ __device__ inline int pow_shift(int E, int e)
+{
+    if (e == kPowEmpty || E == kPowEmpty) return 64;
+    const long long d = (long long)E - (long long)e;
+    return d >= 64 ? 64 : (int)d;
+}
+
+__host__ __device__ inline uint64_t shr64(uint64_t x, int s) { return s >= 64 ? 0 : x >> s; }
+
 // Systematic threshold of output i: (i * 2^32 + U) * total.
 __host__ __device__ inline u128 sys_target(uint32_t i, uint32_t U, uint64_t total)
 {

@@ -42,6 +42,9 @@ struct FrameScratch {
     unsigned int status;           // bit 0: a degenerate (all-zero) resampling happened
     uint32_t U_pending;            // uniform of the pending resampling
     uint32_t pad;
+    int pow_max[3];                // frame t mod 3: largest block exponent of frame t's log-weights
+                                   // (raised by the step kernels that write their own weights)
+    int pad2;
 };
 
 // The pending systematic resampling of one weight vector of length L into N
@@ -50,6 +53,8 @@ struct Pending {
     const unsigned long long *__restrict__ cloc;
     const unsigned long long *__restrict__ bsum;
     const double *__restrict__ bexp;
+    const int *__restrict__ bpow = nullptr;      // block exponents of block-scaled log-weights (null: none)
+    const int *pow_max = nullptr;                // their maximum, when its producer recorded it (else reduced)
     int L, nb;
     int bshift = 8;                              // log2 of the inputs per block (kWeightsBlock)
     int stage = kStageBlocks * kWeightsBlock;    // LDS elements of the staged search
@@ -87,6 +92,13 @@ struct StepArgs {
     float pdl;    // P_D / lambda
     float b0;     // 1 - P_D * P_G
     float gate;   // chi-square gate
+    // Optional: the frame's resampler weights written by the step itself
+    // (blocks of kWeightsBlock threads = resampler blocks); block 0 also
+    // records the pending uniform uni[2 t + 1] and writes t + 1 to frame_next.
+    unsigned long long *w_cloc, *w_bsum;
+    double *w_bexp;
+    int *w_bpow;
+    int *frame_next;
 };
 
 template <typename T>
@@ -98,43 +110,75 @@ __device__ __forceinline__ T wave_sum(T v)
 }
 
 // ------------------------------------------------------------- resampling
-// Systematic resampling (pf2DRao.cpp:175-210) with fixed-point weights:
-//   q_i = floor(2^32 * w_i / max w) (or det_exp(lw_i - max lw)),
-//   C = inclusive prefix sum (exact, order independent),
+// Systematic resampling (pf2DRao.cpp:175-210) with fixed-point weights q_i:
+//   linear weights: q_i = floor(2^32 * w_i / max w) (or a known bound),
+//   log-weights: block-scaled, q_i = floor(2^32 exp(lw_i - e_b ln 2)) with the
+//     block exponent e_b of pf_common.hpp, aligned by >> (E - e_b),
+//   C = inclusive prefix sum of the aligned values (exact integers),
 //   a_i = min{idx : C[idx] * N * 2^32 > (i * 2^32 + U) * C[L-1]}.
-// The prefix sum is held as block-local scans (cloc) plus block sums (bsum);
-// resolve_ancestor() finds a_i for one block of outputs.
+// The prefix sum is held as block-local scans (cloc) plus block sums (bsum)
+// and exponents (bpow): C at input k of block b is pre_b + (cloc_k >> s_b),
+// pre_b the sum of the aligned block sums before b.  resolve_ancestor()
+// finds a_i for one block of outputs; C[L-1] = 0 (no positive weight) gives
+// the uniform resampling.
 
 // LDS workspace of resolve_ancestor: nb + 1 prefix words, the staged input
-// elements (Pending::stage), wave totals and a few broadcast slots.
+// elements (Pending::stage), 16 broadcast / wave slots, nb block shifts.
 __host__ __device__ constexpr int resolve_lds_words(int nb, int stage = kStageBlocks * kWeightsBlock)
 {
-    return nb + 1 + stage + 8;
+    return nb + 1 + stage + 16 + (nb + 1) / 2;
 }
 
-// Phase 1 of the ancestor resolution: exclusive prefix of the block sums in
-// LDS (pre[0..nb], pre[nb] = total) and, if want_es, the FP64 sum of the
-// block weight sums in es.  Every thread of the block must call it (want_es
-// block-uniform).
+// Phase 1 of the ancestor resolution: the largest block exponent E, the
+// exclusive prefix of the aligned block sums in LDS (pre[0..nb], pre[nb] =
+// total) with each block's shift, and, if want_es, the FP64 sum of the block
+// weight sums scaled to 2^E in es.  Every thread of the block must call it
+// (want_es block-uniform).
 __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, unsigned long long *w, bool want_es,
-                                                             double &es_out)
+                                                             double &es_out, int &pow_out)
 {
     const int tid = threadIdx.x, nt = blockDim.x, nb = pd.nb;
     const int lane = tid & 63, wid = tid >> 6, nwaves = (nt + 63) >> 6;
     unsigned long long *pre = w;
     unsigned long long *stg = w + nb + 1;
     unsigned long long *misc = stg + pd.stage;
+    int *shf = reinterpret_cast<int *>(misc + 16);
     const int per = (nb + nt - 1) / nt;
     const int c0 = min(nb, tid * per), c1 = min(nb, c0 + per);
-    // loads in groups of 8 so that their latencies overlap
-    unsigned long long loc = 0;
-    for (int k = c0; k < c1; k += 8) {
-        unsigned long long t8[8];
+    // A thread's blocks c0 .. c1 - 1; up to 8 of them stay in registers from
+    // one round of loads (more are loaded again in each pass).
+    constexpr int kReg = 8;
+    unsigned long long rs[kReg];
+    int re[kReg];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) t8[u] = (k + u < c1) ? pd.bsum[k + u] : 0ull;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) loc += t8[u];
+    for (int u = 0; u < kReg; ++u) {
+        rs[u] = (c0 + u < c1) ? pd.bsum[c0 + u] : 0ull;
+        re[u] = (pd.bpow && c0 + u < c1) ? pd.bpow[c0 + u] : kPowEmpty;
     }
+    // 0. E, the largest block exponent (0 without exponents)
+    int E = 0;
+    if (pd.bpow && pd.pow_max) {
+        E = *pd.pow_max;
+    } else if (pd.bpow) {
+        int em = kPowEmpty;
+#pragma unroll
+        for (int u = 0; u < kReg; ++u) em = max(em, re[u]);
+        for (int k = c0 + kReg; k < c1; ++k) em = max(em, pd.bpow[k]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) em = max(em, __shfl_xor(em, o, 64));
+        __syncthreads();                 // LDS of a previous use is free
+        if (lane == 0) misc[8 + wid] = (unsigned long long)(long long)em;
+        __syncthreads();
+        E = kPowEmpty;
+        for (int k = 0; k < nwaves; ++k) E = max(E, (int)(long long)misc[8 + k]);
+    }
+    auto shift_of = [&](int e) { return pd.bpow ? pow_shift(E, e) : 0; };
+    // 1. the aligned block sums of this thread's blocks
+    unsigned long long loc = 0;
+#pragma unroll
+    for (int u = 0; u < kReg; ++u)
+        if (c0 + u < c1) loc += shr64(rs[u], shift_of(re[u]));
+    for (int k = c0 + kReg; k < c1; ++k) loc += shr64(pd.bsum[k], shift_of(pd.bpow ? pd.bpow[k] : 0));
     unsigned long long sc = loc;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -142,16 +186,24 @@ __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, 
         if (lane >= o) sc += n;
     }
     // FP64 weight sum in an order fixed by nb alone (lane l of wave 0 takes
-    // blocks l, l + 64, ...), so it does not depend on the block size.
+    // blocks l, l + 64, ...), so it does not depend on the block size; block
+    // b's sum is relative to 2^e_b and scaled to 2^E (exactly).
     double es = 0.0;
     if (want_es && wid == 0) {
         for (int k = lane; k < nb; k += 8 * 64) {       // same order, 8 loads in flight
             double t8[8];
+            int e8[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) t8[u] = (k + 64 * u < nb) ? pd.bexp[k + 64 * u] : 0.0;
+            for (int u = 0; u < 8; ++u) {
+                t8[u] = (k + 64 * u < nb) ? pd.bexp[k + 64 * u] : 0.0;
+                e8[u] = (pd.bpow && k + 64 * u < nb) ? pd.bpow[k + 64 * u] : 0;
+            }
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                if (k + 64 * u < nb) es += t8[u];
+                if (k + 64 * u < nb) {
+                    if (!pd.bpow) es += t8[u];
+                    else if (e8[u] != kPowEmpty) es += ldexp(t8[u], e8[u] - E);
+                }
         }
         es = wave_sum(es);
     }
@@ -164,19 +216,23 @@ __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, 
     unsigned long long total = 0;
     for (int k = 0; k < nwaves; ++k) total += misc[k];
     if (want_es) es_out = reinterpret_cast<double *>(stg)[0];
-    for (int k = c0; k < c1; k += 8) {
-        unsigned long long t8[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) t8[u] = (k + u < c1) ? pd.bsum[k + u] : 0ull;
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (k + u < c1) {
-                pre[k + u] = run;
-                run += t8[u];
-            }
+    for (int u = 0; u < kReg; ++u)
+        if (c0 + u < c1) {
+            const int sh = shift_of(re[u]);
+            pre[c0 + u] = run;
+            shf[c0 + u] = sh;
+            run += shr64(rs[u], sh);
+        }
+    for (int k = c0 + kReg; k < c1; ++k) {
+        const int sh = shift_of(pd.bpow ? pd.bpow[k] : 0);
+        pre[k] = run;
+        shf[k] = sh;
+        run += shr64(pd.bsum[k], sh);
     }
     if (tid == 0) pre[nb] = total;
     __syncthreads();
+    pow_out = E;
     return total;
 }
 
@@ -189,6 +245,7 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
     unsigned long long *pre = w;
     unsigned long long *stg = w + nb + 1;
     unsigned long long *misc = stg + pd.stage;
+    const int *shf = reinterpret_cast<const int *>(misc + 16);
     const int bs = 1 << pd.bshift;
     // 2. the input block of each output: largest b with pre[b] * S <= t_j
     const int j = j0 + tid;
@@ -215,9 +272,9 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
     __syncthreads();
     const int blo = (int)misc[4], bhi = (int)misc[5];
     // 3. a span of input blocks that fits the stage (pd.stage elements) is
-    //    staged in LDS (C = pre + cloc, one memory latency) and searched
-    //    there; wider spans (strongly concentrated weights) search the input
-    //    block in global memory, one dependent probe per level.
+    //    staged in LDS (C = pre + aligned cloc, one memory latency) and
+    //    searched there; wider spans (strongly concentrated weights) search
+    //    the input block in global memory, one dependent probe per level.
     int a;
     if (((bhi - blo + 1) << pd.bshift) <= pd.stage) {
         const int base = blo << pd.bshift;
@@ -229,7 +286,10 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int k = k0 + u * nt;
-                if (k < n) stg[k] = pre[blo + (k >> pd.bshift)] + c4[u];
+                if (k < n) {
+                    const int b = blo + (k >> pd.bshift);
+                    stg[k] = pre[b] + (pd.bpow ? shr64(c4[u], shf[b]) : c4[u]);
+                }
             }
         }
         __syncthreads();
@@ -242,10 +302,11 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
         a = base + l;
     } else {
         const unsigned long long pb = pre[bj];
+        const int sh = pd.bpow ? shf[bj] : 0;
         int l = bj * bs, h = min(pd.L, (bj + 1) * bs) - 1;
         while (l < h) {
             const int mid = (l + h) >> 1;
-            if ((u128)(pb + pd.cloc[mid]) * S > tj) h = mid;
+            if ((u128)(pb + shr64(pd.cloc[mid], sh)) * S > tj) h = mid;
             else l = mid + 1;
         }
         a = l;
@@ -253,29 +314,82 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
     return a;
 }
 
+// The frame statistics of a resolved resampling: the largest block exponent
+// and the total of the aligned weights (0: every weight zero).
+struct ResolveStats {
+    unsigned long long total;
+    int pow;
+};
+
 // Both phases for one block of outputs.
 __device__ __forceinline__ int resolve_ancestor(const Pending &pd, int N, uint32_t U, int j0, unsigned long long *w,
-                                                bool want_es, double &es_out)
+                                                bool want_es, double &es_out, ResolveStats &st)
 {
-    const unsigned long long total = resolve_prefix(pd, w, want_es, es_out);
-    return resolve_search(pd, N, U, total, j0, w);
+    st.total = resolve_prefix(pd, w, want_es, es_out, st.pow);
+    return resolve_search(pd, N, U, st.total, j0, w);
 }
 
 // Frame statistics of a consumed resampling (block 0, thread 0): the log
-// mean weight, the degenerate flag, and the reset of the maximum slot.
-__device__ __forceinline__ void finish_resampling(FrameScratch *fs, int parity, double bexp_sum, int N,
-                                                  int kind_linear, double *lml_slot)
+// mean weight, the degenerate flag (no positive weight), and the reset of the
+// maximum slot (linear weights).  bexp_sum is the block weight sums' total,
+// relative to 2^E for log-weights.
+__device__ __forceinline__ void finish_resampling(FrameScratch *fs, int parity, double bexp_sum, const ResolveStats &st,
+                                                  int N, int kind_linear, double *lml_slot)
 {
-    const unsigned long long key = fs->maxkey[parity];
-    if (key == 0ull) fs->status |= 1u;
+    if (st.total == 0ull) fs->status |= 1u;
     if (lml_slot) {
         double l;
-        if (key == 0ull) l = -INFINITY;
+        if (st.total == 0ull) l = -INFINITY;
         else if (kind_linear) l = log(bexp_sum / (double)N);
-        else l = key_value(key) + log(bexp_sum) - log((double)N);
+        else l = pow_ref(st.pow) + log(bexp_sum) - log((double)N);
         *lml_slot = l;
     }
     fs->maxkey[parity] = 0ull;
+}
+
+// Block-scaled fixed-point weights of one resampler block (kWeightsBlock
+// threads, log-weight x of thread i; invalid threads and non-finite x count
+// as empty): the block exponent e of its largest finite x (pf_common.hpp),
+// q = floor(2^32 exp(x - e ln 2)), the block-local inclusive scan of q to
+// *cloc (valid threads), and from the last thread the block sum, the FP64
+// sum of exp(x - e ln 2) and e; returns e.  Every thread of the block calls it.
+__device__ __forceinline__ int block_weights_log(double x, bool valid, unsigned long long *cloc,
+                                                 unsigned long long *bsum, double *bexp, int *bpow)
+{
+    __shared__ double s_m[kWeightsBlock / 64];
+    __shared__ unsigned long long s_wave[kWeightsBlock / 64];
+    __shared__ double s_dwave[kWeightsBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const bool fin = valid && isfinite(x);
+    double m = wave_max(fin ? x : -INFINITY);
+    if (lane == 0) s_m[wid] = m;
+    __syncthreads();
+    double bm = s_m[0];
+#pragma unroll
+    for (int w = 1; w < kWeightsBlock / 64; ++w) bm = fmax(bm, s_m[w]);
+    const int e = isfinite(bm) ? block_pow(bm) : kPowEmpty;
+    const double v = (fin && e != kPowEmpty) ? det_exp(x - pow_ref(e)) : 0.0;
+    const unsigned long long q = fixed32(v);
+    unsigned long long sc = q;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long n = __shfl_up(sc, o, 64);
+        if (lane >= o) sc += n;
+    }
+    const double vs = wave_sum(v);
+    if (lane == 63) s_wave[wid] = sc;
+    if (lane == 0) s_dwave[wid] = vs;
+    __syncthreads();
+    unsigned long long off = 0;
+    for (int w = 0; w < wid; ++w) off += s_wave[w];
+    sc += off;
+    if (valid) *cloc = sc;
+    if (threadIdx.x == kWeightsBlock - 1) {
+        *bsum = sc;
+        *bexp = (s_dwave[0] + s_dwave[1]) + (s_dwave[2] + s_dwave[3]);
+        *bpow = e;
+    }
+    return e;
 }
 
 // --------------------------------------------------------------- step kernel
@@ -679,10 +793,15 @@ __device__ __forceinline__ void step_body(const StepArgs &a)
         unsigned long long *wsp = reinterpret_cast<unsigned long long *>(
             reinterpret_cast<float *>(sm4) + a.stage_floats);
         double es = 0.0;
-        src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0, es);
+        ResolveStats rs;
+        // the previous frame's resampling uniform (its step recorded it as
+        // pending too) and its largest block exponent (raised by its blocks)
+        Pending pd = a.pend;
+        if (a.w_cloc) pd.pow_max = &a.fs->pow_max[(t - 1) % 3];
+        src = resolve_ancestor(pd, a.N, a.uni[2 * (t - 1) + 1], j0, wsp, blockIdx.x == 0, es, rs);
         if (live) a.anc[j] = src;
         if (blockIdx.x == 0 && threadIdx.x == 0)
-            finish_resampling(a.fs, a.parity ^ 1, es, a.N, 0, a.lml ? a.lml + (t - 1) : nullptr);
+            finish_resampling(a.fs, a.parity ^ 1, es, rs, a.N, 0, a.lml ? a.lml + (t - 1) : nullptr);
     } else {
         src = live ? a.anc[j] : 0;
     }
@@ -718,17 +837,18 @@ __device__ __forceinline__ void step_body(const StepArgs &a)
         for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[p]);
         a.lw[j] = lwj;
     }
-    // block maximum of the finite log-weights -> one atomic per block
-    double v = isfinite(lwj) ? (double)lwj : -INFINITY;
-    v = wave_max(v);
-    __shared__ double wmax[4];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) wmax[wid] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = wmax[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, wmax[w]);
-        if (isfinite(m)) atomicMax(&a.fs->maxkey[a.parity], order_key(m));
+    if (a.w_cloc) {
+        // this frame's resampler weights: the block is one resampler block;
+        // its exponent raises slot t mod 3, and block 0 clears slot t + 1
+        // (last read by frame t - 1's resolution, written next by frame t + 1)
+        const int e = block_weights_log((double)lwj, live, a.w_cloc + (live ? j : 0), a.w_bsum + blockIdx.x,
+                                        a.w_bexp + blockIdx.x, a.w_bpow + blockIdx.x);
+        if (threadIdx.x == 0 && e != kPowEmpty) atomicMax(&a.fs->pow_max[t % 3], e);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.fs->U_pending = a.uni[2 * t + 1];
+            a.fs->pow_max[(t + 1) % 3] = kPowEmpty;
+            *a.frame_next = t + 1;
+        }
     }
 }
 
@@ -755,6 +875,7 @@ struct WeightArgs {
     unsigned long long *cloc;     // L block-local inclusive prefix sums
     unsigned long long *bsum;     // nb block sums
     double *bexp;                 // nb block sums of the FP64 weights
+    int *bpow;                    // nb block exponents (log-weights)
     FrameScratch *fs;
     const int *frame;             // frame counter read this frame
     int *frame_next;              // frame counter of the next frame (block 0 writes t + 1)
@@ -765,46 +886,46 @@ struct WeightArgs {
 template <int KIND>
 __device__ __forceinline__ void weights_body(const WeightArgs &a)
 {
-    __shared__ unsigned long long s_wave[4];
-    __shared__ double s_dwave[4];
-    const unsigned long long key = a.fs->maxkey[a.parity];
-    const bool degenerate = (key == 0ull);
-    const double ref = degenerate ? 0.0 : key_value(key);
     const int i = blockIdx.x * kWeightsBlock + threadIdx.x;
-    unsigned long long q = 0;
-    double v = 0.0;
-    if (i < a.L) {
-        if (degenerate) {
-            q = 1;
-        } else if (KIND == kLogF32 || KIND == kLogF64) {
-            const double x = (KIND == kLogF32) ? (double)a.lw32[i] : a.w64[i];
-            v = isfinite(x) ? det_exp(x - ref) : 0.0;
-            q = qlog(x, ref);
-        } else {
+    if (KIND == kLogF32 || KIND == kLogF64) {
+        const bool valid = i < a.L;
+        const double x = !valid ? -INFINITY : (KIND == kLogF32 ? (double)a.lw32[i] : a.w64[i]);
+        block_weights_log(x, valid, a.cloc + (valid ? i : 0), a.bsum + blockIdx.x, a.bexp + blockIdx.x,
+                          a.bpow + blockIdx.x);
+    } else {
+        // linear weights relative to their maximum (slot `parity`)
+        __shared__ unsigned long long s_wave[4];
+        __shared__ double s_dwave[4];
+        const unsigned long long key = a.fs->maxkey[a.parity];
+        const bool degenerate = (key == 0ull);
+        const double ref = degenerate ? 0.0 : key_value(key);
+        unsigned long long q = 0;
+        double v = 0.0;
+        if (i < a.L) {
             const double x = a.w64[i];
-            const bool ok = isfinite(x) && x > 0.0;
+            const bool ok = !degenerate && isfinite(x) && x > 0.0;
             v = ok ? x : 0.0;
             q = ok ? fixed32(x / ref) : 0;
         }
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    unsigned long long s = q;
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        unsigned long long s = q;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long n = __shfl_up(s, o, 64);
-        if (lane >= o) s += n;
-    }
-    const double vs = wave_sum(v);
-    if (lane == 63) s_wave[wid] = s;
-    if (lane == 0) s_dwave[wid] = vs;
-    __syncthreads();
-    unsigned long long off = 0;
-    for (int w = 0; w < wid; ++w) off += s_wave[w];
-    s += off;
-    if (i < a.L) a.cloc[i] = s;
-    if (threadIdx.x == kWeightsBlock - 1) {
-        a.bsum[blockIdx.x] = s;
-        a.bexp[blockIdx.x] = (s_dwave[0] + s_dwave[1]) + (s_dwave[2] + s_dwave[3]);
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned long long n = __shfl_up(s, o, 64);
+            if (lane >= o) s += n;
+        }
+        const double vs = wave_sum(v);
+        if (lane == 63) s_wave[wid] = s;
+        if (lane == 0) s_dwave[wid] = vs;
+        __syncthreads();
+        unsigned long long off = 0;
+        for (int w = 0; w < wid; ++w) off += s_wave[w];
+        s += off;
+        if (i < a.L) a.cloc[i] = s;
+        if (threadIdx.x == kWeightsBlock - 1) {
+            a.bsum[blockIdx.x] = s;
+            a.bexp[blockIdx.x] = (s_dwave[0] + s_dwave[1]) + (s_dwave[2] + s_dwave[3]);
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const int t = *a.frame;
@@ -842,10 +963,11 @@ static __global__ void __launch_bounds__(256) k_ancestors(AncArgs a)
     const int j0 = blockIdx.x * blockDim.x;
     const int j = j0 + threadIdx.x;
     double es = 0.0;
-    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0 && a.lml, es);
+    ResolveStats rs;
+    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0 && a.lml, es, rs);
     if (j < a.N) a.anc[j] = src;
     if (blockIdx.x == 0 && threadIdx.x == 0)
-        finish_resampling(a.fs, a.parity, es, a.N, a.kind_linear,
+        finish_resampling(a.fs, a.parity, es, rs, a.N, a.kind_linear,
                           (a.lml && a.lml_frame) ? a.lml + *a.lml_frame : a.lml);
 }
 
@@ -1115,10 +1237,11 @@ __device__ __forceinline__ void ancestors_estimate_body(const AncArgs &a, const 
     const int j0 = blockIdx.x * blockDim.x;
     const int j = j0 + threadIdx.x;
     double es = 0.0;
-    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0 && a.lml, es);
+    ResolveStats rs;
+    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0 && a.lml, es, rs);
     if (j < a.N) a.anc[j] = src;
     if (blockIdx.x == 0 && threadIdx.x == 0)
-        finish_resampling(a.fs, a.parity, es, a.N, a.kind_linear,
+        finish_resampling(a.fs, a.parity, es, rs, a.N, a.kind_linear,
                           (a.lml && a.lml_frame) ? a.lml + *a.lml_frame : a.lml);
     double v[D];
 #pragma unroll

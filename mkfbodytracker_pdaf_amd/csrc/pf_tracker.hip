@@ -376,7 +376,8 @@ __device__ __forceinline__ void prop_select_body(const PropParams &p, int arm, c
     const bool uniform = rec->uniform != 0;
     const double y0 = rec->y[2 * arm], y1 = rec->y[2 * arm + 1];
     double es = 0.0;
-    const unsigned long long total = resolve_prefix(sa.pend, wsp, false, es);
+    int pw;
+    const unsigned long long total = resolve_prefix(sa.pend, wsp, false, es, pw);
     const int c = resolve_search(sa.pend, N, sa.fs->U_pending, total, j0, wsp);
     if (blockIdx.x == 0 && threadIdx.x == 0 && total == 0ull) sa.fs->status |= 1u;
     if (j >= N) return;

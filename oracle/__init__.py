@@ -65,6 +65,8 @@ def lib():
         L.orc_resample_linear.argtypes = [_dp, C.c_int, C.c_int, C.c_uint32, _ip]
         L.orc_resample_log.argtypes = [_dp, C.c_int, C.c_int, C.c_uint32, _ip]
         L.orc_resample_scaled.argtypes = [_dp, C.c_int, C.c_int, C.c_uint32, C.c_double, _ip]
+        L.orc_block_pow.argtypes = [C.c_double]
+        L.orc_block_pow.restype = C.c_int
         L.orc_resample_wheel.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_void_p, _ip]
         L.orc_kf_predict.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_kf_update.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -148,6 +150,12 @@ def resample_scaled(w, N: int, U: int, scale: float):
     out = np.zeros(N, np.int32)
     st = lib().orc_resample_scaled(w, w.size, N, int(U), float(scale), out)
     return out, st
+
+
+def block_pow(m: float) -> int:
+    """Exponent of a block of log-weights whose largest finite value is m:
+    ceil(m log2 e), clamped to +-2^30."""
+    return int(lib().orc_block_pow(float(m)))
 
 
 def resample_log(lw, N: int, U: int):

@@ -148,18 +148,71 @@ int orc_resample_scaled(const double *w, int L, int N, uint32_t U, double scale,
     return st;
 }
 
+/* Block-scaled log-weights (blocks of ORC_WBLOCK consecutive inputs): the
+ * largest finite lw of block b gives e_b = ceil(m_b log2 e) (clamped to
+ * +-2^30), q_i = fixed32(det_exp(lw_i - e_b ln 2)); with E the largest e_b,
+ * the cumulative weight at input k of block b is
+ *   C_k = sum_{b' < b} (Q_b' >> s_b') + (cloc_k >> s_b),  s_b = E - e_b,
+ * Q the block sums, cloc the block-local inclusive prefix of q, a shift of 64
+ * or more giving 0 (empty blocks: no finite lw).  No positive total: the
+ * uniform resampling of q_i = 1. */
+#define ORC_WBLOCK 256
+#define ORC_POW_EMPTY (-2147483647 - 1)
+#define ORC_POW_LIMIT (1 << 30)
+
+int orc_block_pow(double m)
+{
+    const double c = ceil(m * 1.4426950408889634);
+    return c > (double)ORC_POW_LIMIT ? ORC_POW_LIMIT : (c < -(double)ORC_POW_LIMIT ? -ORC_POW_LIMIT : (int)c);
+}
+
+static double pow_ref(int e) { return (double)e * 0.6931471805599453; }
+
+static uint64_t shr64(uint64_t x, int s) { return s >= 64 ? 0 : x >> s; }
+
 int orc_resample_log(const double *lw, int L, int N, uint32_t U, int *out)
 {
     if (L <= 0 || N <= 0 || L > (1 << 28) || N > (1 << 28)) return -1;
-    uint64_t *q = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)L);
-    double lmax = -INFINITY;
-    for (int i = 0; i < L; ++i)
-        if (isfinite(lw[i]) && lw[i] > lmax) lmax = lw[i];
-    for (int i = 0; i < L; ++i)
-        q[i] = (isfinite(lmax) && isfinite(lw[i])) ? fixed32(orc_det_exp(lw[i] - lmax)) : 0;
-    int st = resample_fixed(q, L, N, U, out);
-    free(q);
-    return st;
+    const int nb = (L + ORC_WBLOCK - 1) / ORC_WBLOCK;
+    uint64_t *C = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)L);
+    int *e = (int *)malloc(sizeof(int) * (size_t)nb);
+    int E = ORC_POW_EMPTY;
+    for (int b = 0; b < nb; ++b) {
+        const int i0 = b * ORC_WBLOCK, i1 = i0 + ORC_WBLOCK < L ? i0 + ORC_WBLOCK : L;
+        double m = -INFINITY;
+        for (int i = i0; i < i1; ++i)
+            if (isfinite(lw[i]) && lw[i] > m) m = lw[i];
+        e[b] = isfinite(m) ? orc_block_pow(m) : ORC_POW_EMPTY;
+        if (e[b] != ORC_POW_EMPTY && (E == ORC_POW_EMPTY || e[b] > E)) E = e[b];
+    }
+    uint64_t pre = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int i0 = b * ORC_WBLOCK, i1 = i0 + ORC_WBLOCK < L ? i0 + ORC_WBLOCK : L;
+        const int s = (e[b] == ORC_POW_EMPTY) ? 64 : (((long long)E - e[b]) >= 64 ? 64 : E - e[b]);
+        uint64_t loc = 0;
+        for (int i = i0; i < i1; ++i) {
+            if (e[b] != ORC_POW_EMPTY && isfinite(lw[i])) loc += fixed32(orc_det_exp(lw[i] - pow_ref(e[b])));
+            C[i] = pre + shr64(loc, s);
+        }
+        pre += shr64(loc, s);
+    }
+    const uint64_t T = pre;
+    int status = 0;
+    if (T == 0) {                               /* Q3: degenerate -> uniform */
+        for (int i = 0; i < L; ++i) C[i] = (uint64_t)i + 1;
+        status = 1;
+    }
+    const uint64_t Tt = T == 0 ? (uint64_t)L : T;
+    const u128 scale = (u128)(uint64_t)N << 32;
+    int idx = 0;
+    for (int i = 0; i < N; ++i) {
+        const u128 t = (u128)((((uint64_t)i) << 32) | U) * Tt;
+        while ((u128)C[idx] * scale <= t) ++idx;
+        out[i] = idx;
+    }
+    free(C);
+    free(e);
+    return status;
 }
 
 /* pf2DRao.cpp:175-210 as written, with the RNG draws replaced by u / fallback. */

@@ -49,6 +49,7 @@ double orc_det_exp(double x);
  * positive every q_i = 1 (uniform fallback, Q3) and the return value is 1.
  * Returns 0 on success, 1 on the degenerate fallback, -1 on bad arguments. */
 int orc_resample_linear(const double *w, int L, int N, uint32_t U, int *out);
+int orc_block_pow(double m);
 int orc_resample_scaled(const double *w, int L, int N, uint32_t U, double scale, int *out);
 int orc_resample_log(const double *lw, int L, int N, uint32_t U, int *out);
 /* Literal FP64 wheel of pf2DRao.cpp:193-208 with beta0 = u/N (quirk mode Q4):

@@ -79,6 +79,67 @@ def test_resample_degenerate_and_u_extremes():
     assert list(a0) == [1, 1, 3, 3] and list(a1) == [1, 1, 3, 3]
 
 
+def _block_scaled_brute(lw, N, U):
+    """The block-scaled resampling of log-weights restated with Python
+    integers: blocks of 256, e_b = ceil(m_b log2 e), q_i = floor(2^32
+    det_exp(lw_i - e_b ln 2)), C_k = sum of the earlier blocks' sums >> (E - e_b)
+    plus the block-local prefix >> (E - e_b), a_j by bisection."""
+    import bisect
+    L = len(lw)
+    blocks = [list(range(b, min(b + 256, L))) for b in range(0, L, 256)]
+    es = []
+    for ids in blocks:
+        fin = [lw[i] for i in ids if math.isfinite(lw[i])]
+        es.append(None if not fin else max(-2 ** 30, min(2 ** 30, math.ceil(max(fin) * 1.4426950408889634))))
+    E = max((e for e in es if e is not None), default=None)
+    C, pre = [], 0
+    for ids, e in zip(blocks, es):
+        s = 64 if (e is None or E - e >= 64) else E - e
+        loc = 0
+        for i in ids:
+            if e is not None and math.isfinite(lw[i]):
+                v = oracle.det_exp(lw[i] - e * 0.6931471805599453)
+                loc += min(2 ** 32, math.floor(math.ldexp(v, 32))) if v > 0 else 0
+            C.append(pre + (loc >> s if s < 64 else 0))
+        pre += loc >> s if s < 64 else 0
+    T = pre
+    if T == 0:
+        C, T = list(range(1, L + 1)), L
+    scaled = [c * N * 2 ** 32 for c in C]
+    return np.array([bisect.bisect_right(scaled, (j * 2 ** 32 + U) * T) for j in range(N)]), int(pre == 0)
+
+
+def test_resample_log_block_scaled_exact():
+    rng = np.random.default_rng(5)
+    for trial in range(12):
+        L, N = int(rng.integers(1, 1800)), int(rng.integers(1, 700))
+        nb = (L + 255) // 256
+        offs = rng.choice([0.0, -30.0, -60.0, 25.0, -500.0], size=nb)     # blocks 2^40 .. 2^700 apart
+        lw = rng.normal(-3.0, 2.0, L) + np.repeat(offs, 256)[:L]
+        lw[rng.random(L) < 0.1] = -np.inf
+        if nb > 2 and trial % 3 == 0:
+            lw[256:512] = -np.inf                                      # an empty block
+        U = int(rng.integers(0, 2 ** 32))
+        a, st = oracle.resample_log(lw, N, U)
+        b, stb = _block_scaled_brute(list(lw), N, U)
+        assert st == stb == 0 and np.array_equal(a, b), trial
+    # every weight zero: the uniform resampling
+    a, st = oracle.resample_log(np.full(700, -np.inf), 9, 77)
+    b, stb = _block_scaled_brute([-math.inf] * 700, 9, 77)
+    assert st == stb == 1 and np.array_equal(a, b)
+    assert np.array_equal(a, (np.arange(9) * 2 ** 32 + 77) * 700 // (9 * 2 ** 32))
+
+
+def test_block_pow_closed_form():
+    for m in (-1000.5, -1.0, -1e-300, 0.0, 1e-300, 0.69, 0.7, 1.0, 100.0, 3e9, -3e9):
+        assert oracle.block_pow(m) == max(-2 ** 30, min(2 ** 30, math.ceil(m * 1.4426950408889634)))
+    # the block's largest weight keeps at least 2^31 quanta: exp(m - e ln 2) in (1/2, 1]
+    for m in np.linspace(-50, 50, 1001):
+        e = oracle.block_pow(float(m))
+        v = oracle.det_exp(float(m) - e * 0.6931471805599453)
+        assert 0.5 < v <= 1.0 + 1e-15
+
+
 def test_wheel_quirk_agrees_with_spec_off_ties():
     """The reference's literal FP64 wheel (pf2DRao.cpp:193-208) and the spec
     pick the same indices for generic weights (they differ only on exact ties
