@@ -12,13 +12,15 @@
 //           [g, g^2, (1-g) mu_k (D), (1-g^2) Sigma_k packed], then the shared
 //           H (m x D), BH (m) and the 2x2 joint blocks of R (r00, r01, r11).
 //
-// Frame pipeline (two launches per frame):
+// Frame pipeline (device-resident runs: one launch per frame):
 //   k_step    resolves each particle's ancestor from the previous frame's
 //             pending resampling (or reads a materialised ancestor array),
-//             gathers, predicts, updates, writes the log-weight and the block
-//             maximum (atomicMax of an order-preserving key).
-//   k_weights fixed-point weights q_i and per-block inclusive scans; block 0
+//             gathers, predicts, updates, writes the log-weight and, in
+//             blocks that are the resampler's blocks, the frame's
+//             block-scaled fixed-point weights, scans and exponent; block 0
 //             records the resampling uniform and advances the frame counter.
+//   k_weights the same weights as a separate launch (host-driven steps,
+//             other callers of the filter).
 // k_ancestors materialises the pending ancestors (end of a run, host steps,
 // stand-alone resampling).
 #pragma once
