@@ -457,7 +457,13 @@ def run_scaled(args):
 # (log-weight read 4, scan write 8) and ancestors (4); the estimate (12 floats
 # + index).
 def tracker_bytes_per_particle(d: int = 12, ppp: int = 10) -> int:
-    return ppp * 16 + 28 + step_bytes_per_particle(d) + 24 + 12 + 4 + 4 * d + 4
+    """Algorithmic HBM bytes per particle and arm of one tracker frame:
+    ppp proposals (FP64 weight and block-local scan written, the scan read
+    back by the selection), the measurement build (chosen index and 6-row
+    column written), the step (plus its measurement column read), the frame
+    weights' scan written by the step and read back by the ancestor search,
+    the ancestor written, the resampled state read by the estimate."""
+    return ppp * 24 + (4 + 24) + (frame_bytes_per_particle(d) + 24) + 4 + 4 * d
 
 
 def run_tracker(args):

@@ -325,9 +325,10 @@ int launch_weights(pf_handle *h, const uint32_t *uni, int parity)
 // Materialise the pending resampling of L weights into N ancestors.  The log
 // mean weight goes to lml, or to lml[*lml_frame] when a frame counter is given.
 int launch_ancestors(pf_handle *h, int L, int N, int *out, int parity, int kind_linear, double *lml,
-                     const int *lml_frame = nullptr, int set = 0)
+                     const int *lml_frame = nullptr, int set = 0, const int *pow_frame = nullptr)
 {
     AncArgs an{};
+    an.pow_frame = pow_frame;
     an.pend = pending_of(h, L, !kind_linear, set);
     an.fs = h->fs;
     an.anc = out;
@@ -464,6 +465,9 @@ int set_frame(pf_handle *h, int f)
 {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, f);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame + 1, f);
+    // the exponent slot of frame f (a step that writes its own weights
+    // raises it; later slots are cleared by the step before them)
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, &h->fs->pow_max[f % 3], kPowEmpty);
     return check_launch("k_set_int");
 }
 
@@ -507,20 +511,36 @@ AncArgs frame_anc_args(pf_handle *h, int p, double *lml)
     return an;
 }
 
+// With blocks of kWeightsBlock particles the step kernel writes the frame's
+// resampler weights itself (scratch set 0) and records their largest block
+// exponent in fs->pow_max[t mod 3] (no weights launch); returns whether it does.
+bool fuse_step_weights(pf_handle *h, StepArgs &a, int p)
+{
+    if ((h->cfg.block_size > 0 ? h->cfg.block_size : 256) != kWeightsBlock) return false;
+    a.w_cloc = h->cloc;
+    a.w_bsum = h->bsum;
+    a.w_bexp = h->bexp;
+    a.w_bpow = h->bpow;
+    a.frame_next = h->frame + (1 - p);
+    return true;
+}
+
 int step_ref(pf_handle *h, const float *Z, const uint32_t *uni, int parity, double *lml, bool with_estimate)
 {
     const int p = parity & 1;
     StepArgs a = step_args(h, kModeRef, uni, Z, 0, h->N, nullptr, nullptr, nullptr, p, kAncArray, nullptr);
+    const bool fused = fuse_step_weights(h, a, p);
     int st = launch_step(h, a, kModeRef);
     if (st) return st;
-    st = launch_weights(h, uni, p);
+    if (!fused) st = launch_weights(h, uni, p);
     if (st) return st;
     if (with_estimate) {
-        const AncArgs an = frame_anc_args(h, p, lml);
+        AncArgs an = frame_anc_args(h, p, lml);
+        if (fused) an.pow_frame = h->frame + p;
         st = dispatch_d<AncEstimateL>(h->D, h, an);      // reads the new state, h->xB before the swap
         if (!st) st = check_launch("k_ancestors_estimate");
     } else {
-        st = launch_ancestors(h, h->N, h->N, h->anc, p, 0, lml, h->frame + p);
+        st = launch_ancestors(h, h->N, h->N, h->anc, p, 0, lml, h->frame + p, 0, fused ? h->frame + p : nullptr);
     }
     if (st) return st;
     swap_state(h);
@@ -541,14 +561,14 @@ template <int D>
 struct PairL {
     static void run(pf_handle *h0, pf_handle *h1, const StepArgs &a0, const StepArgs &a1, const WeightArgs &w0,
                     const WeightArgs &w1, const AncEstArgs &e0, const AncEstArgs &e1, const EstFinalArgs &f0,
-                    const EstFinalArgs &f1, int block)
+                    const EstFinalArgs &f1, int block, bool fused)
     {
         hipStream_t s = h0->stream;
         const dim3 sg((h0->N + block - 1) / block, 2);
         hipLaunchKernelGGL((k_step_pair<D, kModeRef, 3>), sg, dim3(block),
                            (size_t)step_lds_bytes(a0.stage_floats, 0), s, a0, a1);
         const dim3 wg((h0->N + kWeightsBlock - 1) / kWeightsBlock, 2);
-        hipLaunchKernelGGL((k_weights_pair<kLogF32>), wg, dim3(kWeightsBlock), 0, s, w0, w1);
+        if (!fused) hipLaunchKernelGGL((k_weights_pair<kLogF32>), wg, dim3(kWeightsBlock), 0, s, w0, w1);
         const int nb = (h0->N + 255) / 256;
         hipLaunchKernelGGL((k_ancestors_estimate_pair<D>), dim3(nb, 2), dim3(256),
                            (size_t)anc_estimate_lds_bytes(e0.an.pend.nb, D), s, e0, e1);
@@ -569,10 +589,13 @@ int step_ref_pair(pf_handle *h0, pf_handle *h1, const float *Z0, const float *Z1
     WeightArgs w[2];
     AncEstArgs e[2];
     EstFinalArgs f[2];
+    bool fused = true;
     for (int i = 0; i < 2; ++i) {
         a[i] = step_args(h[i], kModeRef, uni[i], Z[i], 0, h[i]->N, nullptr, nullptr, nullptr, p, kAncArray, nullptr);
+        fused = fuse_step_weights(h[i], a[i], p) && fused;
         w[i] = weight_args(h[i], uni[i], p);
         e[i].an = frame_anc_args(h[i], p, lml[i]);
+        if (fused) e[i].an.pow_frame = h[i]->frame + p;
         e[i].x = h[i]->xB;                      // the updated state, before the swap
         e[i].partial = h[i]->est_partial;
         e[i].ld = h[i]->ld;
@@ -585,7 +608,7 @@ int step_ref_pair(pf_handle *h0, pf_handle *h1, const float *Z0, const float *Z1
     if (a[0].stage_floats != a[1].stage_floats)
         return set_error(PF_ERR_UNSUPPORTED, "the two filters stage different model images");
     const int block = h0->cfg.block_size > 0 ? h0->cfg.block_size : 256;
-    int st = dispatch_d<PairL>(h0->D, h0, h1, a[0], a[1], w[0], w[1], e[0], e[1], f[0], f[1], block);
+    int st = dispatch_d<PairL>(h0->D, h0, h1, a[0], a[1], w[0], w[1], e[0], e[1], f[0], f[1], block, fused);
     if (!st) st = check_launch("pair step");
     if (st) return st;
     swap_state(h0);

@@ -956,8 +956,19 @@ struct AncArgs {
     int *anc;
     double *lml;     // optional: written with the frame's log mean weight
     const int *lml_frame;   // optional: the slot is lml[*lml_frame] (read at run time, graph replays)
+    const int *pow_frame;   // optional: frame t whose step recorded the largest block exponent in
+                            // fs->pow_max[t mod 3] (else the resolver reduces it)
     int N, parity, kind_linear;
 };
+
+// The pending resampling of a, with the recorded largest block exponent when
+// its producer wrote one.
+__device__ __forceinline__ Pending anc_pending(const AncArgs &a)
+{
+    Pending pd = a.pend;
+    if (a.pow_frame && pd.bpow) pd.pow_max = &a.fs->pow_max[*a.pow_frame % 3];
+    return pd;
+}
 
 static __global__ void __launch_bounds__(256) k_ancestors(AncArgs a)
 {
@@ -966,7 +977,7 @@ static __global__ void __launch_bounds__(256) k_ancestors(AncArgs a)
     const int j = j0 + threadIdx.x;
     double es = 0.0;
     ResolveStats rs;
-    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0 && a.lml, es, rs);
+    const int src = resolve_ancestor(anc_pending(a), a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0 && a.lml, es, rs);
     if (j < a.N) a.anc[j] = src;
     if (blockIdx.x == 0 && threadIdx.x == 0)
         finish_resampling(a.fs, a.parity, es, rs, a.N, a.kind_linear,
@@ -1240,7 +1251,7 @@ __device__ __forceinline__ void ancestors_estimate_body(const AncArgs &a, const 
     const int j = j0 + threadIdx.x;
     double es = 0.0;
     ResolveStats rs;
-    const int src = resolve_ancestor(a.pend, a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0 && a.lml, es, rs);
+    const int src = resolve_ancestor(anc_pending(a), a.N, a.fs->U_pending, j0, wsp, blockIdx.x == 0 && a.lml, es, rs);
     if (j < a.N) a.anc[j] = src;
     if (blockIdx.x == 0 && threadIdx.x == 0)
         finish_resampling(a.fs, a.parity, es, rs, a.N, a.kind_linear,
