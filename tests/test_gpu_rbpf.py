@@ -32,8 +32,12 @@ def _relerr(a, b):
 
 
 # ------------------------------------------------------------------ resampling
+# (200000, 300), (65536, 4000), (600000, 20000): many inputs between
+# consecutive outputs, so an output block's input span exceeds the LDS stage
+# and the search probes the input block in global memory.
 @pytest.mark.parametrize("L,N", [(1, 1), (1, 7), (7, 1), (15, 500), (500, 500), (1000, 333),
-                                 (4097, 4096), (100000, 100000), (257, 70000)])
+                                 (4097, 4096), (100000, 100000), (257, 70000),
+                                 (200000, 300), (65536, 4000), (600000, 20000)])
 def test_resample_linear_bitexact(gpu, L, N):
     rb = Rbpf(max(N, 1))
     rng = np.random.default_rng(L * 7 + N)
@@ -70,15 +74,15 @@ def test_resample_edge_cases(gpu):
     rb.close()
 
 
-@pytest.mark.parametrize("L", [10, 1000, 65537])
-def test_resample_log_bitexact(gpu, L):
-    rb = Rbpf(L)
-    rng = np.random.default_rng(L)
+@pytest.mark.parametrize("L,N", [(10, 10), (1000, 1000), (65537, 65537), (65537, 500), (300000, 3000)])
+def test_resample_log_bitexact(gpu, L, N):
+    rb = Rbpf(N)
+    rng = np.random.default_rng(L + N)
     lw = rng.normal(-50, 10, L)
     lw[rng.random(L) < 0.1] = -np.inf
     U = int(rng.integers(0, 2 ** 32))
-    g, st = rb.resample(lw, L, U, log=True)
-    o, sto = oracle.resample_log(lw, L, U)
+    g, st = rb.resample(lw, N, U, log=True)
+    o, sto = oracle.resample_log(lw, N, U)
     assert st == sto == 0
     assert np.array_equal(g, o)
     rb.close()
