@@ -140,6 +140,61 @@ def test_step_candidate_mode_teacher_forced(gpu, N):
     rb.close()
 
 
+def test_headline_shape_against_oracle_and_device_run(gpu):
+    """The single-sequence workload's own shape (N = 100,000 particles, K = 15,
+    d = 12, 8 candidates per hand of capacity 16, clutter density 7 / (640 x
+    480)): two teacher-forced frames against the FP64 oracle (components and
+    ancestors bit-exact, states 1e-4), then the same three frames run resident
+    on the device with graph replay, equal bit for bit to host-driven steps."""
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    N, cap, cpj = 100_000, 16, 8
+    lam = (cpj - 1) / (640.0 * 480.0)
+    mdl = synthetic.make_model(seed=2024)
+    seq = synthetic.simulate(mdl, 3, seed=2024, cand_per_joint=cpj, cap=cap)
+    om = oracle.Model(mdl.weights, mdl.gamma, mdl.means, mdl.Sigma, mdl.H(), mdl.BH(), mdl.R)
+    rbs = []
+    for _ in range(2):
+        rb = Rbpf(N, cand_capacity=cap, clutter_density=lam)
+        rb.load_model(mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean,
+                      sel=mdl.sel, R=mdl.R)
+        rb.set_joint_clutter(mdl.clutter)
+        rbs.append(rb)
+    rb, rb2 = rbs
+    bins, _ = oracle.resample_linear(mdl.weights, N, 5)
+    x0, P0 = om.reset(bins)
+    rb.set_tracks(x0, P0)
+    rb2.set_tracks(x0, P0)
+    lml_host = []
+    for t in range(3):
+        U1, U2 = int(seq.uniforms[t, 0]), int(seq.uniforms[t, 1])
+        st, out = rb.step_candidates(seq.count[t], seq.cand[t], U1, U2)
+        lml_host.append(out["lml"])
+        if t < 2:
+            ref = om.update_pdaf(x0, P0, mdl.clutter, seq.count[t], seq.cand[t], U1, U2,
+                                 0.95, 9.210340371976184, lam)
+            assert np.array_equal(out["comp"], ref["comp"])
+            assert np.all(np.abs(out["lw"] - ref["lw"]) <= 1e-4 * np.maximum(1.0, np.abs(ref["lw"])))
+            anc_o, _ = oracle.resample_log(out["lw"].astype(np.float64), N, U2)
+            assert np.array_equal(out["anc"], anc_o)
+            assert np.all(np.diff(out["anc"]) >= 0)
+            x, P = rb.get_tracks()
+            assert _relerr(x, ref["xu"][out["anc"]]).max() <= 1e-4
+            assert _relerr(P, ref["Pu"][out["anc"]]).max() <= 1e-4
+            x0, P0 = x, P
+    x1, P1 = rb.get_tracks()
+    d_uni = DeviceArray(seq.uniforms.astype(np.uint32))
+    d_lml = DeviceArray.empty(3, np.float64)
+    d_cnt = DeviceArray(seq.count.astype(np.int32))
+    d_cand = DeviceArray(seq.cand.astype(np.float32))
+    rb2.run_candidates_dev(3, d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, 1)
+    rb2.synchronize()
+    x2, P2 = rb2.get_tracks()
+    assert np.array_equal(x1, x2) and np.array_equal(P1, P2)
+    assert np.array_equal(d_lml.get(), np.array(lml_host))
+    assert np.all(np.isfinite(lml_host))
+    rb.close(); rb2.close()
+
+
 def test_candidate_mode_no_detections(gpu):
     """A cluttered joint with no detection leaves the state at its prediction."""
     N = 128

@@ -168,6 +168,23 @@ def test_frames_match_oracle(gpu, frames):
         a.close()
 
 
+def test_tracker_shape_frames_match_oracle(gpu, frames):
+    """The tracker workload's own shape: 100,000 particles per arm (10^6
+    proposals per arm), the shipped models, two frames checked stage by stage
+    against the oracle.  The twins go through the FP64 basis round trip of
+    get_tracks / set_tracks, so they are compared within 1e-6."""
+    N = 100_000
+    imgs, rois = frames
+    tr, arms = _tracker(N)
+    twins = [_arm(p, N, a)[0] for a, p in enumerate((LEFT, RIGHT))]
+    arms_o = [_oracle_model(LEFT)[0], _oracle_model(RIGHT)[0]]
+    _against_oracle(tr, arms, twins, arms_o, [_pca(LEFT), _pca(RIGHT)], N, imgs, rois, frames=2,
+                    twin_exact=False)
+    tr.close()
+    for a in arms + twins:
+        a.close()
+
+
 def _resized_arm(path, D, seed):
     """The shipped arm model with state dimension D: its first D coordinates
     (a marginal of the mixture) for D < 12, or padded with D - 12 coordinates
