@@ -450,12 +450,6 @@ def run_scaled(args):
     print(json.dumps(out))
 
 
-# Algorithmic HBM bytes of one tracker frame, per particle of one arm (d = 12,
-# 10 proposals per particle): proposals 10 x (FP64 weight write 8 + block scan
-# write 8); measurement build (index 4 + 6 rows x 4); the filter step
-# (step_bytes_per_particle + the measurement column 24); its weights scan
-# (log-weight read 4, scan write 8) and ancestors (4); the estimate (12 floats
-# + index).
 def tracker_bytes_per_particle(d: int = 12, ppp: int = 10) -> int:
     """Algorithmic HBM bytes per particle and arm of one tracker frame:
     ppp proposals (FP64 weight and block-local scan written, the scan read

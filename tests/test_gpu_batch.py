@@ -177,3 +177,75 @@ def test_batch_d24_matches_single_runs(gpu):
         assert np.array_equal(x, ref[s][0]) and np.array_equal(P, ref[s][1]), s
         assert np.array_equal(lml[offs[s]:offs[s] + Ts[s]], ref[s][2]), s
     b.close()
+
+
+def test_scaled_shape_batch_against_oracle_and_single_runs(gpu):
+    """The scaled workload's own shape: 32 sequences x 31,250 particles in one
+    batch (d = 24, K = 64, m = 10, 5 PDAF joints, 16 candidates per joint),
+    two frames.  Sequence 0's first frame against the FP64 oracle (components
+    bit-exact, log-weights and states 1e-4), and sequences 0 and 31 equal bit
+    for bit to independent single-sequence runs."""
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    S, N, T, K, d, cap, cpj = 32, 31_250, 2, 64, 24, 16, 11
+    lam = (cpj - 1) / (640.0 * 480.0)
+    joints = ("head", "left_hand", "right_hand", "left_elbow", "right_elbow")
+    models = [synthetic.make_model(seed=13_000 + s, K=K, d=d, joints=joints, clutter=(1, 1, 1, 1, 1))
+              for s in range(S)]
+    seqs = [synthetic.simulate_fast(models[s], T, seed=13_000 + s, cand_per_joint=cpj, cap=cap) for s in range(S)]
+    bins = [oracle.resample_linear(models[s].weights, N, 500 + s)[0] for s in range(S)]
+
+    def single(s):
+        mdl = models[s]
+        rb = Rbpf(N, cand_capacity=cap, clutter_density=lam)
+        rb.load_model(mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, sel=mdl.sel, R=mdl.R)
+        rb.set_joint_clutter(mdl.clutter)
+        rb.reset(bins[s])
+        return rb
+
+    # sequence 0, frame 0 against the oracle, from the reset tracks
+    mdl = models[0]
+    rb = single(0)
+    om = oracle.Model(mdl.weights, mdl.gamma, mdl.means, mdl.Sigma, mdl.H(), mdl.BH(), mdl.R)
+    x0, P0 = rb.get_tracks()
+    U1, U2 = int(seqs[0].uniforms[0, 0]), int(seqs[0].uniforms[0, 1])
+    st, out = rb.step_candidates(seqs[0].count[0], seqs[0].cand[0], U1, U2)
+    ref = om.update_pdaf(x0, P0, mdl.clutter, seqs[0].count[0], seqs[0].cand[0], U1, U2, 0.95, 9.210340371976184,
+                         lam)
+    assert np.array_equal(out["comp"], ref["comp"])
+    assert np.all(np.abs(out["lw"] - ref["lw"]) <= 1e-4 * np.maximum(1.0, np.abs(ref["lw"])))
+    anc_o, _ = oracle.resample_log(out["lw"].astype(np.float64), N, U2)
+    assert np.array_equal(out["anc"], anc_o)
+    x, P = rb.get_tracks()
+    n = x.shape[0]
+    rel = lambda a, b: (np.linalg.norm((a - b).reshape(n, -1), axis=1)
+                        / np.maximum(np.linalg.norm(b.reshape(n, -1), axis=1), 1e-30))
+    assert rel(x, ref["xu"][out["anc"]]).max() <= 1e-4
+    assert rel(P, ref["Pu"][out["anc"]]).max() <= 1e-4
+    rb.close()
+
+    # the whole batch against single runs of its first and last sequences
+    singles = {}
+    for s in (0, S - 1):
+        rb = single(s)
+        dd = [DeviceArray(seqs[s].count.astype(np.int32)), DeviceArray(seqs[s].cand.astype(np.float32)),
+              DeviceArray(seqs[s].uniforms.astype(np.uint32)), DeviceArray.empty(T, np.float64)]
+        rb.run_candidates_dev(T, dd[0].ptr, dd[1].ptr, dd[2].ptr, dd[3].ptr, 1)
+        rb.synchronize()
+        singles[s] = (*rb.get_tracks(), dd[3].get())
+        rb.close()
+    b = RbpfBatch([N] * S, [T] * S, K=K, m=10, d=d, cand_capacity=cap, clutter_density=lam)
+    for s, mdl in enumerate(models):
+        b.load_model(s, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
+    b.set_joint_clutter(models[0].clutter)
+    for s in range(S):
+        b.reset(s, bins[s])
+    dd = _batch_inputs(seqs)
+    b.run_dev(dd[0].ptr, dd[1].ptr, dd[2].ptr, dd[3].ptr)
+    lml = dd[3].get()
+    assert np.all(np.isfinite(lml))
+    offs = b.frame_offsets
+    for s, (xs, Ps, ls) in singles.items():
+        x, P = b.get_tracks(s)
+        assert np.array_equal(x, xs) and np.array_equal(P, Ps), s
+        assert np.array_equal(lml[offs[s]:offs[s] + T], ls), s
+    b.close()
