@@ -249,3 +249,52 @@ def test_scaled_shape_batch_against_oracle_and_single_runs(gpu):
         assert np.array_equal(x, xs) and np.array_equal(P, Ps), s
         assert np.array_equal(lml[offs[s]:offs[s] + T], ls), s
     b.close()
+
+
+def test_batched_workload_shape_matches_single_runs(gpu):
+    """The batched workload's own shape: 256 sequences with lengths U{100..2000}
+    frames and particle counts log-uniform in [1k, 64k] (seeded as the bench
+    draws them), each with its own model, in one batch; the sequences with
+    the most particles, the most frames and the fewest particles equal bit for
+    bit to independent single-sequence runs, every frame's log mean weight
+    finite."""
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    seed, S, cap, cpj = 7, 256, 16, 8
+    lam = (cpj - 1) / (640.0 * 480.0)
+    rng = np.random.default_rng(seed)
+    Ts = rng.integers(100, 2001, S).astype(np.int32)
+    Ns = np.exp(rng.uniform(np.log(1000), np.log(64000), S)).astype(np.int32)
+    models = [synthetic.make_model(seed=seed * 1000 + s) for s in range(S)]
+    seqs = [synthetic.simulate_fast(models[s], int(Ts[s]), seed=seed * 1000 + s, cand_per_joint=cpj, cap=cap)
+            for s in range(S)]
+    bins = [oracle.resample_linear(models[s].weights, int(Ns[s]), 300 + s)[0] for s in range(S)]
+    picks = sorted({int(np.argmax(Ns)), int(np.argmax(Ts)), int(np.argmin(Ns))})
+    singles = {}
+    for s in picks:
+        mdl, N, T = models[s], int(Ns[s]), int(Ts[s])
+        rb = Rbpf(N, cand_capacity=cap, clutter_density=lam)
+        rb.load_model(mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, sel=mdl.sel, R=mdl.R)
+        rb.set_joint_clutter(mdl.clutter)
+        rb.reset(bins[s])
+        dd = [DeviceArray(seqs[s].count.astype(np.int32)), DeviceArray(seqs[s].cand.astype(np.float32)),
+              DeviceArray(seqs[s].uniforms.astype(np.uint32)), DeviceArray.empty(T, np.float64)]
+        rb.run_candidates_dev(T, dd[0].ptr, dd[1].ptr, dd[2].ptr, dd[3].ptr, 1)
+        rb.synchronize()
+        singles[s] = (*rb.get_tracks(), dd[3].get())
+        rb.close()
+    b = RbpfBatch(Ns, Ts, K=models[0].K, m=models[0].m, cand_capacity=cap, clutter_density=lam)
+    for s, mdl in enumerate(models):
+        b.load_model(s, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
+    b.set_joint_clutter(models[0].clutter)
+    for s in range(S):
+        b.reset(s, bins[s])
+    dd = _batch_inputs(seqs)
+    b.run_dev(dd[0].ptr, dd[1].ptr, dd[2].ptr, dd[3].ptr)
+    lml = dd[3].get()
+    assert lml.shape[0] == int(Ts.sum()) and np.all(np.isfinite(lml))
+    offs = b.frame_offsets
+    for s, (xs, Ps, ls) in singles.items():
+        x, P = b.get_tracks(s)
+        assert np.array_equal(x, xs) and np.array_equal(P, Ps), s
+        assert np.array_equal(lml[offs[s]:offs[s] + int(Ts[s])], ls), s
+    b.close()
