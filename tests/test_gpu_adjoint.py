@@ -146,3 +146,46 @@ def test_torch_function_backward(gpu, run):
     for p, k in zip(params, ("means", "logits", "rates", "chol")):
         assert _rel(p.grad.numpy(), gd[k].sum(axis=0)) <= 1e-6, k
     b.close()
+
+
+def test_adjoint_workload_shape(gpu):
+    """The adjoint workload's own shape: 64 sequences x 16,384 particles x 500
+    frames (own model per sequence, seeded as the bench draws them),
+    checkpoint every 50 frames.  Size-independent properties: the recorded
+    forward gives the plain run's log mean weights bit for bit, the gradient
+    does not depend on the checkpoint interval (50 vs 125; within 1e-5: the
+    float atomics of the component sums add in a varying order over 500
+    frames) and every entry is finite."""
+    from mkfbodytracker_pdaf_amd.batch import RbpfBatch
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    S, N, T, seed, cap, cpj = 64, 16_384, 500, 11, 16, 8
+    lam = (cpj - 1) / (640.0 * 480.0)
+    models = [synthetic.make_model(seed=seed * 1000 + s) for s in range(S)]
+    seqs = [synthetic.simulate_fast(models[s], T, seed=seed * 1000 + s, cand_per_joint=cpj, cap=cap)
+            for s in range(S)]
+    bins = [oracle.resample_linear(models[s].weights, N, 700 + s)[0] for s in range(S)]
+    b = RbpfBatch([N] * S, [T] * S, K=models[0].K, m=models[0].m, cand_capacity=cap, clutter_density=lam)
+    for s, mdl in enumerate(models):
+        b.load_model(s, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
+    b.set_joint_clutter(models[0].clutter)
+    d_cnt = DeviceArray(np.concatenate([q.count for q in seqs]).astype(np.int32))
+    d_cand = DeviceArray(np.concatenate([q.cand for q in seqs]).astype(np.float32))
+    d_uni = DeviceArray(np.concatenate([q.uniforms for q in seqs]).astype(np.uint32))
+    d_lml = DeviceArray.empty(S * T, np.float64)
+
+    def reset_all():
+        for s in range(S):
+            b.reset(s, bins[s])
+
+    reset_all()
+    b.run_dev(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr)
+    lml_plain = d_lml.get()
+    grads = []
+    for C in (50, 125):
+        reset_all()
+        grads.append(b.lml_grad(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, checkpoint_every=C))
+        assert np.array_equal(d_lml.get(), lml_plain), C
+    for k in grads[0]:
+        assert np.all(np.isfinite(grads[0][k])), k
+        assert _rel(grads[1][k], grads[0][k]) <= 1e-5, (k, _rel(grads[1][k], grads[0][k]))
+    b.close()
