@@ -167,6 +167,10 @@ int check_launch(const char *what)
 }
 
 // ---------------------------------------------------------- kernel dispatch
+// Frames per captured graph of the device-resident runs (even: frame f uses
+// scratch set and frame counter f & 1).
+constexpr int kGraphFrames = 64;
+
 template <int D, int SELJ>
 int launch_step_dj(pf_handle *h, const StepArgs &a, int mode, int block)
 {
@@ -1093,25 +1097,27 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
     } else {
         st = one_frame(0);
         if (st) return st;
-        // Frames 1..T-1: an (odd, even) pair captured once and replayed.
-        const int pairs = (T - 1) / 2;
-        if (pairs > 0) {
+        // Frames 1..T-1: kGraphFrames consecutive frames (an even count, so
+        // the buffers and counters are back in place) captured once and
+        // replayed; the remaining frames are launched one by one.
+        const int reps = (T - 1) / kGraphFrames;
+        if (reps > 0) {
             hipGraph_t g;
             hipGraphExec_t ge;
             HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            int s1 = one_frame(1);
-            int s2 = s1 ? s1 : one_frame(2);
+            int sc = 0;
+            for (int f = 1; f <= kGraphFrames && !sc; ++f) sc = one_frame(f);
             hipError_t ec = hipStreamEndCapture(h->stream, &g);
-            if (s2) return s2;
+            if (sc) return sc;
             if (ec != hipSuccess) return set_error(PF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
             HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-            for (int p = 0; p < pairs; ++p) HIP_TRY(hipGraphLaunch(ge, h->stream));
+            for (int r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(ge, h->stream));
             HIP_TRY(hipStreamSynchronize(h->stream));
             (void)hipGraphExecDestroy(ge);
             (void)hipGraphDestroy(g);
         }
-        if ((T - 1) & 1) {
-            st = one_frame(T - 1);
+        for (int f = 1 + reps * kGraphFrames; f < T; ++f) {
+            st = one_frame(f);
             if (st) return st;
         }
     }

@@ -49,6 +49,7 @@ constexpr int kMaxRadius = 15;
 constexpr int kBlurW = 32, kBlurH = 16, kBlurThreads = 128;
 constexpr int kProposeBlock = 256;
 constexpr int kMaxArmD = 24;              // k_propose stages 3 D + 2 projection operands per arm
+constexpr int kTrGraphFrames = 16;        // frames per captured graph of a device-resident run (even)
 constexpr int kPropShift = 10;            // the proposal resampler's input blocks: 1024 proposals
 constexpr int kMaxProposals = 1 << 20;    // resampler scan limit
 
@@ -1011,23 +1012,25 @@ int pf_tracker_run_dev(pf_tracker *tr, int32_t T, const uint8_t *likelihood, con
     auto frame = [&](int f) { return enqueue_frame(tr, f & 1, T, likelihood, roi, out); };
     st = frame(0);
     if (st) return st;
-    const int pairs = graph ? (T - 1) / 2 : 0;
-    if (pairs > 0) {
+    // kTrGraphFrames consecutive frames (an even count: frame f uses the
+    // buffers and counters of parity f & 1) captured once and replayed
+    const int reps = graph ? (T - 1) / kTrGraphFrames : 0;
+    if (reps > 0) {
         hipGraph_t gr;
         hipGraphExec_t ge;
         TR_TRY(hipStreamBeginCapture(tr->s[0], hipStreamCaptureModeThreadLocal));
-        int s1 = frame(1);
-        int s2 = s1 ? s1 : frame(2);
+        int sc = 0;
+        for (int f = 1; f <= kTrGraphFrames && !sc; ++f) sc = frame(f);
         hipError_t ec = hipStreamEndCapture(tr->s[0], &gr);
-        if (s2) return s2;
+        if (sc) return sc;
         if (ec != hipSuccess) return set_error(PF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
         TR_TRY(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
-        for (int q = 0; q < pairs; ++q) TR_TRY(hipGraphLaunch(ge, tr->s[0]));
+        for (int q = 0; q < reps; ++q) TR_TRY(hipGraphLaunch(ge, tr->s[0]));
         TR_TRY(hipStreamSynchronize(tr->s[0]));
         (void)hipGraphExecDestroy(ge);
         (void)hipGraphDestroy(gr);
     }
-    for (int f = 1 + 2 * pairs; f < T; ++f) {
+    for (int f = 1 + reps * kTrGraphFrames; f < T; ++f) {
         st = frame(f);
         if (st) return st;
     }

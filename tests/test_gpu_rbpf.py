@@ -308,7 +308,7 @@ def test_kf_model_predict_update(gpu):
 # ------------------------------------------------- device-resident sequences
 @pytest.mark.parametrize("mode", ["candidates", "reference"])
 @pytest.mark.parametrize("flags", [0, 1])
-@pytest.mark.parametrize("T", [1, 2, 7])
+@pytest.mark.parametrize("T", [1, 2, 7, 70])
 def test_device_run_matches_host_steps(gpu, mode, flags, T):
     """pf_run_*_dev (ancestors resolved inside the next step kernel, optional
     hipGraph replay) gives bit-identical tracks, ancestors and per-frame log

@@ -238,10 +238,10 @@ def test_arms_of_other_state_dimensions(gpu, frames):
             a.close()
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_device_run_equals_frame_calls(gpu, graph):
+@pytest.mark.parametrize("graph,T", [(True, 6), (False, 6), (True, 20)])
+def test_device_run_equals_frame_calls(gpu, graph, T):
     from mkfbodytracker_pdaf_amd.devmem import DeviceArray
-    N, T = 700, 6
+    N = 700
     imgs, rois = synthetic.tracker_sequence(T, seed=8)
     tr_a, arms_a = _tracker(N)
     outs = [tr_a.callback(imgs[t], rois[t]) for t in range(T)]
