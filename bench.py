@@ -465,7 +465,7 @@ def run_tracker(args):
     shipped GMM files (left and right arm), N particles per arm, 10 N hand
     proposals per arm, synthetic 480 x 640 likelihood images with moving
     hand blobs and clutter, face ROIs; T frames in one device-resident run
-    (a captured frame pair replayed)."""
+    (16 captured frames replayed at a time)."""
     from mkfbodytracker_pdaf_amd import synthetic
     from mkfbodytracker_pdaf_amd.devmem import DeviceArray, synchronize
     from mkfbodytracker_pdaf_amd.tracker import PoseTracker

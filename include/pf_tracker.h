@@ -116,7 +116,7 @@ int pf_tracker_frame(pf_tracker *tr, const uint8_t *likelihood, const int32_t *r
 
 /* T frames back to back, all pointers on the device: likelihood T x rows x
  * cols, roi T x 4 (every frame has a face), outputs as above.  flags bit 0:
- * capture a frame pair into a HIP graph and replay it.  Returns after
+ * capture 16 frames into a HIP graph and replay them.  Returns after
  * enqueueing; call pf_tracker_synchronize. */
 int pf_tracker_run_dev(pf_tracker *tr, int32_t T, const uint8_t *likelihood, const int32_t *roi,
                        const pf_tracker_outputs *out, int32_t flags);
