@@ -130,8 +130,10 @@ int pf_get_sample_prob(pf_handle *h, const double *Hp, const double *M, int32_t 
  * pointers on the handle's device: uniforms T x 2 (u_comp, u_res per frame),
  * lml T (FP64, written), and either Z (T x m x N floats, reference mode) or
  * count (T x J) + cand (T x J x cap x 2 floats, candidate mode).
- * If use_graph != 0 the frame loop is captured once into a hipGraph and
- * replayed.  Returns after enqueueing; pf_synchronize waits. */
+ * use_graph bit 0 (value 1): the frames are captured into a hipGraph (64
+ * at a time) and replayed; bits 1 and 2 time the run (see
+ * pf_last_run_kernel_time; bit 1 turns the graph off).  Returns after
+ * enqueueing; pf_synchronize waits. */
 int pf_run_ref_dev(pf_handle *h, int32_t T, const float *Z, const uint32_t *uniforms,
                    double *lml, int32_t use_graph);
 int pf_run_candidates_dev(pf_handle *h, int32_t T, const int32_t *count, const float *cand,
@@ -139,9 +141,13 @@ int pf_run_candidates_dev(pf_handle *h, int32_t T, const int32_t *count, const f
 int pf_synchronize(pf_handle *h);
 /* The handle's HIP stream (hipStream_t) for callers that order their own work. */
 int pf_get_stream(pf_handle *h, void **stream);
-/* Device time of the step kernel alone over the last pf_run_*_dev call, in ms,
- * and the number of step launches it covered (HIP events on the handle's
- * stream; used for the roofline figure). */
+/* Device time of the last pf_run_*_dev call, in ms, and the number of step
+ * launches it covered (HIP events on the handle's stream; used for the
+ * roofline figure).  use_graph bit 1 (value 2): an event pair around every
+ * step launch, no graph; step_ms is their sum, frame_ms the whole run.
+ * Bit 2 (value 4): one event pair around all frames of the run (graph or
+ * not); step_ms = frame_ms = that time, which is the step launches back to
+ * back when a frame is one launch.  Without either bit nothing is timed. */
 int pf_last_run_kernel_time(pf_handle *h, double *step_ms, double *frame_ms, int32_t *launches);
 
 /* ---------------------------------------------------------- KF_model */

@@ -1027,14 +1027,15 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
     if (st) return st;
     if (!h->tracks) return set_error(PF_ERR_STATE, "tracks not initialised (pf_reset or pf_set_tracks)");
     if (T <= 0 || !uniforms || !lml) return set_error(PF_ERR_ARG, "bad run arguments");
-    const bool timing = (flags & 2) != 0;
+    const bool timing = (flags & 2) != 0;                 // an event pair around every launch (no graph)
+    const bool run_timing = (flags & 4) != 0 && !timing;  // one event pair around all frames
     const bool graph = (flags & 1) != 0 && !timing;
     const long long zstride = (long long)h->m * h->N;
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
     st = check_launch("k_set_int");
     if (st) return st;
-    if (timing) {
-        const size_t need = 2 * (size_t)T + 2;
+    if (timing || run_timing) {
+        const size_t need = timing ? 2 * (size_t)T + 2 : 2;
         while (h->ev.size() < need) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
@@ -1087,6 +1088,7 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
         swap_state(h);
         return PF_OK;
     };
+    if (run_timing) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
     if (!graph) {
         if (timing) HIP_TRY(hipEventRecord(h->ev[2 * T], h->stream));
         for (int f = 0; f < T; ++f) {
@@ -1121,9 +1123,18 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
             if (st) return st;
         }
     }
+    if (run_timing) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
     // Materialise the last frame's ancestors and its log mean weight.
     st = launch_ancestors(h, h->N, h->N, h->anc, (T - 1) & 1, 0, lml + (T - 1), nullptr, fused ? (T - 1) & 1 : 0);
     if (st) return st;
+    if (run_timing) {
+        HIP_TRY(hipEventSynchronize(h->ev[1]));
+        float tot = 0.f;
+        HIP_TRY(hipEventElapsedTime(&tot, h->ev[0], h->ev[1]));
+        h->last_step_ms = tot;            // one step launch per frame when fused
+        h->last_frame_ms = tot;
+        h->timed_launches = T;
+    }
     if (timing) {
         HIP_TRY(hipEventSynchronize(h->ev[2 * T + 1]));
         double sum = 0.0;

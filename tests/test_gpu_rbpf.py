@@ -350,3 +350,32 @@ def test_device_run_matches_host_steps(gpu, mode, flags, T):
     assert np.array_equal(x1, x2) and np.array_equal(P1, P2)
     assert np.array_equal(d_lml.get(), np.array(lml_host))
     rb.close(); rb2.close()
+
+
+def test_timed_device_run_is_the_same_run(gpu):
+    """Timing a device-resident run (one event pair around all frames, with
+    or without the graph) changes nothing in its results and reports one
+    step launch per frame."""
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    N, T = 3000, 70
+    mdl, _, _ = _setup(N)
+    seq = synthetic.simulate(mdl, T, seed=41, cand_per_joint=4, cap=8)
+    bins, _ = oracle.resample_linear(mdl.weights, N, 17)
+    d_cnt = DeviceArray(seq.count.astype(np.int32))
+    d_cand = DeviceArray(seq.cand.astype(np.float32))
+    d_uni = DeviceArray(seq.uniforms.astype(np.uint32))
+    res = []
+    for flags in (1, 1 | 4, 4):
+        _, rb, _ = _setup(N)
+        rb.set_joint_clutter(mdl.clutter)
+        rb.reset(bins)
+        d_lml = DeviceArray.empty(T, np.float64)
+        rb.run_candidates_dev(T, d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, flags)
+        rb.synchronize()
+        if flags & 4:
+            step_ms, frame_ms, launches = rb.last_kernel_time()
+            assert launches == T and step_ms > 0 and step_ms == frame_ms
+        res.append((*rb.get_tracks(), d_lml.get()))
+        rb.close()
+    for r in res[1:]:
+        assert all(np.array_equal(a, b) for a, b in zip(r, res[0]))
