@@ -565,7 +565,7 @@ template <int D>
 struct PairL {
     static void run(pf_handle *h0, pf_handle *h1, const StepArgs &a0, const StepArgs &a1, const WeightArgs &w0,
                     const WeightArgs &w1, const AncEstArgs &e0, const AncEstArgs &e1, const EstFinalArgs &f0,
-                    const EstFinalArgs &f1, int block, bool fused)
+                    const EstFinalArgs &f1, int block, bool fused, bool final_sums)
     {
         hipStream_t s = h0->stream;
         const dim3 sg((h0->N + block - 1) / block, 2);
@@ -576,12 +576,23 @@ struct PairL {
         const int nb = (h0->N + 255) / 256;
         hipLaunchKernelGGL((k_ancestors_estimate_pair<D>), dim3(nb, 2), dim3(256),
                            (size_t)anc_estimate_lds_bytes(e0.an.pend.nb, D), s, e0, e1);
-        hipLaunchKernelGGL((k_estimate_final_pair<D>), dim3(2), dim3(256), 0, s, f0, f1);
+        if (final_sums) hipLaunchKernelGGL((k_estimate_final_pair<D>), dim3(2), dim3(256), 0, s, f0, f1);
     }
 };
 
+EstFinalArgs estimate_final_args(pf_handle *h)
+{
+    EstFinalArgs f{};
+    f.partial = h->est_partial;
+    f.Ti = h->Tinv;
+    f.out = h->est_out;
+    f.nb = (h->N + 255) / 256;
+    f.N = h->N;
+    return f;
+}
+
 int step_ref_pair(pf_handle *h0, pf_handle *h1, const float *Z0, const float *Z1, const uint32_t *uni0,
-                  const uint32_t *uni1, int parity, double *lml0, double *lml1)
+                  const uint32_t *uni1, int parity, double *lml0, double *lml1, bool final_sums)
 {
     if (!pair_compatible(h0, h1)) return set_error(PF_ERR_UNSUPPORTED, "the two filters cannot share launches");
     const int p = parity & 1;
@@ -603,16 +614,13 @@ int step_ref_pair(pf_handle *h0, pf_handle *h1, const float *Z0, const float *Z1
         e[i].x = h[i]->xB;                      // the updated state, before the swap
         e[i].partial = h[i]->est_partial;
         e[i].ld = h[i]->ld;
-        f[i].partial = h[i]->est_partial;
-        f[i].Ti = h[i]->Tinv;
-        f[i].out = h[i]->est_out;
-        f[i].nb = (h[i]->N + 255) / 256;
-        f[i].N = h[i]->N;
+        f[i] = estimate_final_args(h[i]);
     }
     if (a[0].stage_floats != a[1].stage_floats)
         return set_error(PF_ERR_UNSUPPORTED, "the two filters stage different model images");
     const int block = h0->cfg.block_size > 0 ? h0->cfg.block_size : 256;
-    int st = dispatch_d<PairL>(h0->D, h0, h1, a[0], a[1], w[0], w[1], e[0], e[1], f[0], f[1], block, fused);
+    int st = dispatch_d<PairL>(h0->D, h0, h1, a[0], a[1], w[0], w[1], e[0], e[1], f[0], f[1], block, fused,
+                               final_sums);
     if (!st) st = check_launch("pair step");
     if (st) return st;
     swap_state(h0);

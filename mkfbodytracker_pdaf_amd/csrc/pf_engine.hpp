@@ -49,9 +49,14 @@ FrameScratch *scratch(pf_handle *h);
 bool pair_compatible(pf_handle *h0, pf_handle *h1);
 // step_ref(h0, ...) and step_ref(h1, ...) with with_estimate, in one launch
 // per stage on h0's stream (grid.y = filter); the frame counter `parity` of
-// each handle is read.  The caller orders h1's stream around it.
+// each handle is read.  The caller orders h1's stream around it.  Without
+// final_sums the estimates' block sums are left for the caller to add (the
+// arguments of estimate_final_args; k_estimate_final's order keeps the
+// result identical).
 int step_ref_pair(pf_handle *h0, pf_handle *h1, const float *Z0, const float *Z1, const uint32_t *uni0,
-                  const uint32_t *uni1, int parity, double *lml0, double *lml1);
+                  const uint32_t *uni1, int parity, double *lml0, double *lml1, bool final_sums = true);
+// The block sums of the handle's last fused estimate and where it goes.
+EstFinalArgs estimate_final_args(pf_handle *h);
 // Reads and clears the handle's status word (synchronises the handle's stream).
 int take_status(pf_handle *h, int *degenerate);
 

@@ -498,7 +498,9 @@ __device__ double pose2d_entry(const double (*e)[64], int i)
 // Pose output of both arms: e (one lane per row), the six angle functions
 // of each arm (one lane each), the 3 x 5 joint positions (one lane each),
 // then the TF-tree and 2-D point lists.
-__global__ void __launch_bounds__(64) k_pose(PoseArgs a)
+// x0, x1: the arms' estimates (model basis).  Every thread of the block
+// calls it.
+__device__ inline void pose_body(const PoseArgs &a, const double *x0, const double *x1)
 {
     __shared__ double s_e[2][64];
     __shared__ double s_tg[2][6];
@@ -507,7 +509,7 @@ __global__ void __launch_bounds__(64) k_pose(PoseArgs a)
     for (int i = threadIdx.x; i < 2 * a.npose; i += blockDim.x) {
         const int arm = i / a.npose, r = i - arm * a.npose;
         const ArmDev &A = arm ? a.arm[1] : a.arm[0];
-        s_e[arm][r] = project_row(A.hp, A.pm, A.xbar, A.D, r);
+        s_e[arm][r] = project_row(A.hp, A.pm, arm ? x1 : x0, A.D, r);
     }
     __syncthreads();
     if (threadIdx.x < 12) {
@@ -532,6 +534,61 @@ __global__ void __launch_bounds__(64) k_pose(PoseArgs a)
         }
     __syncthreads();
     if (threadIdx.x == 0) *a.frame = t + 1;
+}
+
+__global__ void __launch_bounds__(64) k_pose(PoseArgs a)
+{
+    pose_body(a, a.arm[0].xbar, a.arm[1].xbar);
+}
+
+// The arms' estimates from their block sums, then the pose: threads
+// 256 h .. 256 h + 255 add arm h's sums exactly as k_estimate_final does
+// (thread t takes blocks t, t + 256, ... in order, then the fixed tree of
+// block_sum256, then Ti / N), write the estimate and keep it in LDS for the
+// projection.  For arms of the same state dimension D (one launch instead of
+// the estimate launch and the pose launch).
+template <int D>
+__global__ void __launch_bounds__(512) k_pose_est(PoseArgs a, EstFinalArgs e0, EstFinalArgs e1)
+{
+    __shared__ double s_v[2][D][16][17];
+    __shared__ double s_run[2][D][16];
+    __shared__ double s_sum[2][D];
+    __shared__ double s_x[2][D];
+    const int h = threadIdx.x >> 8, t = threadIdx.x & 255;
+    const EstFinalArgs &e = h ? e1 : e0;
+    double v[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) v[i] = 0.0;
+    for (int b = t; b < e.nb; b += 256) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) v[i] += e.partial[(size_t)b * D + i];
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) s_v[h][i][t >> 4][t & 15] = v[i];
+    __syncthreads();
+    for (int q = t; q < D * 16; q += 256) {
+        const int i = q >> 4, k = q & 15;
+        double r = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) r += s_v[h][i][k][u];
+        s_run[h][i][k] = r;
+    }
+    __syncthreads();
+    if (t < D) {
+        double r = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) r += s_run[h][t][k];
+        s_sum[h][t] = r;
+    }
+    __syncthreads();
+    if (t < D) {
+        double r = 0.0;
+        for (int k = 0; k < D; ++k) r = fma(e.Ti[t * D + k], s_sum[h][k] / (double)e.N, r);
+        e.out[t] = r;
+        s_x[h][t] = r;
+    }
+    __syncthreads();
+    pose_body(a, s_x[0], s_x[1]);
 }
 
 __global__ void k_tr_set(int *p, int v)
@@ -650,7 +707,7 @@ int launch_blur(pf_tracker *tr, const uint8_t *src, const int *frame, uint8_t *d
     return launch_check("k_blur");
 }
 
-int launch_pose(pf_tracker *tr, int *frame, const pf_tracker_outputs &out)
+PoseArgs pose_args(pf_tracker *tr, int *frame, const pf_tracker_outputs &out)
 {
     PoseArgs pa{};
     pa.arm[0] = arm_dev(tr, 0);
@@ -665,8 +722,28 @@ int launch_pose(pf_tracker *tr, int *frame, const pf_tracker_outputs &out)
     pa.fy = tr->cfg.fy;
     pa.cx = tr->cfg.cx;
     pa.cy = tr->cfg.cy;
-    hipLaunchKernelGGL(k_pose, dim3(1), dim3(64), 0, tr->s[0], pa);
+    return pa;
+}
+
+int launch_pose(pf_tracker *tr, int *frame, const pf_tracker_outputs &out)
+{
+    hipLaunchKernelGGL(k_pose, dim3(1), dim3(64), 0, tr->s[0], pose_args(tr, frame, out));
     return launch_check("k_pose");
+}
+
+// Whether the pose kernel adds the estimates' block sums itself (paired arms
+// of a state dimension whose reduction fits its LDS).
+bool pose_adds_estimates(pf_tracker *tr) { return tr->D[0] == tr->D[1] && (tr->D[0] == 8 || tr->D[0] == 12); }
+
+int launch_pose_est(pf_tracker *tr, int *frame, const pf_tracker_outputs &out)
+{
+    const PoseArgs pa = pose_args(tr, frame, out);
+    const EstFinalArgs e0 = eng::estimate_final_args(tr->arm[0]), e1 = eng::estimate_final_args(tr->arm[1]);
+    if (tr->D[0] == 8)
+        hipLaunchKernelGGL((k_pose_est<8>), dim3(1), dim3(512), 0, tr->s[0], pa, e0, e1);
+    else
+        hipLaunchKernelGGL((k_pose_est<12>), dim3(1), dim3(512), 0, tr->s[0], pa, e0, e1);
+    return launch_check("k_pose_est");
 }
 
 // One frame of the pipeline; `parity` alternates from frame to frame (it
@@ -698,10 +775,11 @@ int enqueue_frame(pf_tracker *tr, int parity, int T, const uint8_t *lik, const i
         hipLaunchKernelGGL(k_prop_select_pair, dim3((tr->N + 255) / 256, 2), dim3(256), sel_lds, tr->s[0], p, sa[0],
                            sa[1], (const FrameRec *)tr->rec, tr->N);
         st = launch_check("k_prop_select_pair");
+        const bool fold = pose_adds_estimates(tr);
         if (!st) st = eng::step_ref_pair(tr->arm[0], tr->arm[1], tr->Z[0], tr->Z[1], u_arm[0], u_arm[1], parity,
-                                         lml[0], lml[1]);
+                                         lml[0], lml[1], !fold);
         if (st) return st;
-        return launch_pose(tr, tr->fctr, out);
+        return fold ? launch_pose_est(tr, tr->fctr, out) : launch_pose(tr, tr->fctr, out);
     }
     // otherwise each arm on its own stream, forked here and joined before the pose
     TR_TRY(hipEventRecord(tr->ev_fork[parity], tr->s[0]));
