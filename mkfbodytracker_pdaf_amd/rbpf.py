@@ -148,9 +148,13 @@ class Rbpf:
 
     # -------------------------------------------- device-resident sequences
     def run_candidates_dev(self, T, count_ptr, cand_ptr, uni_ptr, lml_ptr, flags=1):
+        """T device-resident frames (pf_run_candidates_dev).  flags: 1 replays
+        captured graphs of frames, 2 times every launch (no graph), 4 times
+        the whole run; last_kernel_time() reads the timing."""
         check(self.lib.pf_run_candidates_dev(self.h, int(T), count_ptr, cand_ptr, uni_ptr, lml_ptr, int(flags)))
 
     def run_ref_dev(self, T, Z_ptr, uni_ptr, lml_ptr, flags=1):
+        """As run_candidates_dev with one measurement column per particle."""
         check(self.lib.pf_run_ref_dev(self.h, int(T), Z_ptr, uni_ptr, lml_ptr, int(flags)))
 
     def synchronize(self):
