@@ -114,6 +114,23 @@ int pf_resample(pf_handle *h, const double *w, int32_t L, int32_t N, uint32_t u,
 /* Same on log-weights (the form the step kernel uses internally). */
 int pf_resample_log(pf_handle *h, const double *lw, int32_t L, int32_t N, uint32_t u, int32_t *out);
 
+/* Largest L accepted by pf_resample_log_noalloc. */
+#define PF_NOALLOC_MAX_WEIGHTS 524288
+/* No counterpart in the reference: pf_resample_log for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc).  It needs no handle and calls no hipMalloc / hipFree:
+ * its scratch is a static device array of 4.1 MiB per device, kept until the
+ * process ends.  lw (L FP64 log-weights) and anc (N int32, written) are
+ * DEVICE pointers on the calling thread's current device; stream is a
+ * hipStream_t (NULL = the null stream).  The work is enqueued and the call
+ * returns without waiting.  status (device int32, may be NULL) receives 0, or
+ * 1 if every weight was zero or non-finite (uniform fallback).  Same indices
+ * as pf_resample_log.  L > PF_NOALLOC_MAX_WEIGHTS is rejected (PF_ERR_ARG)
+ * before any launch.  Every call on a device shares the one scratch array:
+ * calls that overlap on one device (other streams, other threads) race. */
+int pf_resample_log_noalloc(const double *lw, int32_t L, int32_t N, uint32_t u, int32_t *anc,
+                            int32_t *status, void *stream);
+
 /* ParticleFilter::getSamples(H, M, n, scale) (pf2DRao.cpp:85-103): with
  * y = Hp xbar + M (Hp npose x d = pca_proj^T, M npose), out (2 x n) =
  * y[0:2] + 0.8*scale*normals (normals 2 x n, host-supplied). */

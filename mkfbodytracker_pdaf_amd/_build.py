@@ -11,7 +11,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.environ.get("MKFBT_LIB") or os.path.join(HERE, "libmkfbt.so")
-SOURCES = ["pf_capi.hip", "pf_model.cpp", "pf_io.cpp", "pf_host.cpp", "pf2d.hip", "pf_batch.hip", "pf_adjoint.hip", "pf_tracker.hip"]
+SOURCES = ["pf_capi.hip", "pf_model.cpp", "pf_io.cpp", "pf_host.cpp", "pf2d.hip", "pf_batch.hip", "pf_adjoint.hip", "pf_tracker.hip", "pf_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -20,13 +20,14 @@ PF_ERR_NOT_SPD = -3
 PF_ERR_STATE = -4
 PF_ERR_UNSUPPORTED = -5
 PF_ERR_IO = -6
+PF_NOALLOC_MAX_WEIGHTS = 524288   # include/pf_rbpf.h
 
 # Every symbol declared in include/pf_rbpf.h, pf2d.h, pf_batch.h and pf_tracker.h.
 EXPORTS = [
     "pf_config_default", "pf_last_error", "pf_create", "pf_destroy", "pf_load_model",
     "pf_set_joint_clutter", "pf_reset", "pf_set_tracks", "pf_get_tracks", "pf_step",
     "pf_step_candidates", "pf_estimate", "pf_resample", "pf_resample_log", "pf_get_samples",
-    "pf_get_sample_prob", "pf_run_ref_dev", "pf_run_candidates_dev", "pf_synchronize",
+    "pf_resample_log_noalloc", "pf_get_sample_prob", "pf_run_ref_dev", "pf_run_candidates_dev", "pf_synchronize",
     "pf_get_stream", "pf_last_run_kernel_time", "pf_kf_predict", "pf_kf_update",
     "pf_yml_read_matrix", "pf_read_camera_info", "pf_load_model_yml", "pf_philox_u32",
     "pf2d_create", "pf2d_destroy", "pf2d_load_gmm", "pf2d_init", "pf2d_set_particles",
@@ -106,6 +107,7 @@ def lib():
     _sig(L, "pf_estimate", C.c_int, _vp, _vp)
     _sig(L, "pf_resample", C.c_int, _vp, _vp, i32, i32, u32, _vp)
     _sig(L, "pf_resample_log", C.c_int, _vp, _vp, i32, i32, u32, _vp)
+    _sig(L, "pf_resample_log_noalloc", C.c_int, _vp, i32, i32, u32, _vp, _vp, _vp)
     _sig(L, "pf_get_samples", C.c_int, _vp, _vp, _vp, i32, i32, dbl, _vp, _vp)
     _sig(L, "pf_get_sample_prob", C.c_int, _vp, _vp, _vp, i32, _vp, _vp, i32, dbl, _vp, _vp)
     _sig(L, "pf_run_ref_dev", C.c_int, _vp, i32, _vp, _vp, _vp, i32)
