@@ -19,7 +19,9 @@ batched engine and reports the default and the autotuned schedule.
 against the forward pass.
 ``--workload scaled`` runs the 24-dimensional state: K=64, m=10 (5 joints,
 PDAF on each, 16 candidates, Poisson(10) clutter), 32 sequences x 31,250
-particles x 1000 frames through the batched engine (FP32 VALU).
+particles x 1000 frames through the batched engine, with the covariance tiles
+on the matrix cores (the line's value) beside the one-particle-per-lane and
+four-lane vector-ALU kernels.
 ``--workload tracker`` runs the pose tracker's whole frame for both arms
 (likelihood blur, 10 N hand proposals per arm, association weights, proposal
 resampling, measurements, both filter updates, pose output) with the shipped
@@ -137,14 +139,17 @@ def main():
     dt = time.perf_counter() - t0
     lml = d_lml.get()[W:]
 
+    graph_frames = rb.last_graph_frames()
+
     # kernel timing over the same frames, HIP events on the handle's stream:
-    # one pair around the graph-replayed frames (one k_step launch per
-    # frame), then a pair around every launch without the graph (each
-    # launch's own dispatch included), reported beside it
+    # one pair around the K frames of the run (one k_step launch per frame,
+    # replayed from graphs unless --no-graph), then a pair around every launch
+    # without graphs (each launch's own dispatch included), reported beside it
     rb.reset(bins)
     rb.run_candidates_dev(K, c_ptr, k_ptr, u_ptr, l_ptr, flags | 4)
     rb.synchronize()
     step_ms_total, _, launches = rb.last_kernel_time()
+    timed_graph_frames = rb.last_graph_frames()
     step_avg_s = step_ms_total / launches / 1e3
     rb.reset(bins)
     rb.run_candidates_dev(K, c_ptr, k_ptr, u_ptr, l_ptr, 2)
@@ -178,7 +183,7 @@ def main():
         "config": {"workload": "single-sequence throughput: K=15, d=12, m=6, N=%d, %d frames, "
                                "8 candidates per hand (1 true + Poisson(7) clutter), chi2 99%% gate" % (N, K),
                    "particles": N, "frames": K, "state_dim": mdl.d, "components": mdl.K,
-                   "graph": bool(flags & 1)},
+                   "graph": graph_frames == K, "graph_frames": graph_frames},
         "frames_per_s": K / dt,
         "lml_total": float(np.sum(lml)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -190,16 +195,29 @@ def main():
                      "algorithmic_bytes": bpp * N,
                      "kernel": "k_step<12,cand>", "bytes_per_particle": bpp,
                      "avg_launch_us": step_avg_s * 1e6, "launches": launches,
-                     "timing": "HIP events on the handle's stream around the %s frames" %
-                               ("graph-replayed" if flags & 1 else "launched"),
+                     "timing": "one HIP event pair on the handle's stream around the %d frames of a run "
+                               "(%d of them replayed from graphs): frame time, launch gaps included" %
+                               (launches, timed_graph_frames),
                      "avg_launch_us_no_graph": step_ms_ng / launches_ng * 1e3,
                      "frame_us_no_graph": frame_ms_total / launches_ng * 1e3},
     }
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mdl, seq, N, cap, lam, args.cpu_seconds)
+        out["cpu_baseline"]["all_cores"] = cpu_baseline(mdl, seq, N, cap, lam, args.cpu_seconds,
+                                                        threads=host_cores())
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        out["speedup_vs_cpu_all_cores"] = out["value"] / out["cpu_baseline"]["all_cores"]["value"]
     print(json.dumps(out))
+
+
+def host_cores() -> int:
+    """Cores this process may use, at most 16 (the CPU share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def measured_traffic(N, d, K):
@@ -423,16 +441,25 @@ def run_scaled(args):
     d_cand = DeviceArray(np.concatenate([q.cand for q in seqs]).astype(np.float32))
     d_uni = DeviceArray(np.concatenate([q.uniforms for q in seqs]).astype(np.uint32))
     d_lml = DeviceArray.empty(S * T, np.float64)
-    for rep in range(2):                          # warm-up run, then the timed run
-        for s in range(S):
-            b.reset(s, bins[s])
-        synchronize()
-        t0 = time.perf_counter()
-        b.run_dev(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, graph=not args.no_graph)
-        synchronize()
-        dt = time.perf_counter() - t0
+    from mkfbodytracker_pdaf_amd import capi
+    names = {capi.PF_BATCH_MFMA: "mfma", capi.PF_BATCH_QUAD_VALU: "quad_valu", capi.PF_BATCH_VALU: "valu"}
     evals = float(S) * N * T
     bpp = step_bytes_per_particle(d)
+    variants = {}
+    for v in (capi.PF_BATCH_VALU, capi.PF_BATCH_QUAD_VALU, capi.PF_BATCH_MFMA):
+        b.variant = v
+        for rep in range(2):                      # warm-up run, then the timed run
+            for s in range(S):
+                b.reset(s, bins[s])
+            synchronize()
+            t0 = time.perf_counter()
+            b.run_dev(d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, graph=not args.no_graph)
+            synchronize()
+            dt = time.perf_counter() - t0
+        variants[names[v]] = {"evals_per_s": evals / dt, "wall_s": dt, "device_s": b.last_run_ms() / 1e3,
+                              "ms_per_frame": dt / T * 1e3, "hbm_frac": bpp * evals / dt / 1e9 / HBM_PEAK_GBS,
+                              "lml_total": float(np.sum(d_lml.get()))}
+    dt = variants["mfma"]["wall_s"]
     achieved = bpp * evals / dt / 1e9
     out = {
         "metric": "likelihood evals/s, scaled state (d=24, K=64, 5 PDAF joints), batched",
@@ -448,13 +475,16 @@ def run_scaled(args):
         "dtype": "f32",
         "data": "synthetic (per-sequence seeded GMM K=64 d=24, pinhole-projected 5-joint pose, Philox uniforms)",
         "config": {"workload": "scaled state: %d sequences x %d particles x %d frames, K=64, d=24, m=10, "
-                               "16 candidates per joint (1 true + Poisson(10) clutter), FP32 VALU" % (S, N, T),
+                               "16 candidates per joint (1 true + Poisson(10) clutter), FP32 matrix cores "
+                               "(v_mfma_f32_4x4x1f32, four lanes per particle)" % (S, N, T),
                    "sequences": S, "particles": N, "frames": T, "state_dim": d, "components": K},
         "frames_per_s": T / dt,
-        "lml_total": float(np.sum(d_lml.get())),
+        "lml_total": variants["mfma"]["lml_total"],
+        "variants": variants,
+        "mfma_over_valu": variants["valu"]["wall_s"] / variants["mfma"]["wall_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_batch_step<24> + k_batch_weights (whole run)", "bytes_per_particle": bpp},
+                     "kernel": "k_batch_step_q24<5, mfma> + k_batch_weights (whole run)", "bytes_per_particle": bpp},
     }
     print(json.dumps(out))
 
@@ -574,12 +604,12 @@ def cpu_baseline_tracker(left, right, N, imgs, rois, budget_s, seed=21):
             "cpu": _cpu_model(), "host_nproc": os.cpu_count()}
 
 
-def cpu_baseline(mdl, seq, N, cap, lam, budget_s, seed=2024):
-    """FP64 oracle (1 thread) on the same workload: same model, the same
-    frames' detections and uniforms, N particles, as many frames as fit in the
-    budget (at least 2)."""
+def cpu_baseline(mdl, seq, N, cap, lam, budget_s, seed=2024, threads=1):
+    """FP64 oracle on the same workload: same model, the same frames'
+    detections and uniforms, N particles, as many frames of the sequence as fit
+    in the budget (at least 2); `threads` OpenMP threads over the particles."""
     import oracle
-    oracle.set_threads(1)
+    oracle.set_threads(threads)
     om = oracle.Model(mdl.weights, mdl.gamma, mdl.means, mdl.Sigma, mdl.H(), mdl.BH(), mdl.R)
     from mkfbodytracker_pdaf_amd import synthetic
     bins = synthetic.init_bins(mdl, N, seed, lambda w, n, U: oracle.resample_linear(w, n, U)[0])
@@ -596,9 +626,10 @@ def cpu_baseline(mdl, seq, N, cap, lam, budget_s, seed=2024):
         el = time.perf_counter() - t0
         if frames >= 2 and (el >= budget_s or frames >= seq.count.shape[0]):
             break
-    return {"value": N * frames / el, "unit": "evals/s", "cores": 1, "kind": "port",
-            "sample": f"{frames} frames x N={N} of the same sequence, FP64 oracle, 1 thread",
-            "cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+    oracle.set_threads(1)
+    return {"value": N * frames / el, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} frames x N={N} of the same sequence, FP64 oracle, {threads} thread(s)",
+            "seconds": el, "cpu": _cpu_model(), "host_nproc": os.cpu_count()}
 
 
 def _cpu_model():

@@ -50,9 +50,22 @@ int pf_batch_reset(pf_batch *b, int32_t s, const int32_t *bins);
  * concatenated in index order with frame offsets F_s = sum_{r<s} n_frames[r]:
  * uniforms (F x 2), count (F x J), cand (F x J x cap x 2 floats), lml (F,
  * written: log mean weight of every sequence-frame).  flags bit 0: capture
- * the frame loop into a hipGraph. */
+ * the frame loop into a hipGraph.  Waits for the run; returns its status
+ * (warning bits or PF_ERR_NOT_SPD as pf_step, over all sequences). */
 int pf_batch_run_dev(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uniforms,
                      double *lml, int32_t flags);
+/* Tracks in for sequence s (as pf_set_tracks: x N_s x d, P N_s x d x d,
+ * model basis); the next run starts from them. */
+int pf_batch_set_tracks(pf_batch *b, int32_t s, const double *x, const double *P);
+/* The last frame of sequence s after a run: its log-weights (lw, N_s) and
+ * the ancestors its resampling drew (anc, N_s); NULL to skip. */
+int pf_batch_get_last_frame(pf_batch *b, int32_t s, float *lw, int32_t *anc);
+/* Status of every sequence in the last run (S entries): 0, or the bits
+ * PF_WARN_DEGENERATE (a uniform-fallback resampling), PF_WARN_ILLCOND (an
+ * innovation factored in FP64) and 4 (an innovation covariance not positive
+ * definite: pf_batch_run_dev then returned PF_ERR_NOT_SPD).  The run itself
+ * returns the bits of all sequences together. */
+int pf_batch_get_status(pf_batch *b, int32_t *status);
 /* Tracks and estimate of sequence s after a run (as pf_get_tracks / pf_estimate). */
 int pf_batch_get_tracks(pf_batch *b, int32_t s, double *x, double *P);
 int pf_batch_estimate(pf_batch *b, int32_t s, double *xbar);
@@ -67,6 +80,24 @@ int pf_batch_set_schedule(pf_batch *b, const pf_batch_schedule *sch);
 int pf_batch_autotune(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uniforms,
                       double *lml, int32_t frames, pf_batch_schedule *best, double *best_ms,
                       double *default_ms);
+/* Step kernel of the 24-dimensional state (d = 24 only; d = 12 always runs
+ * one particle per lane):
+ *   PF_BATCH_VALU       one particle per lane, the 324 state values in its
+ *                       registers, FP32 vector ALU;
+ *   PF_BATCH_MFMA       four lanes per particle (16 particles per wave), the
+ *                       covariance as 21 upper-triangular 4 x 4 tiles whose
+ *                       rank-1 updates run on the matrix cores
+ *                       (v_mfma_f32_4x4x1f32: FP32 inputs, FP32 accumulation),
+ *                       PDAF candidates split over the four lanes (default);
+ *   PF_BATCH_QUAD_VALU  the MFMA variant's layout with the tile updates on
+ *                       the vector ALU (for comparison).
+ * All three run the algorithm of pf_step_candidates(); results agree to FP32
+ * rounding (the order of the sums differs). */
+#define PF_BATCH_VALU 0
+#define PF_BATCH_MFMA 1
+#define PF_BATCH_QUAD_VALU 2
+int pf_batch_set_variant(pf_batch *b, int32_t variant);
+int pf_batch_get_variant(pf_batch *b, int32_t *variant);
 /* Device time of the last pf_batch_run_dev in ms. */
 int pf_batch_last_run_ms(pf_batch *b, double *ms);
 

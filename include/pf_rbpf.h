@@ -8,10 +8,16 @@
  * different handles may be used concurrently.
  *
  * Conventions: plain C types only, no exceptions cross this boundary.  Every
- * entry point returns an int status: 0 = OK, 1 = OK but the resampler fell
- * back to uniform weights (every weight was zero or non-finite, the
- * reference's maxWeight()==0 branch), negative = error (see PF_ERR_*), with a
- * message available from pf_last_error() (thread-local).  Host matrices are
+ * entry point returns an int status: 0 = OK; positive = OK with warnings, a
+ * bit mask of PF_WARN_DEGENERATE (1: the resampler fell back to uniform
+ * weights because every weight was zero or non-finite, the reference's
+ * maxWeight()==0 branch) and PF_WARN_ILLCOND (2: some particle's 2 x 2
+ * innovation covariance had cond > ~1e6 and was factored in FP64); negative
+ * = error (see PF_ERR_*), with a message available from pf_last_error()
+ * (thread-local).  PF_ERR_NOT_SPD from a step means the frame completed but
+ * some particle's innovation covariance was not positive definite even in
+ * FP64: that particle got zero weight and kept its predicted state (the
+ * reference's Cholesky failure, src/pf2DRao.cpp:36-52).  Host matrices are
  * row-major double, i.e. the memory of a continuous CV_64F cv::Mat.  Device
  * state is FP32 structure-of-arrays with packed symmetric covariances;
  * log-likelihood accumulation is FP64.
@@ -32,6 +38,7 @@ extern "C" {
 
 #define PF_OK 0
 #define PF_WARN_DEGENERATE 1
+#define PF_WARN_ILLCOND 2
 #define PF_ERR_ARG (-1)
 #define PF_ERR_HIP (-2)
 #define PF_ERR_NOT_SPD (-3)
@@ -147,14 +154,20 @@ int pf_get_sample_prob(pf_handle *h, const double *Hp, const double *M, int32_t 
  * pointers on the handle's device: uniforms T x 2 (u_comp, u_res per frame),
  * lml T (FP64, written), and either Z (T x m x N floats, reference mode) or
  * count (T x J) + cand (T x J x cap x 2 floats, candidate mode).
- * use_graph bit 0 (value 1): the frames are captured into a hipGraph (64
- * at a time) and replayed; bits 1 and 2 time the run (see
- * pf_last_run_kernel_time; bit 1 turns the graph off).  Returns after
+ * use_graph bit 0 (value 1): every frame is replayed from hipGraphs of 64,
+ * 32, ..., 2, 1 frames (for either parity of their first frame) captured on
+ * the handle's first such run (the frames read the run's pointers from
+ * device memory, so the graphs serve every later run of any length; they
+ * are captured again after pf_load_model or pf_set_joint_clutter); bits 1
+ * and 2 time the run (see
+ * pf_last_run_kernel_time; bit 1 turns the graphs off).  Returns after
  * enqueueing; pf_synchronize waits. */
 int pf_run_ref_dev(pf_handle *h, int32_t T, const float *Z, const uint32_t *uniforms,
                    double *lml, int32_t use_graph);
 int pf_run_candidates_dev(pf_handle *h, int32_t T, const int32_t *count, const float *cand,
                           const uint32_t *uniforms, double *lml, int32_t use_graph);
+/* Waits for the handle's work and returns its status since the last step or
+ * synchronize (warnings and PF_ERR_NOT_SPD as for pf_step). */
 int pf_synchronize(pf_handle *h);
 /* The handle's HIP stream (hipStream_t) for callers that order their own work. */
 int pf_get_stream(pf_handle *h, void **stream);
@@ -166,6 +179,8 @@ int pf_get_stream(pf_handle *h, void **stream);
  * not); step_ms = frame_ms = that time, which is the step launches back to
  * back when a frame is one launch.  Without either bit nothing is timed. */
 int pf_last_run_kernel_time(pf_handle *h, double *step_ms, double *frame_ms, int32_t *launches);
+/* Frames of the last pf_run_*_dev call that were replayed from graphs (0 or T). */
+int pf_last_run_graph_frames(pf_handle *h, int32_t *frames);
 
 /* ---------------------------------------------------------- KF_model */
 /* KF_model::predict (KF_model.cpp:11-15) for n independent states sharing

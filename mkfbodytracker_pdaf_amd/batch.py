@@ -62,16 +62,44 @@ class RbpfBatch:
     def reset(self, s, bins):
         check(self.lib.pf_batch_reset(self.h, int(s), ptr(i32(bins))))
 
-    def run_dev(self, count_ptr, cand_ptr, uni_ptr, lml_ptr, graph: bool = True):
+    def run_dev(self, count_ptr, cand_ptr, uni_ptr, lml_ptr, graph: bool = True) -> int:
         """Device pointers to the concatenated per-sequence inputs (frame
         offsets ``frame_offsets``): count (F x J int32), cand (F x J x cap x 2
-        float32), uniforms (F x 2 uint32); lml (F float64) is written."""
-        check(self.lib.pf_batch_run_dev(self.h, count_ptr, cand_ptr, uni_ptr, lml_ptr, 1 if graph else 0))
+        float32), uniforms (F x 2 uint32); lml (F float64) is written.
+        Returns the run's warning bits (capi.PF_WARN_*)."""
+        return check(self.lib.pf_batch_run_dev(self.h, count_ptr, cand_ptr, uni_ptr, lml_ptr, 1 if graph else 0))
+
+    def status(self):
+        """Per-sequence status bits of the last run (capi.PF_WARN_* and 4 = not SPD)."""
+        out = np.zeros(self.S, np.int32)
+        check(self.lib.pf_batch_get_status(self.h, ptr(out)))
+        return out
 
     def last_run_ms(self) -> float:
         v = C.c_double(0.0)
         check(self.lib.pf_batch_last_run_ms(self.h, C.byref(v)))
         return v.value
+
+    def set_tracks(self, s, x, P):
+        check(self.lib.pf_batch_set_tracks(self.h, int(s), ptr(f64(x)), ptr(f64(P))))
+
+    def last_frame(self, s):
+        """(log-weights, ancestors) of sequence s's last frame after a run."""
+        n = int(self.N[s])
+        lw = np.empty(n, np.float32); anc = np.empty(n, np.int32)
+        check(self.lib.pf_batch_get_last_frame(self.h, int(s), ptr(lw), ptr(anc)))
+        return lw, anc
+
+    @property
+    def variant(self) -> int:
+        """Step kernel of the d = 24 state: capi.PF_BATCH_VALU, PF_BATCH_MFMA or PF_BATCH_QUAD_VALU."""
+        v = C.c_int32(0)
+        check(self.lib.pf_batch_get_variant(self.h, C.byref(v)))
+        return v.value
+
+    @variant.setter
+    def variant(self, v: int):
+        check(self.lib.pf_batch_set_variant(self.h, int(v)))
 
     def get_tracks(self, s):
         n = int(self.N[s])

@@ -14,12 +14,14 @@ from ._build import LIB
 
 PF_OK = 0
 PF_WARN_DEGENERATE = 1
+PF_WARN_ILLCOND = 2
 PF_ERR_ARG = -1
 PF_ERR_HIP = -2
 PF_ERR_NOT_SPD = -3
 PF_ERR_STATE = -4
 PF_ERR_UNSUPPORTED = -5
 PF_ERR_IO = -6
+PF_BATCH_VALU, PF_BATCH_MFMA, PF_BATCH_QUAD_VALU = 0, 1, 2   # include/pf_batch.h
 PF_NOALLOC_MAX_WEIGHTS = 524288   # include/pf_rbpf.h
 
 # Every symbol declared in include/pf_rbpf.h, pf2d.h, pf_batch.h and pf_tracker.h.
@@ -28,12 +30,14 @@ EXPORTS = [
     "pf_set_joint_clutter", "pf_reset", "pf_set_tracks", "pf_get_tracks", "pf_step",
     "pf_step_candidates", "pf_estimate", "pf_resample", "pf_resample_log", "pf_get_samples",
     "pf_resample_log_noalloc", "pf_get_sample_prob", "pf_run_ref_dev", "pf_run_candidates_dev", "pf_synchronize",
-    "pf_get_stream", "pf_last_run_kernel_time", "pf_kf_predict", "pf_kf_update",
+    "pf_get_stream", "pf_last_run_kernel_time", "pf_last_run_graph_frames", "pf_kf_predict", "pf_kf_update",
     "pf_yml_read_matrix", "pf_read_camera_info", "pf_load_model_yml", "pf_philox_u32",
-    "pf2d_create", "pf2d_destroy", "pf2d_load_gmm", "pf2d_init", "pf2d_set_particles",
-    "pf2d_get_particles", "pf2d_update", "pf2d_estimate",
+    "pf2d_create", "pf2d_destroy", "pf2d_load_gmm", "pf2d_load_gmm_precision", "pf2d_init", "pf2d_set_particles",
+    "pf2d_get_particles", "pf2d_update", "pf2d_predict", "pf2d_estimate",
     "pf_batch_create", "pf_batch_destroy", "pf_batch_load_model", "pf_batch_set_joint_clutter",
     "pf_batch_reset", "pf_batch_run_dev", "pf_batch_get_tracks", "pf_batch_estimate",
+    "pf_batch_set_tracks", "pf_batch_get_last_frame", "pf_batch_set_variant", "pf_batch_get_variant",
+    "pf_batch_get_status",
     "pf_batch_get_schedule", "pf_batch_set_schedule", "pf_batch_autotune", "pf_batch_last_run_ms",
     "pf_batch_lml_grad", "pf_batch_get_choices", "pf_batch_last_adjoint_ms",
     "pf_tracker_config_default", "pf_tracker_create", "pf_tracker_destroy", "pf_tracker_reset",
@@ -115,6 +119,7 @@ def lib():
     _sig(L, "pf_synchronize", C.c_int, _vp)
     _sig(L, "pf_get_stream", C.c_int, _vp, C.POINTER(_vp))
     _sig(L, "pf_last_run_kernel_time", C.c_int, _vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(i32))
+    _sig(L, "pf_last_run_graph_frames", C.c_int, _vp, C.POINTER(i32))
     _sig(L, "pf_kf_predict", C.c_int, _vp, i32, i32, _vp, _vp, _vp, _vp, _vp)
     _sig(L, "pf_kf_update", C.c_int, _vp, i32, i32, i32, _vp, _vp, _vp, _vp, _vp, _vp)
     _sig(L, "pf_yml_read_matrix", C.c_int, C.c_char_p, C.c_char_p, C.POINTER(i32), C.POINTER(i32), _vp, i64)
@@ -124,10 +129,12 @@ def lib():
     _sig(L, "pf2d_create", C.c_int, C.POINTER(_vp), i32, i32, i32, i32)
     _sig(L, "pf2d_destroy", C.c_int, _vp)
     _sig(L, "pf2d_load_gmm", C.c_int, _vp, i32, _vp, _vp, _vp)
+    _sig(L, "pf2d_load_gmm_precision", C.c_int, _vp, i32, _vp, _vp, _vp, _vp)
     _sig(L, "pf2d_init", C.c_int, _vp, _vp)
     _sig(L, "pf2d_set_particles", C.c_int, _vp, _vp, _vp)
     _sig(L, "pf2d_get_particles", C.c_int, _vp, _vp, _vp)
     _sig(L, "pf2d_update", C.c_int, _vp, _vp, u32, _vp, _vp, _vp, _vp)
+    _sig(L, "pf2d_predict", C.c_int, _vp, _vp)
     _sig(L, "pf2d_estimate", C.c_int, _vp, _vp)
     _sig(L, "pf_batch_create", C.c_int, C.POINTER(_vp), i32, _vp, _vp, i32, i32, i32, C.POINTER(Config))
     _sig(L, "pf_batch_destroy", C.c_int, _vp)
@@ -137,6 +144,11 @@ def lib():
     _sig(L, "pf_batch_run_dev", C.c_int, _vp, _vp, _vp, _vp, _vp, i32)
     _sig(L, "pf_batch_get_tracks", C.c_int, _vp, i32, _vp, _vp)
     _sig(L, "pf_batch_estimate", C.c_int, _vp, i32, _vp)
+    _sig(L, "pf_batch_set_tracks", C.c_int, _vp, i32, _vp, _vp)
+    _sig(L, "pf_batch_get_last_frame", C.c_int, _vp, i32, _vp, _vp)
+    _sig(L, "pf_batch_set_variant", C.c_int, _vp, i32)
+    _sig(L, "pf_batch_get_variant", C.c_int, _vp, C.POINTER(i32))
+    _sig(L, "pf_batch_get_status", C.c_int, _vp, _vp)
     _sig(L, "pf_batch_get_schedule", C.c_int, _vp, C.POINTER(Schedule))
     _sig(L, "pf_batch_set_schedule", C.c_int, _vp, C.POINTER(Schedule))
     _sig(L, "pf_batch_autotune", C.c_int, _vp, _vp, _vp, _vp, _vp, i32, C.POINTER(Schedule), C.POINTER(dbl),
@@ -170,8 +182,10 @@ def check(code: int) -> int:
 
 
 def ptr(a):
-    """Address of a numpy array (or None)."""
-    return None if a is None else a.ctypes.data
+    """Pointer argument for a numpy array (or None).  The returned ctypes
+    pointer holds a reference to the array, so a temporary such as
+    ptr(f64(x)) stays alive for the whole foreign call."""
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def f64(a):

@@ -172,8 +172,9 @@ __global__ void __launch_bounds__(kAdjTile) k_adj_replay(AdjArgs a)
     float x[D], P[NP];
     load_state(a.in_x, a.in_P, ba.ld, sd.pofs + src, x, P);
     const uint32_t U1 = ba.uni[(size_t)(sd.tofs + a.t) * 2];
+    unsigned flags = 0;   // the recorded forward run reported them
     (void)particle_update<D, kModeCand, SELJ>(ba.base, a.t, j, U1, reinterpret_cast<const float *>(sm4), x, P,
-                                              sd.pi_total, seq_comp_global(ba, sd), nullptr);
+                                              sd.pi_total, seq_comp_global(ba, sd), nullptr, flags);
     store_state(a.out_x, a.out_P, ba.ld, sd.pofs + j, x, P);
 }
 
@@ -612,6 +613,23 @@ int pf_batch_lml_grad(pf_batch *b, const int32_t *count, const float *cand, cons
         if ((st = launch_batch_finish(b, fa))) return st;
     }
     HIP_TRYB(hipEventRecord(e1, b->stream));
+    {
+        // The reverse sweep differentiates the FP32 factorization of every
+        // joint; a forward run that factored an innovation in FP64 (cond > ~1e6)
+        // or met a non-SPD one has no gradient here.
+        std::vector<unsigned> w((size_t)b->S);
+        HIP_TRYB(hipMemcpy(w.data(), b->status, sizeof(unsigned) * b->S, hipMemcpyDeviceToHost));
+        unsigned word = 0;
+        for (unsigned v : w) word |= v;
+        if (word & (kStatusIllCond | kStatusNotSpd)) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            (void)hipEventDestroy(e2);
+            return set_error(PF_ERR_UNSUPPORTED, "the forward run factored an innovation covariance in FP64 (cond > "
+                                                 "1e6) or met a non-SPD one; the adjoint differentiates the FP32 "
+                                                 "factorization only");
+        }
+    }
     // ---- backward
     const size_t nt = (size_t)b->t256_total * b->K * kGV;
     hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, b->stream, b->tacc, nt);

@@ -61,7 +61,7 @@ __device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc
         }
     }
     if (first && threadIdx.x == 0) {
-        if (total == 0ull) a.status[s] |= 1u;
+        if (total == 0ull) atomicOr(&a.status[s], kStatusDegenerate);
         if (lml_slot) *lml_slot = (total == 0ull) ? -INFINITY : pow_ref(E) + log(es) - log((double)sd.N);
     }
     (void)stats_parity;
@@ -131,28 +131,47 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
         const float *comp_global = seq_comp_global(a, sd);
         const uint32_t U1 = a.uni[(size_t)(sd.tofs + t) * 2];
         const __amdgpu_buffer_rsrc_t wx = soa_rsrc(xout, rowb * D), wP = soa_rsrc(Pout, rowb * NP);
+        unsigned pflags = 0;
 #pragma unroll
         for (int r = 0; r < PPL; ++r) {
             const int j = jt0 + r * bd + (int)threadIdx.x;
             if (j < sd.N) {
-                const float lwj = particle_update<D, kModeCand, SELJ>(a.base, t, j, U1,
-                                                                      reinterpret_cast<const float *>(sm4), x[r], P[r],
-                                                                      sd.pi_total, comp_global, comp_out);
+                unsigned f = 0;
+                float lwj = particle_update<D, kModeCand, SELJ>(a.base, t, j, U1, reinterpret_cast<const float *>(sm4),
+                                                                x[r], P[r], sd.pi_total, comp_global, comp_out, f);
                 const unsigned doff = (unsigned)(sd.pofs + j) * 4u;
-                if constexpr (D > 16) {
+                auto store = [&]() {
+                    if constexpr (D > 16) {
 #pragma unroll
-                    for (int i = 0; i < D; ++i) soa_st(wx, doff + rowb * i, 0, x[r][i]);
+                        for (int i = 0; i < D; ++i) soa_st(wx, doff + rowb * i, 0, x[r][i]);
 #pragma unroll
-                    for (int p = 0; p < NP; ++p) soa_st(wP, doff + rowb * p, 0, P[r][p]);
-                } else {
+                        for (int p = 0; p < NP; ++p) soa_st(wP, doff + rowb * p, 0, P[r][p]);
+                    } else {
 #pragma unroll
-                    for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[r][i]);
+                        for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[r][i]);
 #pragma unroll
-                    for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[r][p]);
+                        for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[r][p]);
+                    }
+                };
+                store();
+                if (f & kNeedFp64) {
+                    // redo this particle with the ill-conditioned joints in FP64
+                    f = 0;
+                    const unsigned voff = (unsigned)(sd.pofs + src[r]) * 4u;
+#pragma unroll
+                    for (int i = 0; i < D; ++i) x[r][i] = soa_ld(rx, voff + rowb * i, 0);
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) P[r][p] = soa_ld(rP, voff + rowb * p, 0);
+                    lwj = particle_update<D, kModeCand, SELJ, true>(a.base, t, j, U1,
+                                                                    reinterpret_cast<const float *>(sm4), x[r], P[r],
+                                                                    sd.pi_total, comp_global, comp_out, f);
+                    store();
                 }
+                pflags |= f;
                 a.lw[sd.pofs + j] = lwj;
             }
         }
+        report_flags(pflags, &a.status[s]);
     }
 }
 
@@ -210,6 +229,10 @@ __global__ void k_identity_seq(int *anc, int pofs, int N)
 
 
 }  // namespace
+
+// the four-lanes-per-particle step of the 24-dimensional state (uses
+// tile_ancestors above)
+#include "pf_quad24.hpp"
 
 namespace mkfbt {
 namespace batch {
@@ -385,6 +408,27 @@ int launch_batch_step(pf_batch *b, const BatchArgs &a)
     for (int s = 0; s < b->S; ++s) all_sel = all_sel && b->sel_ok[s];
     const int selj = all_sel ? b->J : 0;
     const int ppl = b->sch.ppl;
+    if (b->D == 24 && b->variant != PF_BATCH_VALU) {
+        // four lanes per particle, covariance tiles on the matrix cores (or
+        // the vector ALU); one 256-particle tile per block
+        if (b->sch.block != 256)
+            return set_error(PF_ERR_UNSUPPORTED, "the four-lane d = 24 step runs 256-thread tiles");
+        const size_t qlds = lds + 256 * sizeof(int);
+        const int grid = b->all_tiles;
+        const bool mf = b->variant == PF_BATCH_MFMA;
+        if (selj == 3 && mf)
+            hipLaunchKernelGGL((k_batch_step_q24<3, true>), dim3(grid), dim3(256), qlds, b->stream, a);
+        else if (selj == 3)
+            hipLaunchKernelGGL((k_batch_step_q24<3, false>), dim3(grid), dim3(256), qlds, b->stream, a);
+        else if (selj == 5 && mf)
+            hipLaunchKernelGGL((k_batch_step_q24<5, true>), dim3(grid), dim3(256), qlds, b->stream, a);
+        else if (selj == 5)
+            hipLaunchKernelGGL((k_batch_step_q24<5, false>), dim3(grid), dim3(256), qlds, b->stream, a);
+        else
+            return set_error(PF_ERR_UNSUPPORTED, "batched d = 24 is compiled for 3 or 5 joints with a full-rank H");
+        HIP_TRYB(hipGetLastError());
+        return PF_OK;
+    }
     if (b->D == 24) {              // scaled state: 3 or 5 joints in the selection basis, one particle per lane
         const int grid = b->all_tiles;
         if (selj == 3)
@@ -494,6 +538,20 @@ int run_all(pf_batch *b, const int32_t *count, const float *cand, const uint32_t
 }  // namespace mkfbt
 
 using namespace mkfbt::batch;
+
+namespace {
+
+// The C ABI status of the last run: the per-sequence status words ORed.
+int batch_status(pf_batch *b)
+{
+    std::vector<unsigned> w((size_t)b->S);
+    HIP_TRYB(hipMemcpy(w.data(), b->status, sizeof(unsigned) * b->S, hipMemcpyDeviceToHost));
+    unsigned word = 0;
+    for (unsigned v : w) word |= v;
+    return status_code(word);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -697,6 +755,17 @@ int pf_batch_run_dev(pf_batch *b, const int32_t *count, const float *cand, const
     HIP_TRYB(hipEventElapsedTime(&b->last_ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return batch_status(b);
+}
+
+int pf_batch_get_status(pf_batch *b, int32_t *status)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (!status) return set_error(PF_ERR_ARG, "null argument");
+    std::vector<unsigned> w((size_t)b->S);
+    HIP_TRYB(hipMemcpy(w.data(), b->status, sizeof(unsigned) * b->S, hipMemcpyDeviceToHost));
+    for (int s = 0; s < b->S; ++s) status[s] = (int32_t)(w[(size_t)b->rank[s]] & 7u);
     return PF_OK;
 }
 
@@ -737,6 +806,56 @@ int pf_batch_get_tracks(pf_batch *b, int32_t s, double *x, double *P)
     return PF_OK;
 }
 
+int pf_batch_set_tracks(pf_batch *b, int32_t s, const double *x, const double *P)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (s < 0 || s >= b->S || !x || !P) return set_error(PF_ERR_ARG, "bad set_tracks arguments");
+    for (int i = 0; i < b->S; ++i)
+        if (!b->loaded[i]) return set_error(PF_ERR_STATE, "load every sequence's model first");
+    const int i = b->rank[s], d = b->D;
+    const SeqDesc &sd = b->h_seqs[i];
+    const size_t nx = (size_t)sd.N * d, np = (size_t)sd.N * d * d;
+    if (b->gbuf_bytes < sizeof(double) * (nx + np)) {
+        (void)hipFree(b->gbuf);
+        b->gbuf = nullptr;
+        b->gbuf_bytes = 0;
+        HIP_TRYB(hipMalloc((void **)&b->gbuf, sizeof(double) * (nx + np)));
+        b->gbuf_bytes = sizeof(double) * (nx + np);
+    }
+    HIP_TRYB(hipMemcpyAsync(b->gbuf, x, sizeof(double) * nx, hipMemcpyHostToDevice, b->stream));
+    HIP_TRYB(hipMemcpyAsync(b->gbuf + nx, P, sizeof(double) * np, hipMemcpyHostToDevice, b->stream));
+    // x' = T x, P' = T P T^T into buffer 0 (frame 0 reads it), ancestors = identity
+    if (d == 24)
+        hipLaunchKernelGGL((k_import_rows<24>), dim3((unsigned)(((long long)sd.N * 24 + 255) / 256)), dim3(256), 0,
+                           b->stream, (const double *)b->gbuf, (const double *)(b->gbuf + nx),
+                           (const double *)(b->Ts + (size_t)i * d * d), b->bx[0] + sd.pofs, b->bP[0] + sd.pofs, sd.N,
+                           b->ld);
+    else
+        hipLaunchKernelGGL((k_import<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream,
+                           (const double *)b->gbuf, (const double *)(b->gbuf + nx),
+                           (const double *)(b->Ts + (size_t)i * d * d), b->bx[0] + sd.pofs, b->bP[0] + sd.pofs, sd.N,
+                           b->ld);
+    hipLaunchKernelGGL(k_identity_seq, dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->anc, sd.pofs, sd.N);
+    HIP_TRYB(hipGetLastError());
+    HIP_TRYB(hipStreamSynchronize(b->stream));
+    b->fresh[s] = 1;
+    b->tracks = std::all_of(b->fresh.begin(), b->fresh.end(), [](char c) { return c != 0; });
+    return PF_OK;
+}
+
+int pf_batch_get_last_frame(pf_batch *b, int32_t s, float *lw, int32_t *anc)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (s < 0 || s >= b->S) return set_error(PF_ERR_ARG, "sequence out of range");
+    const SeqDesc &sd = b->h_seqs[b->rank[s]];
+    if (lw) HIP_TRYB(hipMemcpyAsync(lw, b->lw + sd.pofs, sizeof(float) * sd.N, hipMemcpyDeviceToHost, b->stream));
+    if (anc) HIP_TRYB(hipMemcpyAsync(anc, b->anc + sd.pofs, sizeof(int) * sd.N, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRYB(hipStreamSynchronize(b->stream));
+    return PF_OK;
+}
+
 int pf_batch_estimate(pf_batch *b, int32_t s, double *xbar)
 {
     int st = bcheck(b);
@@ -772,8 +891,34 @@ int pf_batch_set_schedule(pf_batch *b, const pf_batch_schedule *sch)
         sch->blocks_per_cu < 1 || sch->blocks_per_cu > 8)
         return set_error(PF_ERR_ARG, "bad schedule");
     if (b->D == 24 && sch->ppl != 1) return set_error(PF_ERR_ARG, "d = 24 runs one particle per lane");
+    if (b->D == 24 && b->variant != PF_BATCH_VALU && sch->block != 256)
+        return set_error(PF_ERR_ARG, "the four-lane d = 24 variants run 256-thread tiles");
     b->sch = *sch;
     return build_tiles(b);
+}
+
+int pf_batch_set_variant(pf_batch *b, int32_t variant)
+{
+    int st = bcheck(b);
+    if (st) return st;
+    if (variant != PF_BATCH_VALU && variant != PF_BATCH_MFMA && variant != PF_BATCH_QUAD_VALU)
+        return set_error(PF_ERR_ARG, "unknown variant %d", variant);
+    if (variant != PF_BATCH_VALU && b->D != 24)
+        return set_error(PF_ERR_UNSUPPORTED, "the four-lane variants are the d = 24 state's");
+    if (variant != PF_BATCH_VALU && b->sch.block != 256) {
+        const pf_batch_schedule s256{256, 1, b->sch.blocks_per_cu};
+        st = pf_batch_set_schedule(b, &s256);
+        if (st) return st;
+    }
+    b->variant = variant;
+    return PF_OK;
+}
+
+int pf_batch_get_variant(pf_batch *b, int32_t *variant)
+{
+    if (!b || !variant) return set_error(PF_ERR_ARG, "null argument");
+    *variant = b->D == 24 ? b->variant : PF_BATCH_VALU;
+    return PF_OK;
 }
 
 int pf_batch_autotune(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uniforms, double *lml,
@@ -791,7 +936,8 @@ int pf_batch_autotune(pf_batch *b, const int32_t *count, const float *cand, cons
     for (int blk : {64, 128, 256})
         for (int ppl : {1, 2})
             for (int bpc : {1, 2, 3, 4})
-                if (b->D != 24 || ppl == 1) cands.push_back({blk, ppl, bpc});
+                if (b->D != 24 || (ppl == 1 && (b->variant == PF_BATCH_VALU || blk == 256)))
+                    cands.push_back({blk, ppl, bpc});
     hipEvent_t e0, e1;
     HIP_TRYB(hipEventCreate(&e0));
     HIP_TRYB(hipEventCreate(&e1));

@@ -134,6 +134,7 @@ struct pf_batch {
     int stage_floats() const { return table_floats + mkfbt::cand_lds_floats(J, cap); }
     bool tracks = false;
     pf_batch_schedule sch{256, 1, 2};
+    int variant = PF_BATCH_MFMA;   // kernel of the 24-dimensional state
     hipStream_t stream = nullptr;
     float *bx[2] = {nullptr, nullptr}, *bP[2] = {nullptr, nullptr};
     float *lw = nullptr, *tables = nullptr, *mus = nullptr, *sigs = nullptr;

@@ -36,6 +36,14 @@ int set_error(int code, const char *fmt, ...)
     return code;
 }
 
+int status_code(unsigned word)
+{
+    if (word & kStatusNotSpd)
+        return set_error(PF_ERR_NOT_SPD, "an innovation covariance was not positive definite even in FP64: those "
+                                         "particles got zero weight and kept their predicted state");
+    return (int)(word & (kStatusDegenerate | kStatusIllCond));
+}
+
 }  // namespace mkfbt
 
 #define HIP_TRY(expr)                                                                          \
@@ -93,6 +101,17 @@ struct pf_handle {
     std::vector<hipEvent_t> ev;
     int timed_launches = 0;
     double last_step_ms = 0.0, last_frame_ms = 0.0;
+    int graph_frames = 0;              // frames of the last run replayed from graphs
+    // device-resident runs: the state buffers by identity (xA/xB swap roles),
+    // the run's inputs and frame counters (frame t reads counter t & 1 and
+    // writes t + 1 to the other), and the captured frame groups per mode and
+    // parity of their first frame, group g holding 2^g frames (dropped when
+    // anything they bake in changes: model tables, resampler scratch)
+    float *xbuf[2] = {nullptr, nullptr}, *Pbuf[2] = {nullptr, nullptr};
+    RunDesc *run_desc = nullptr;
+    int *run_ctr = nullptr;
+    static constexpr int kGroups = 7;  // 1, 2, 4, ..., 64 frames
+    hipGraphExec_t groups[2][2][kGroups] = {};
 };
 
 namespace {
@@ -110,9 +129,22 @@ hipError_t dalloc(T **p, size_t n)
     return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1));
 }
 
+// Destroys the captured frame groups (they bake in device addresses).
+void drop_graphs(pf_handle *h)
+{
+    for (auto &per_mode : h->groups)
+        for (auto &per_parity : per_mode)
+            for (hipGraphExec_t &g : per_parity)
+                if (g) {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
+}
+
 int ensure_resample_scratch(pf_handle *h, int L)
 {
     if (L <= h->Lcap) return PF_OK;
+    drop_graphs(h);
     (void)hipFree(h->cloc);
     (void)hipFree(h->bsum);
     (void)hipFree(h->bexp);
@@ -167,10 +199,6 @@ int check_launch(const char *what)
 }
 
 // ---------------------------------------------------------- kernel dispatch
-// Frames per captured graph of the device-resident runs (even: frame f uses
-// scratch set and frame counter f & 1).
-constexpr int kGraphFrames = 64;
-
 template <int D, int SELJ>
 int launch_step_dj(pf_handle *h, const StepArgs &a, int mode, int block)
 {
@@ -396,12 +424,14 @@ void swap_state(pf_handle *h)
     std::swap(h->PA, h->PB);
 }
 
-int read_status(pf_handle *h, int *degenerate)
+// Waits for the handle's stream, then reads and clears its status word
+// (kStatus* bits raised by the device work since the last read).
+int read_status(pf_handle *h, unsigned *word)
 {
     FrameScratch fs;
     HIP_TRY(hipMemcpyAsync(&fs, h->fs, sizeof(fs), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    *degenerate = (fs.status & 1u) ? 1 : 0;
+    *word = fs.status;
     if (fs.status) {
         const unsigned int zero = 0;
         HIP_TRY(hipMemcpyAsync(&h->fs->status, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
@@ -415,6 +445,7 @@ int read_status(pf_handle *h, int *degenerate)
 // handle's model and uploads them.
 int build_tables(pf_handle *h)
 {
+    drop_graphs(h);
     ModelImage img;
     int st = make_model_image(h->hm, h->order_packed, h->N, img);
     if (st) return st;
@@ -628,7 +659,7 @@ int step_ref_pair(pf_handle *h0, pf_handle *h1, const float *Z0, const float *Z1
     return PF_OK;
 }
 
-int take_status(pf_handle *h, int *degenerate) { return read_status(h, degenerate); }
+int take_status(pf_handle *h, unsigned *word) { return read_status(h, word); }
 
 }  // namespace eng
 }  // namespace mkfbt
@@ -692,6 +723,8 @@ int pf_create(pf_handle **out, const pf_config *cfg)
     e = e ? e : dalloc(&h->lw, (size_t)h->ld);
     e = e ? e : dalloc(&h->est_out, 64);
     e = e ? e : dalloc(&h->est_partial, (size_t)((h->N + 255) / 256) * 64);
+    e = e ? e : dalloc(&h->run_desc, 1);
+    e = e ? e : dalloc(&h->run_ctr, 2);
     if (e == hipSuccess) e = hipMemsetAsync(h->fs, 0, sizeof(FrameScratch), h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
@@ -711,7 +744,8 @@ int pf_destroy(pf_handle *h)
     if (!h) return PF_OK;
     (void)hipSetDevice(h->cfg.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void *ptrs[] = {h->xA, h->PA, h->xB, h->PB, h->anc, h->comp, h->bins, h->lw, h->table,
+    drop_graphs(h);
+    void *ptrs[] = {h->run_desc, h->run_ctr, h->xA, h->PA, h->xB, h->PB, h->anc, h->comp, h->bins, h->lw, h->table,
                     h->mu, h->sig, h->fs, h->frame, h->uni, h->lml1, h->cloc, h->bsum, h->bexp, h->bpow,
                     h->cloc1, h->bsum1, h->bexp1, h->bpow1, h->zbuf, h->cntbuf, h->candbuf, h->est_partial, h->est_out, h->gbuf, h->ibuf,
                     h->Tdev, h->Tinv};
@@ -748,6 +782,8 @@ int pf_load_model(pf_handle *h, int32_t K, int32_t d, int32_t npose, int32_t m, 
     HIP_TRY(dalloc(&h->xB, (size_t)d * h->ld));
     HIP_TRY(dalloc(&h->PA, (size_t)NP * h->ld));
     HIP_TRY(dalloc(&h->PB, (size_t)NP * h->ld));
+    h->xbuf[0] = h->xA; h->xbuf[1] = h->xB;
+    h->Pbuf[0] = h->PA; h->Pbuf[1] = h->PB;
     h->clutter_mask = 0;
     h->order_packed = 0;
     for (int b = 0; b < J; ++b) h->order_packed |= (unsigned)b << (4 * b);
@@ -839,10 +875,10 @@ static int finish_host_step(pf_handle *h, float *lw, int32_t *anc, int32_t *comp
     if (anc) HIP_TRY(hipMemcpyAsync(anc, h->anc, sizeof(int) * h->N, hipMemcpyDeviceToHost, h->stream));
     if (comp) HIP_TRY(hipMemcpyAsync(comp, h->comp, sizeof(int) * h->N, hipMemcpyDeviceToHost, h->stream));
     if (lml) HIP_TRY(hipMemcpyAsync(lml, h->lml1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    int deg = 0;
-    st = read_status(h, &deg);
+    unsigned word = 0;
+    st = read_status(h, &word);
     if (st) return st;
-    return deg ? PF_WARN_DEGENERATE : PF_OK;
+    return status_code(word);
 }
 
 int pf_step(pf_handle *h, const double *Z, uint32_t u_comp, uint32_t u_res, float *lw, int32_t *anc,
@@ -952,10 +988,10 @@ static int resample_generic(pf_handle *h, const double *w, int L, int N, uint32_
     st = launch_ancestors(h, L, N, dout, 0, kind == kLinF64, nullptr);
     if (st) return st;
     HIP_TRY(hipMemcpyAsync(out, dout, sizeof(int) * (size_t)N, hipMemcpyDeviceToHost, h->stream));
-    int deg = 0;
-    st = read_status(h, &deg);
+    unsigned word = 0;
+    st = read_status(h, &word);
     if (st) return st;
-    return deg ? PF_WARN_DEGENERATE : PF_OK;
+    return status_code(word);
 }
 
 int pf_resample(pf_handle *h, const double *w, int32_t L, int32_t N, uint32_t u, int32_t *out)
@@ -1028,6 +1064,107 @@ int pf_get_sample_prob(pf_handle *h, const double *Hp, const double *M, int32_t 
 }
 
 // ------------------------------------------------ device-resident sequences
+namespace {
+
+// The step of a device-resident run for frames of parity p (the frame
+// index read from counter p, everything else chosen in the kernel from it;
+// see StepArgs::run).
+StepArgs run_step_args(pf_handle *h, int mode, int p)
+{
+    StepArgs a = step_args(h, mode, nullptr, nullptr, (long long)h->m * h->N, h->N, nullptr, nullptr, nullptr, p,
+                           kAncFused, nullptr);
+    a.run = h->run_desc;
+    a.frame = h->run_ctr + p;
+    a.frame_next = h->run_ctr + (1 - p);
+    a.xin = h->xbuf[0];
+    a.Pin = h->Pbuf[0];
+    a.xout = h->xbuf[1];
+    a.Pout = h->Pbuf[1];
+    a.pend = pending_of(h, h->N, true, 1);     // even frames resolve set 1
+    a.pend1 = pending_of(h, h->N, true, 0);    // odd frames resolve set 0
+    a.w_cloc = h->cloc;
+    a.w_bsum = h->bsum;
+    a.w_bexp = h->bexp;
+    a.w_bpow = h->bpow;
+    a.w1_cloc = h->cloc1;
+    a.w1_bsum = h->bsum1;
+    a.w1_bexp = h->bexp1;
+    a.w1_bpow = h->bpow1;
+    return a;
+}
+
+int launch_run_step(pf_handle *h, int mode, int p)
+{
+    return launch_step_block(h, run_step_args(h, mode, p), mode, kWeightsBlock);
+}
+
+// Group g of the run frames starting at a frame of parity p: 2^g step
+// launches, captured once per handle, mode and parity.
+int run_group(pf_handle *h, int mode, int p, int g, hipGraphExec_t *out)
+{
+    hipGraphExec_t &ge = h->groups[mode][p][g];
+    if (!ge) {
+        hipGraph_t graph;
+        HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int sc = 0;
+        for (int r = 0; r < (1 << g) && !sc; ++r) sc = launch_run_step(h, mode, (p + r) & 1);
+        hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+        if (sc) return sc;
+        if (ec != hipSuccess) return set_error(PF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        ec = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ec != hipSuccess) {
+            ge = nullptr;
+            return set_error(PF_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ec));
+        }
+    }
+    *out = ge;
+    return PF_OK;
+}
+
+int ensure_events(pf_handle *h, size_t need)
+{
+    while (h->ev.size() < need) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        h->ev.push_back(e);
+    }
+    return PF_OK;
+}
+
+// Frames whose model image and resolver workspace do not fit one workgroup's
+// LDS: step, weights and ancestor launches per frame, launched one by one.
+int run_frames_split(pf_handle *h, int T, int mode, const float *Z, const int *count, const float *cand,
+                     const uint32_t *uniforms, double *lml, bool timing, bool run_timing)
+{
+    const long long zstride = (long long)h->m * h->N;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
+    int st = check_launch("k_set_int");
+    if (st) return st;
+    if (run_timing) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+    if (timing) HIP_TRY(hipEventRecord(h->ev[2 * T], h->stream));
+    for (int f = 0; f < T; ++f) {
+        const int parity = f & 1;
+        StepArgs a = step_args(h, mode, uniforms, Z, zstride, h->N, count, cand, nullptr, parity, kAncArray, lml);
+        if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f], h->stream));
+        st = launch_step(h, a, mode);
+        if (st) return st;
+        if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f + 1], h->stream));
+        st = launch_weights(h, uniforms, parity);
+        if (st) return st;
+        // counter f & 1 still holds this frame's index (k_weights wrote the
+        // next one into the other counter)
+        st = launch_ancestors(h, h->N, h->N, h->anc, parity, 0, lml, h->frame + parity);
+        if (st) return st;
+        swap_state(h);
+    }
+    if (timing) HIP_TRY(hipEventRecord(h->ev[2 * T + 1], h->stream));
+    if (run_timing) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+    return PF_OK;
+}
+
+}  // namespace
+
 static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *count,
                       const float *cand, const uint32_t *uniforms, double *lml, int flags)
 {
@@ -1038,108 +1175,72 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
     const bool timing = (flags & 2) != 0;                 // an event pair around every launch (no graph)
     const bool run_timing = (flags & 4) != 0 && !timing;  // one event pair around all frames
     const bool graph = (flags & 1) != 0 && !timing;
-    const long long zstride = (long long)h->m * h->N;
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, h->frame, 0);
-    st = check_launch("k_set_int");
+    h->graph_frames = 0;
+    st = ensure_events(h, timing ? 2 * (size_t)T + 2 : 2);
     if (st) return st;
-    if (timing || run_timing) {
-        const size_t need = timing ? 2 * (size_t)T + 2 : 2;
-        while (h->ev.size() < need) {
-            hipEvent_t e;
-            HIP_TRY(hipEventCreate(&e));
-            h->ev.push_back(e);
-        }
-    }
-    // Frame f reads frame counter f & 1 and writes the other one.  Frame 0
-    // reads the materialised ancestor array; later frames resolve their
-    // ancestors from the previous frame's weights inside the step kernel,
-    // which also writes its own frame's resampler weights (scratch set f & 1;
-    // frame f resolves from set (f - 1) & 1), so a frame is one launch.
-    // The fused ancestor resolution needs its LDS workspace next to the model
-    // image; if both do not fit one workgroup, the weights and ancestors run
-    // as separate launches each frame.
+    // A frame is one step launch when the step's LDS holds the model image,
+    // the frame's detections and the resolver's workspace: it resolves its
+    // ancestors from the previous frame's resampler weights and writes its
+    // own.  Otherwise the frame runs as separate launches, without graphs.
     const int nbp = (h->N + kWeightsBlock - 1) / kWeightsBlock;
     const int stage = h->table_floats + (mode == kModeCand ? cand_lds_floats(h->J, h->cap) : 0);
     const bool fused = step_lds_bytes(stage, nbp) <= kMaxLdsBytes;
-    if (fused) {
+    if (!fused) {
+        st = run_frames_split(h, T, mode, Z, count, cand, uniforms, lml, timing, run_timing);
+    } else {
         st = ensure_run_scratch(h);
         if (st) return st;
-        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->stream, &h->fs->pow_max[0], kPowEmpty);
-        st = check_launch("k_set_int");
+        RunDesc v{};
+        v.Z = Z;
+        v.count = count;
+        v.cand = cand;
+        v.uni = uniforms;
+        v.lml = lml;
+        v.flip = (h->xA == h->xbuf[1]) ? 1 : 0;
+        hipLaunchKernelGGL(k_run_begin, dim3(1), dim3(64), 0, h->stream, h->run_desc, v, h->run_ctr, h->fs);
+        st = check_launch("k_run_begin");
         if (st) return st;
+        if (graph) {
+            // every group is captured before the first frame is enqueued
+            hipGraphExec_t ge[2][pf_handle::kGroups];
+            for (int p = 0; p < 2; ++p)
+                for (int g = 0; g < pf_handle::kGroups; ++g) {
+                    st = run_group(h, mode, p, g, &ge[p][g]);
+                    if (st) return st;
+                }
+            if (run_timing) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+            int f = 0;
+            for (int g = pf_handle::kGroups - 1; g >= 0; --g)
+                while (T - f >= (1 << g)) {
+                    HIP_TRY(hipGraphLaunch(ge[f & 1][g], h->stream));
+                    f += 1 << g;
+                }
+            h->graph_frames = T;
+            if (run_timing) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+        } else {
+            if (run_timing) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+            if (timing) HIP_TRY(hipEventRecord(h->ev[2 * T], h->stream));
+            for (int f = 0; f < T; ++f) {
+                if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f], h->stream));
+                st = launch_run_step(h, mode, f & 1);
+                if (st) return st;
+                if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f + 1], h->stream));
+            }
+            if (timing) HIP_TRY(hipEventRecord(h->ev[2 * T + 1], h->stream));
+            if (run_timing) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+        }
+        // frame t wrote buffer 1 - ((t + flip) & 1): the tracks now live in
+        // buffer (T + flip) & 1
+        if (T & 1) swap_state(h);
+        // the last frame's ancestors and log mean weight
+        st = launch_ancestors(h, h->N, h->N, h->anc, (T - 1) & 1, 0, lml + (T - 1), nullptr, (T - 1) & 1);
     }
-    auto one_frame = [&](int f) -> int {
-        const int parity = f & 1;
-        const int amode = (f == 0 || !fused) ? kAncArray : kAncFused;
-        StepArgs a = step_args(h, mode, uniforms, Z, zstride, h->N, count, cand, nullptr, parity, amode, lml);
-        if (fused) {
-            a.pend = pending_of(h, h->N, true, parity ^ 1);
-            a.w_cloc = parity ? h->cloc1 : h->cloc;
-            a.w_bsum = parity ? h->bsum1 : h->bsum;
-            a.w_bexp = parity ? h->bexp1 : h->bexp;
-            a.w_bpow = parity ? h->bpow1 : h->bpow;
-            a.frame_next = h->frame + (1 - parity);
-        }
-        if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f], h->stream));
-        // the fused step's blocks are the resampler's blocks
-        int s = fused ? launch_step_block(h, a, mode, kWeightsBlock) : launch_step(h, a, mode);
-        if (s) return s;
-        if (timing) HIP_TRY(hipEventRecord(h->ev[2 * f + 1], h->stream));
-        if (!fused) s = launch_weights(h, uniforms, parity);
-        if (s) return s;
-        if (!fused && f + 1 < T) {
-            // counter f & 1 still holds this frame's index (k_weights wrote
-            // the next one into the other counter)
-            s = launch_ancestors(h, h->N, h->N, h->anc, parity, 0, lml, h->frame + parity);
-            if (s) return s;
-        }
-        swap_state(h);
-        return PF_OK;
-    };
-    if (run_timing) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
-    if (!graph) {
-        if (timing) HIP_TRY(hipEventRecord(h->ev[2 * T], h->stream));
-        for (int f = 0; f < T; ++f) {
-            st = one_frame(f);
-            if (st) return st;
-        }
-        if (timing) HIP_TRY(hipEventRecord(h->ev[2 * T + 1], h->stream));
-    } else {
-        st = one_frame(0);
-        if (st) return st;
-        // Frames 1..T-1: kGraphFrames consecutive frames (an even count, so
-        // the buffers and counters are back in place) captured once and
-        // replayed; the remaining frames are launched one by one.
-        const int reps = (T - 1) / kGraphFrames;
-        if (reps > 0) {
-            hipGraph_t g;
-            hipGraphExec_t ge;
-            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            int sc = 0;
-            for (int f = 1; f <= kGraphFrames && !sc; ++f) sc = one_frame(f);
-            hipError_t ec = hipStreamEndCapture(h->stream, &g);
-            if (sc) return sc;
-            if (ec != hipSuccess) return set_error(PF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
-            HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-            for (int r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(ge, h->stream));
-            HIP_TRY(hipStreamSynchronize(h->stream));
-            (void)hipGraphExecDestroy(ge);
-            (void)hipGraphDestroy(g);
-        }
-        for (int f = 1 + reps * kGraphFrames; f < T; ++f) {
-            st = one_frame(f);
-            if (st) return st;
-        }
-    }
-    if (run_timing) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
-    // Materialise the last frame's ancestors and its log mean weight.
-    st = launch_ancestors(h, h->N, h->N, h->anc, (T - 1) & 1, 0, lml + (T - 1), nullptr, fused ? (T - 1) & 1 : 0);
     if (st) return st;
     if (run_timing) {
         HIP_TRY(hipEventSynchronize(h->ev[1]));
         float tot = 0.f;
         HIP_TRY(hipEventElapsedTime(&tot, h->ev[0], h->ev[1]));
-        h->last_step_ms = tot;            // one step launch per frame when fused
+        h->last_step_ms = tot;            // the step launches back to back when a frame is one launch
         h->last_frame_ms = tot;
         h->timed_launches = T;
     }
@@ -1178,8 +1279,10 @@ int pf_synchronize(pf_handle *h)
 {
     int st = check_handle(h);
     if (st) return st;
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return PF_OK;
+    unsigned word = 0;
+    st = read_status(h, &word);
+    if (st) return st;
+    return status_code(word);
 }
 
 int pf_get_stream(pf_handle *h, void **stream)
@@ -1195,6 +1298,13 @@ int pf_last_run_kernel_time(pf_handle *h, double *step_ms, double *frame_ms, int
     if (step_ms) *step_ms = h->last_step_ms;
     if (frame_ms) *frame_ms = h->last_frame_ms;
     if (launches) *launches = h->timed_launches;
+    return PF_OK;
+}
+
+int pf_last_run_graph_frames(pf_handle *h, int32_t *frames)
+{
+    if (!h || !frames) return set_error(PF_ERR_ARG, "null argument");
+    *frames = h->graph_frames;
     return PF_OK;
 }
 

@@ -118,12 +118,40 @@ __host__ __device__ inline u128 sys_target(uint32_t i, uint32_t U, uint64_t tota
 }
 
 // Model image of the step kernels: [component thresholds, 4K floats][H
-// (m x D), BH (m), R 2x2 blocks (3J), padded to 4][component rows,
-// K x (2 + D + D(D+1)/2)].  Floats of the part before the component rows:
+// (m x D), BH (m), R 2x2 blocks (3J), padded to even][per joint the FP64
+// innovation block (bh0, bh1, r00, r01, r11) as doubles (5J), padded to 4]
+// [component rows, K x (2 + D + D(D+1)/2)].  Offset of the FP64 blocks (8-byte
+// aligned) and floats of the part before the component rows:
+__host__ __device__ constexpr int image_j64_floats(int K, int m, int D, int J)
+{
+    return 4 * K + ((m * D + m + 3 * J + 1) & ~1);
+}
 __host__ __device__ constexpr int image_head_floats(int K, int m, int D, int J)
 {
-    return 4 * K + ((m * D + m + 3 * J + 3) & ~3);
+    return 4 * K + ((((m * D + m + 3 * J + 1) & ~1) + 10 * J + 3) & ~3);
 }
+
+// A 2 x 2 innovation covariance S = [[s00, s01], [s01, s11]] whose FP32
+// Cholesky is not trusted: a pivot that is not positive, or
+// det S <= 1e-6 (tr S)^2, i.e. cond(S) >= ~1e6 (for a 2 x 2 SPD matrix
+// det / tr^2 = c / (1 + c)^2 with c = 1 / cond).  Such joints are factored in FP64.
+__host__ __device__ inline bool innovation_ill(float s00, float s01, float s11)
+{
+    const float tr = s00 + s11;
+    const float det = fmaf(s00, s11, -s01 * s01);
+    return !(s00 > 0.f) || !(det > 1e-6f * tr * tr);
+}
+
+// Status bits of the device work (FrameScratch::status and the batched
+// engine's per-sequence status): the uniform resampling fallback (no positive
+// weight), a joint factored in FP64, a joint whose innovation covariance is
+// not positive definite even in FP64 (the particle's weight is zero).
+constexpr unsigned kStatusDegenerate = 1u;
+constexpr unsigned kStatusIllCond = 2u;
+constexpr unsigned kStatusNotSpd = 4u;
+// Per-particle flag of a step's FP32 pass: a joint needs the FP64 factor
+// (the particle is redone; never reported).
+constexpr unsigned kNeedFp64 = 0x100u;
 // LDS budget of the staged image: above it only the head is staged and the
 // component rows are read from global memory.
 constexpr int kImageLdsBytes = 48 * 1024;

@@ -57,8 +57,9 @@ int step_ref_pair(pf_handle *h0, pf_handle *h1, const float *Z0, const float *Z1
                   const uint32_t *uni1, int parity, double *lml0, double *lml1, bool final_sums = true);
 // The block sums of the handle's last fused estimate and where it goes.
 EstFinalArgs estimate_final_args(pf_handle *h);
-// Reads and clears the handle's status word (synchronises the handle's stream).
-int take_status(pf_handle *h, int *degenerate);
+// Reads and clears the handle's status word (kStatus* bits; synchronises
+// the handle's stream).
+int take_status(pf_handle *h, unsigned *word);
 
 }  // namespace eng
 }  // namespace mkfbt

@@ -10,6 +10,10 @@ namespace mkfbt {
 
 // Records the message for pf_last_error() and returns code.
 int set_error(int code, const char *fmt, ...);
+// The C ABI status of a device status word (kStatus* bits of pf_common.hpp):
+// PF_ERR_NOT_SPD (message set) if an innovation covariance was not positive
+// definite, else the warning bits PF_WARN_DEGENERATE | PF_WARN_ILLCOND.
+int status_code(unsigned word);
 
 // Philox4x32-10 (Salmon et al., SC'11) stream of 32-bit words.
 void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -62,6 +62,19 @@ struct Pending {
     int stage = kStageBlocks * kWeightsBlock;    // LDS elements of the staged search
 };
 
+// Inputs of a device-resident run, written to device memory once per run
+// (k_run_begin), so that the frames captured into a graph do not depend on
+// the run's buffers and one captured graph serves every run of the handle.
+struct RunDesc {
+    const float *Z;          // reference mode: Z[t][m][N]
+    const int *count;        // candidate mode: count[t][J]
+    const float *cand;       //                 cand[t][J][cap][2]
+    const uint32_t *uni;     // uniforms[t][2]: U_comp, U_res
+    double *lml;             // lml[t], the frame's log mean weight
+    int flip;                // state buffer read by frame 0 (0: xin/Pin, 1: xout/Pout)
+    int pad;
+};
+
 struct StepArgs {
     const float *__restrict__ xin;
     const float *__restrict__ Pin;
@@ -101,7 +114,64 @@ struct StepArgs {
     double *w_bexp;
     int *w_bpow;
     int *frame_next;
+    // Device-resident runs (run != null): the frame t = *frame (block 0 writes
+    // t + 1 to *frame_next) chooses everything else in the kernel: the state
+    // buffer read is (t + run->flip) & 1 (xin/Pin = buffer 0, xout/Pout =
+    // buffer 1) and the other is written; frame t writes its resampler
+    // weights to set t & 1 (w_* = set 0, w1_* = set 1) and resolves frame
+    // t - 1's from the other set (pend views set 1, pend1 set 0); frame 0
+    // reads the ancestor array.  Inputs come from *run.
+    const RunDesc *run;
+    Pending pend1;
+    unsigned long long *w1_cloc, *w1_bsum;
+    double *w1_bexp;
+    int *w1_bpow;
 };
+
+// The StepArgs of frame t of a device-resident run (see StepArgs::run), with
+// t returned; outside runs a itself and t = *a.frame.
+__device__ __forceinline__ StepArgs frame_args(const StepArgs &ain, int &t)
+{
+    StepArgs a = ain;
+    t = *a.frame;
+    if (!a.run) return a;
+    const RunDesc rd = *a.run;
+    if ((t + rd.flip) & 1) {
+        const float *x0 = a.xin, *P0 = a.Pin;
+        a.xin = a.xout;
+        a.Pin = a.Pout;
+        a.xout = const_cast<float *>(x0);
+        a.Pout = const_cast<float *>(P0);
+    }
+    if (t & 1) {
+        a.w_cloc = a.w1_cloc;
+        a.w_bsum = a.w1_bsum;
+        a.w_bexp = a.w1_bexp;
+        a.w_bpow = a.w1_bpow;
+        a.pend = a.pend1;
+    }
+    a.anc_mode = (t == 0) ? kAncArray : kAncFused;
+    a.parity = t & 1;
+    a.Z = rd.Z;
+    a.count = rd.count;
+    a.cand = rd.cand;
+    a.uni = rd.uni;
+    a.lml = rd.lml;
+    return a;
+}
+
+// Start of a device-resident run: the run's inputs, frame 0 in the counter
+// that even frames read, and the exponent slot of frame 0 cleared (the
+// frames raise it).
+static __global__ void k_run_begin(RunDesc *d, RunDesc v, int *ctr, FrameScratch *fs)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *d = v;
+        ctr[0] = 0;
+        ctr[1] = 0;
+        fs->pow_max[0] = kPowEmpty;
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
@@ -338,7 +408,7 @@ __device__ __forceinline__ int resolve_ancestor(const Pending &pd, int N, uint32
 __device__ __forceinline__ void finish_resampling(FrameScratch *fs, int parity, double bexp_sum, const ResolveStats &st,
                                                   int N, int kind_linear, double *lml_slot)
 {
-    if (st.total == 0ull) fs->status |= 1u;
+    if (st.total == 0ull) atomicOr(&fs->status, kStatusDegenerate);
     if (lml_slot) {
         double l;
         if (st.total == 0ull) l = -INFINITY;
@@ -406,8 +476,10 @@ template <int D>
 struct Joint {
     static constexpr int NP = npacked(D);
 
-    // W = L^-1 H_b P (2 x D), y = H_b x + BH_b, S = L L^T.
-    __device__ static __forceinline__ void prepare(const float (&x)[D], const float (&P)[NP],
+    // W = L^-1 H_b P (2 x D), y = H_b x + BH_b, S = L L^T.  Returns false
+    // when S is ill-conditioned for FP32 (innovation_ill; the outputs are then
+    // not used: the joint is redone in FP64).
+    __device__ static __forceinline__ bool prepare(const float (&x)[D], const float (&P)[NP],
                                                    const float *h0, const float *h1, float bh0,
                                                    float bh1, float r00, float r01, float r11,
                                                    float (&w0)[D], float (&w1)[D], float &y0,
@@ -449,11 +521,12 @@ struct Joint {
             w0[c] = hp0[c] * il00;
             w1[c] = (hp1[c] - l10 * w0[c]) * il11;
         }
+        return !innovation_ill(s00, s01, s11);
     }
 
     // Same quantities when the state basis makes H_b the selection of rows
     // r0, r0 + 1: H_b P is the pair of rows of P, y the pair of entries of x.
-    __device__ static __forceinline__ void prepare_sel(const float (&x)[D], const float (&P)[NP], int r0,
+    __device__ static __forceinline__ bool prepare_sel(const float (&x)[D], const float (&P)[NP], int r0,
                                                        float bh0, float bh1, float r00, float r01, float r11,
                                                        float (&w0)[D], float (&w1)[D], float &y0, float &y1,
                                                        float &il00, float &l10, float &il11, float &logl)
@@ -473,6 +546,7 @@ struct Joint {
             w0[c] = P[pk(D, r0, c)] * il00;
             w1[c] = (P[pk(D, r1, c)] - l10 * w0[c]) * il11;
         }
+        return !innovation_ill(s00, s01, s11);
     }
 
     // x += W^T v ; P -= W^T M W with M = [[m00, m01], [m01, m11]].
@@ -644,14 +718,35 @@ __device__ __forceinline__ bool joint_inputs(const StepArgs &a, int t, int b, in
     return true;
 }
 
+// Whitening of an innovation, u = L^-1 (z - y) with S = L L^T, L = [[l00,
+// 0], [l10, l11]]: in FP32 from the FP32 factor, or in FP64 from the FP64
+// factor of an ill-conditioned S (the cancellation is in this step; the
+// whitened values are well scaled and rounded to FP32).
+struct Whiten32 {
+    float y0, y1, il00, l10, il11;
+    __device__ __forceinline__ void operator()(float z0, float z1, float &u0, float &u1) const
+    {
+        u0 = (z0 - y0) * il00;
+        u1 = ((z1 - y1) - l10 * u0) * il11;
+    }
+};
+struct Whiten64 {
+    double y0, y1, il00, l10, il11;
+    __device__ __forceinline__ void operator()(float z0, float z1, float &u0, float &u1) const
+    {
+        const double a0 = ((double)z0 - y0) * il00;
+        u0 = (float)a0;
+        u1 = (float)((((double)z1 - y1) - l10 * a0) * il11);
+    }
+};
+
 // Likelihood of one prepared joint block and the parameters of its update
 // x += W^T v, P -= W^T M W (M = I for a certain measurement); returns the
-// joint's log-likelihood term.
-template <int MODE>
+// joint's log-likelihood term.  logl = log(l00 l11).
+template <int MODE, typename WH>
 __device__ __forceinline__ float joint_measure(const StepArgs &a, int t, int j, int b, bool clutter, int cnt,
-                                               const float *cz, float y0, float y1, float il00, float l10,
-                                               float il11, float logl, float &v0, float &v1, float &m00,
-                                               float &m01, float &m11)
+                                               const float *cz, const WH &wh, float logl, float &v0, float &v1,
+                                               float &m00, float &m01, float &m11)
 {
     const float LOG2PI = 1.8378770664093453f;
     if (!clutter) {
@@ -664,8 +759,7 @@ __device__ __forceinline__ float joint_measure(const StepArgs &a, int t, int j, 
             z0 = cz[0];
             z1 = cz[1];
         }
-        v0 = (z0 - y0) * il00;
-        v1 = ((z1 - y1) - l10 * v0) * il11;
+        wh(z0, z1, v0, v1);
         m00 = 1.f;
         m01 = 0.f;
         m11 = 1.f;
@@ -676,9 +770,8 @@ __device__ __forceinline__ float joint_measure(const StepArgs &a, int t, int j, 
     const float lnorm = -logl - LOG2PI;
     float se = 0.f, sv0 = 0.f, sv1 = 0.f, q00 = 0.f, q01 = 0.f, q11 = 0.f;
     for (int c = 0; c < cnt; ++c) {
-        const float n0 = cz[2 * c] - y0, n1 = cz[2 * c + 1] - y1;
-        const float u0 = n0 * il00;
-        const float u1 = (n1 - l10 * u0) * il11;
+        float u0, u1;
+        wh(cz[2 * c], cz[2 * c + 1], u0, u1);
         const float d2 = fmaf(u0, u0, u1 * u1);
         if (d2 <= a.gate) {
             const float e = a.pdl * __expf(lnorm - 0.5f * d2);
@@ -701,6 +794,114 @@ __device__ __forceinline__ float joint_measure(const StepArgs &a, int t, int j, 
     return logf(ell);
 }
 
+template <int MODE>
+__device__ __forceinline__ float joint_measure(const StepArgs &a, int t, int j, int b, bool clutter, int cnt,
+                                               const float *cz, float y0, float y1, float il00, float l10,
+                                               float il11, float logl, float &v0, float &v1, float &m00,
+                                               float &m01, float &m11)
+{
+    return joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, Whiten32{y0, y1, il00, l10, il11}, logl, v0, v1, m00,
+                               m01, m11);
+}
+
+// The FP64 factor of one joint whose innovation covariance FP32 cannot
+// factor reliably (innovation_ill): S = H_b P H_b^T + R_b and y = H_b x +
+// BH_b from the FP32 state and the FP64 innovation block j64 = (bh0, bh1,
+// r00, r01, r11), the FP64 Cholesky, W = L^-1 H_b P rounded to FP32 (the
+// update P - W^T M W stays within the FP32 rounding of P: W^T W <= P) and
+// the FP64 whitening.  SEL: H_b selects rows r0, r0 + 1 (h0, h1 unused),
+// otherwise h0, h1 are its rows.  Sets kStatusIllCond; returns false (and
+// sets kStatusNotSpd) if S is not positive definite in FP64 either.
+template <int D, bool SEL>
+__device__ __forceinline__ bool joint_factor64(const float (&x)[D], const float (&P)[npacked(D)], int r0,
+                                               const float *h0, const float *h1, const double *j64, Whiten64 &wh,
+                                               float &logl, float (&w0)[D], float (&w1)[D], unsigned &flags)
+{
+    flags |= kStatusIllCond;
+    auto hp = [&](int row, int c) -> double {       // (H_b P)[row][c]
+        if constexpr (SEL) {
+            return (double)P[pk(D, r0 + row, c)];
+        } else {
+            const float *h = row ? h1 : h0;
+            double acc = 0.0;
+            for (int k = 0; k < D; ++k) acc = fma((double)h[k], (double)P[pk(D, k, c)], acc);
+            return acc;
+        }
+    };
+    double y0 = j64[0], y1 = j64[1], s00 = j64[2], s01 = j64[3], s11 = j64[4];
+    if constexpr (SEL) {
+        y0 += (double)x[r0];
+        y1 += (double)x[r0 + 1];
+        s00 += (double)P[pk(D, r0, r0)];
+        s01 += (double)P[pk(D, r0, r0 + 1)];
+        s11 += (double)P[pk(D, r0 + 1, r0 + 1)];
+    } else {
+        for (int c = 0; c < D; ++c) {
+            const double p0 = hp(0, c), p1 = hp(1, c);
+            y0 = fma((double)h0[c], (double)x[c], y0);
+            y1 = fma((double)h1[c], (double)x[c], y1);
+            s00 = fma((double)h0[c], p0, s00);
+            s01 = fma((double)h1[c], p0, s01);
+            s11 = fma((double)h1[c], p1, s11);
+        }
+    }
+    const double d11 = s00 > 0.0 ? s11 - s01 * s01 / s00 : 0.0;
+    if (!(s00 > 0.0) || !(d11 > 0.0)) {
+        flags |= kStatusNotSpd;
+        return false;
+    }
+    const double l00 = sqrt(s00), l11 = sqrt(d11);
+    wh.y0 = y0;
+    wh.y1 = y1;
+    wh.il00 = 1.0 / l00;
+    wh.l10 = s01 * wh.il00;
+    wh.il11 = 1.0 / l11;
+    logl = (float)log(l00 * l11);
+    // W rows: w0 = hp0 / l00 and w1 = (hp1 - rho hp0) / l11 with rho = s01 / s00
+    // as a double-float pair, so the cancellation in hp1 - rho hp0 happens
+    // inside two FMAs (each rounds once) and the FP32 rows stay accurate.
+    const double rho = s01 / s00;
+    const float rhi = (float)rho, rlo = (float)(rho - (double)rhi);
+    const float fil00 = (float)wh.il00, fil11 = (float)wh.il11;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        float p0, p1;
+        if constexpr (SEL) {
+            p0 = P[pk(D, r0, c)];
+            p1 = P[pk(D, r0 + 1, c)];
+        } else {
+            p0 = (float)hp(0, c);
+            p1 = (float)hp(1, c);
+        }
+        w0[c] = p0 * fil00;
+        w1[c] = fmaf(-rlo, p0, fmaf(-rhi, p0, p1)) * fil11;
+    }
+    return true;
+}
+
+// Likelihood and update parameters of a joint that is not positive definite
+// in FP64: zero weight (-inf) and no update.
+template <int D>
+__device__ __forceinline__ float joint_reject(float (&w0)[D], float (&w1)[D], float &v0, float &v1, float &m00,
+                                              float &m01, float &m11)
+{
+#pragma unroll
+    for (int c = 0; c < D; ++c) w0[c] = w1[c] = 0.f;
+    v0 = v1 = m00 = m01 = m11 = 0.f;
+    return -INFINITY;
+}
+
+// ORs the wave's particle flags (kStatusIllCond, kStatusNotSpd) into *status
+// with one atomic per wave, only when a flag is raised.  Every lane of the
+// wave calls it.
+__device__ __forceinline__ void report_flags(unsigned flags, unsigned int *status)
+{
+    const unsigned long long ill = __ballot((flags & kStatusIllCond) != 0u);
+    const unsigned long long bad = __ballot((flags & kStatusNotSpd) != 0u);
+    if ((ill | bad) && (threadIdx.x & 63) == 0)
+        atomicOr(status, (ill ? kStatusIllCond : 0u) | (bad ? kStatusNotSpd : 0u));
+}
+
 // LDS bytes of k_step: model image + ancestor workspace (fused mode).
 __host__ __device__ constexpr int step_lds_bytes(int stage_floats, int nb_pending)
 {
@@ -720,10 +921,16 @@ __host__ __device__ constexpr int step_lds_bytes(int stage_floats, int nb_pendin
 // per-sequence values pi_total (component thresholds' total), comp_global
 // (component rows outside LDS, or null) and comp_out (optional component
 // record) are passed explicitly.
-template <int D, int MODE, int SELJ>
+// F64 = false (the step's own pass): every joint is factored in FP32; a
+// joint whose S is ill-conditioned (innovation_ill) sets kNeedFp64 in flags
+// and the particle's results are then not used: the step redoes that particle
+// from its ancestor's state with F64 = true, where such joints are factored
+// in FP64 (joint_factor64).  Keeping the FP64 code out of the main pass keeps
+// its register allocation unchanged.
+template <int D, int MODE, int SELJ, bool F64 = false>
 __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j, uint32_t U1, const float *sm,
                                                  float (&x)[D], float (&P)[npacked(D)], unsigned long long pi_total,
-                                                 const float *comp_global, int *comp_out)
+                                                 const float *comp_global, int *comp_out, unsigned &flags)
 {
     constexpr int NP = npacked(D);
     constexpr int CS = 2 + D + NP;
@@ -731,6 +938,7 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
     const float *Hs = sm + 4 * a.K;
     const float *BHs = Hs + a.m * D;
     const float *Rs = BHs + a.m;
+    const double *J64 = reinterpret_cast<const double *>(sm + image_j64_floats(a.K, a.m, D, a.J));
     const float *stab = comp_global ? comp_global : sm + image_head_floats(a.K, a.m, D, a.J);
     const int k = draw_component(sthr, a.K, (uint32_t)j, U1, pi_total);
     if (comp_out) comp_out[j] = k;
@@ -751,11 +959,21 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
             bool clutter;
             if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter, sm)) continue;
             float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
-            Joint<D>::prepare_sel(x, P, 2 * oi, BHs[2 * b], BHs[2 * b + 1], Rs[3 * b], Rs[3 * b + 1],
-                                  Rs[3 * b + 2], w0, w1, y0, y1, il00, l10, il11, logl);
+            const bool ok = Joint<D>::prepare_sel(x, P, 2 * oi, BHs[2 * b], BHs[2 * b + 1], Rs[3 * b],
+                                                  Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0, y1, il00, l10, il11,
+                                                  logl);
             float v0, v1, m00, m01, m11;
-            acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
-                                       v1, m00, m01, m11);
+            if (!F64 || ok) {
+                if (!ok) flags |= kNeedFp64;
+                acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
+                                           v1, m00, m01, m11);
+            } else {
+                Whiten64 wh;
+                if (joint_factor64<D, true>(x, P, 2 * oi, nullptr, nullptr, J64 + 5 * b, wh, logl, w0, w1, flags))
+                    acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, wh, logl, v0, v1, m00, m01, m11);
+                else
+                    acc += joint_reject<D>(w0, w1, v0, v1, m00, m01, m11);
+            }
             Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
         }
     } else {
@@ -766,12 +984,22 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
             bool clutter;
             if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter, sm)) continue;
             float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
-            Joint<D>::prepare(x, P, Hs + (2 * b) * D, Hs + (2 * b + 1) * D, BHs[2 * b], BHs[2 * b + 1],
-                              Rs[3 * b], Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0, y1, il00, l10,
-                              il11, logl);
+            const bool ok = Joint<D>::prepare(x, P, Hs + (2 * b) * D, Hs + (2 * b + 1) * D, BHs[2 * b],
+                                              BHs[2 * b + 1], Rs[3 * b], Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0,
+                                              y1, il00, l10, il11, logl);
             float v0, v1, m00, m01, m11;
-            acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
-                                       v1, m00, m01, m11);
+            if (!F64 || ok) {
+                if (!ok) flags |= kNeedFp64;
+                acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
+                                           v1, m00, m01, m11);
+            } else {
+                Whiten64 wh;
+                if (joint_factor64<D, false>(x, P, 0, Hs + (2 * b) * D, Hs + (2 * b + 1) * D, J64 + 5 * b, wh, logl,
+                                             w0, w1, flags))
+                    acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, wh, logl, v0, v1, m00, m01, m11);
+                else
+                    acc += joint_reject<D>(w0, w1, v0, v1, m00, m01, m11);
+            }
             Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
         }
     }
@@ -779,11 +1007,12 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
 }
 
 template <int D, int MODE, int SELJ>
-__device__ __forceinline__ void step_body(const StepArgs &a)
+__device__ __forceinline__ void step_body(const StepArgs &ain)
 {
     constexpr int NP = npacked(D);
     extern __shared__ float4 sm4[];
-    const int t = *a.frame;
+    int t;
+    const StepArgs a = frame_args(ain, t);
     const uint32_t U1 = a.uni[2 * t];
     const int j0 = blockIdx.x * blockDim.x;
     const int j = j0 + threadIdx.x;
@@ -828,17 +1057,34 @@ __device__ __forceinline__ void step_body(const StepArgs &a)
     __syncthreads();
 
     float lwj = -INFINITY;
+    unsigned pflags = 0;
     if (live) {
         lwj = particle_update<D, MODE, SELJ>(a, t, j, U1, reinterpret_cast<const float *>(sm4), x, P, a.pi_total,
-                                             a.comp_global, a.comp_out);
+                                             a.comp_global, a.comp_out, pflags);
         const unsigned doff = (unsigned)j * 4u;
         const __amdgpu_buffer_rsrc_t wx = soa_rsrc(a.xout, rowb * D), wP = soa_rsrc(a.Pout, rowb * NP);
 #pragma unroll
         for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[i]);
 #pragma unroll
         for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[p]);
+        if (pflags & kNeedFp64) {
+            // redo this particle with the ill-conditioned joints in FP64
+            pflags = 0;
+            const __amdgpu_buffer_rsrc_t rx = soa_rsrc(a.xin, rowb * D), rP = soa_rsrc(a.Pin, rowb * NP);
+#pragma unroll
+            for (int i = 0; i < D; ++i) x[i] = soa_ld(rx, voff, rowb * i);
+#pragma unroll
+            for (int p = 0; p < NP; ++p) P[p] = soa_ld(rP, voff, rowb * p);
+            lwj = particle_update<D, MODE, SELJ, true>(a, t, j, U1, reinterpret_cast<const float *>(sm4), x, P,
+                                                       a.pi_total, a.comp_global, a.comp_out, pflags);
+#pragma unroll
+            for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[i]);
+#pragma unroll
+            for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[p]);
+        }
         a.lw[j] = lwj;
     }
+    report_flags(pflags, &a.fs->status);
     if (a.w_cloc) {
         // this frame's resampler weights: the block is one resampler block;
         // its exponent raises slot t mod 3, and block 0 clears slot t + 1
@@ -849,7 +1095,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a)
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             a.fs->U_pending = a.uni[2 * t + 1];
             a.fs->pow_max[(t + 1) % 3] = kPowEmpty;
-            *a.frame_next = t + 1;
+            if (a.frame_next) *a.frame_next = t + 1;
         }
     }
 }

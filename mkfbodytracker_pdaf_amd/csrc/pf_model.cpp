@@ -264,6 +264,15 @@ int make_model_image(const HostModel &hm, unsigned order, int N, ModelImage &img
         Hs[m * d + m + 3 * b + 1] = (float)Rm[(2 * b) * m + 2 * b + 1];
         Hs[m * d + m + 3 * b + 2] = (float)Rm[(2 * b + 1) * m + 2 * b + 1];
     }
+    // FP64 innovation blocks (the joints whose S is factored in FP64)
+    std::vector<double> j64((size_t)5 * J);
+    for (int b = 0; b < J; ++b) {
+        j64[5 * b] = hm.BH[2 * b];
+        j64[5 * b + 1] = hm.BH[2 * b + 1];
+        j64[5 * b + 2] = Rm[(2 * b) * m + 2 * b];
+        j64[5 * b + 3] = Rm[(2 * b) * m + 2 * b + 1];
+        j64[5 * b + 4] = Rm[(2 * b + 1) * m + 2 * b + 1];
+    }
     // Component thresholds for resample(gmm.weight, N) (pf2DRao.cpp:128).
     std::vector<unsigned long long> thr(2 * (size_t)K);
     unsigned long long acc = 0;
@@ -277,6 +286,7 @@ int make_model_image(const HostModel &hm, unsigned order, int N, ModelImage &img
     img.blob.assign((size_t)head + tab.size(), 0.f);
     std::memcpy(img.blob.data(), thr.data(), sizeof(unsigned long long) * thr.size());
     std::memcpy(img.blob.data() + 4 * K, hbr.data(), sizeof(float) * hbr.size());
+    std::memcpy(img.blob.data() + image_j64_floats(K, m, d, J), j64.data(), sizeof(double) * j64.size());
     std::memcpy(img.blob.data() + head, tab.data(), sizeof(float) * tab.size());
     img.blob.resize((img.blob.size() + 3) & ~(size_t)3, 0.f);
     img.head_floats = head;

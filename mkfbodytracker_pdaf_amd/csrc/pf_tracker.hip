@@ -857,14 +857,14 @@ int copy_stage_out(pf_tracker *tr, const pf_tracker_outputs &o)
 
 int status_of_arms(pf_tracker *tr)
 {
-    int warn = 0;
+    unsigned word = 0;
     for (int a = 0; a < 2; ++a) {
-        int deg = 0;
-        int st = eng::take_status(tr->arm[a], &deg);
+        unsigned w = 0;
+        int st = eng::take_status(tr->arm[a], &w);
         if (st) return st;
-        warn |= deg;
+        word |= w;
     }
-    return warn ? PF_WARN_DEGENERATE : PF_OK;
+    return status_code(word);
 }
 
 }  // namespace

@@ -157,8 +157,9 @@ class Rbpf:
         """As run_candidates_dev with one measurement column per particle."""
         check(self.lib.pf_run_ref_dev(self.h, int(T), Z_ptr, uni_ptr, lml_ptr, int(flags)))
 
-    def synchronize(self):
-        check(self.lib.pf_synchronize(self.h))
+    def synchronize(self) -> int:
+        """Waits for the handle's work; returns its warning bits (capi.PF_WARN_*)."""
+        return check(self.lib.pf_synchronize(self.h))
 
     def stream(self):
         s = C.c_void_p()
@@ -169,6 +170,12 @@ class Rbpf:
         a, b, n = C.c_double(0), C.c_double(0), C.c_int32(0)
         check(self.lib.pf_last_run_kernel_time(self.h, C.byref(a), C.byref(b), C.byref(n)))
         return a.value, b.value, n.value
+
+    def last_graph_frames(self) -> int:
+        """Frames of the last device-resident run replayed from graphs."""
+        n = C.c_int32(0)
+        check(self.lib.pf_last_run_graph_frames(self.h, C.byref(n)))
+        return n.value
 
 
 # ============================================ reference-named mirror classes

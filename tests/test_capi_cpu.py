@@ -83,3 +83,18 @@ def test_philox_known_answers():
     b = capi.philox_u32(123456789012345, 7, 1000, 4096)
     c = oracle.philox_u32(123456789012345, 7, 1000, 4096)
     assert np.array_equal(b, c)
+
+
+def test_ptr_keeps_a_temporary_array_alive():
+    """ptr(f64(x)) of an array that f64 must copy: the copy stays alive as long
+    as the pointer object (the foreign call's argument), so the call never
+    reads freed memory."""
+    import ctypes as C
+    import gc
+    src = np.asfortranarray(np.arange(600.0).reshape(6, 100))
+    p = capi.ptr(capi.f64(src))
+    gc.collect()
+    junk = [np.full(600, -1.0) for _ in range(200)]
+    vals = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(600,))
+    assert np.array_equal(vals, np.ascontiguousarray(src).ravel())
+    del junk

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from mkfbodytracker_pdaf_amd import synthetic
+from mkfbodytracker_pdaf_amd import capi, synthetic
 from mkfbodytracker_pdaf_amd.batch import RbpfBatch
 from mkfbodytracker_pdaf_amd.rbpf import Rbpf
 
@@ -163,6 +163,7 @@ def test_batch_d24_matches_single_runs(gpu):
         ref.append((*rb.get_tracks(), d[3].get()))
         rb.close()
     b = RbpfBatch(Ns, Ts, K=64, m=10, d=24, cand_capacity=cap, clutter_density=lam)
+    b.variant = capi.PF_BATCH_VALU          # the single filter's kernel (bit-equal runs)
     for s, mdl in enumerate(models):
         b.load_model(s, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
     b.set_joint_clutter(models[0].clutter)
@@ -234,6 +235,7 @@ def test_scaled_shape_batch_against_oracle_and_single_runs(gpu):
         singles[s] = (*rb.get_tracks(), dd[3].get())
         rb.close()
     b = RbpfBatch([N] * S, [T] * S, K=K, m=10, d=d, cand_capacity=cap, clutter_density=lam)
+    b.variant = capi.PF_BATCH_VALU          # the single filter's kernel (bit-equal runs)
     for s, mdl in enumerate(models):
         b.load_model(s, mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean, mdl.sel, mdl.R)
     b.set_joint_clutter(models[0].clutter)

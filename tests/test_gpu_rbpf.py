@@ -379,3 +379,49 @@ def test_timed_device_run_is_the_same_run(gpu):
         rb.close()
     for r in res[1:]:
         assert all(np.array_equal(a, b) for a, b in zip(r, res[0]))
+
+
+def test_graph_groups_serve_consecutive_runs(gpu):
+    """The captured frame groups are reused by later runs of other lengths
+    and inputs: runs of 5 (odd: the tracks change buffers), 19, 1 and 70
+    frames on one handle, replayed from graphs, give bit-identical tracks,
+    ancestors and log mean weights to the same runs launched frame by frame,
+    and report every frame as graph-replayed; a model reload captures the
+    groups again."""
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    N = 2500
+    lengths = (5, 19, 1, 70)
+    mdl, _, _ = _setup(N)
+    seq = synthetic.simulate(mdl, sum(lengths), seed=57, cand_per_joint=4, cap=8)
+    bins, _ = oracle.resample_linear(mdl.weights, N, 23)
+    d_cnt = DeviceArray(seq.count.astype(np.int32))
+    d_cand = DeviceArray(seq.cand.astype(np.float32))
+    d_uni = DeviceArray(seq.uniforms.astype(np.uint32))
+    res = {}
+    for flags in (0, 1):
+        _, rb, _ = _setup(N)
+        rb.set_joint_clutter(mdl.clutter)
+        rb.reset(bins)
+        d_lml = DeviceArray.empty(sum(lengths), np.float64)
+        out, t0 = [], 0
+        for i, T in enumerate(lengths):
+            if i == 3 and flags:
+                # reloading the model drops the captured groups; the tracks
+                # are re-imported, so the run continues from the same state
+                x, P = rb.get_tracks()
+                rb.load_model(mdl.means, mdl.covs, mdl.weights, mdl.gamma, mdl.pca_proj, mdl.pca_mean,
+                              sel=mdl.sel, R=mdl.R)
+                rb.set_joint_clutter(mdl.clutter)
+                rb.set_tracks(x, P)
+            rb.run_candidates_dev(T, d_cnt.ptr + 4 * seq.count.shape[1] * t0,
+                                  d_cand.ptr + 4 * seq.cand[0].size * t0, d_uni.ptr + 8 * t0,
+                                  d_lml.ptr + 8 * t0, flags)
+            rb.synchronize()
+            assert rb.last_graph_frames() == (T if flags else 0)
+            out.append(rb.get_tracks())
+            t0 += T
+        res[flags] = (out, d_lml.get())
+        rb.close()
+    for (xa, Pa), (xb, Pb) in zip(res[0][0], res[1][0]):
+        assert np.array_equal(xa, xb) and np.array_equal(Pa, Pb)
+    assert np.array_equal(res[0][1], res[1][1])
