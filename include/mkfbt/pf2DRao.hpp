@@ -41,6 +41,9 @@ public:
 
     pf_handle *handle() { return h_; }
     uint32_t draw_u32();
+    // Position of the host random stream: the next draw_u32() returns
+    // Philox4x32-10 word draws() of (seed, stream 0).
+    uint64_t draws() const { return draws_; }
     int lastStatus() const { return last_status_; }
 
 private:

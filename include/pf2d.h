@@ -34,6 +34,15 @@ int pf2d_destroy(pf2d_handle *h);
  * inverse) and 1 / ((2 pi)^(d/2) sqrt(det Sigma_k)). */
 int pf2d_load_gmm(pf2d_handle *h, int32_t K, const double *means, const double *covs, const double *weights);
 
+/* The same GMM in the members ParticleFilter::update reads (pf2D.h:18-21,
+ * pf2D.cpp:159-163): means K x d, sigma_i K x d x d (inverse covariances;
+ * symmetrised), det_s K (1 / ((2 pi)^(d/2) sqrt(det Sigma))), weights K.
+ * Component k contributes weights[k] det_s[k] exp(-(x - mu_k) sigma_i_k
+ * (x - mu_k)^T / 2) to the prior; weights[k] det_s[k] must be finite and
+ * positive.  pf2d_load_gmm computes sigma_i and det_s from covs and calls this. */
+int pf2d_load_gmm_precision(pf2d_handle *h, int32_t K, const double *means, const double *sigma_i,
+                            const double *det_s, const double *weights);
+
 /* Constructor randomisation (pf2D.cpp:58-70): column c of particle i is
  * lo_c + (hi_c - lo_c) * unif[i * d + c], with column 6 restricted to the
  * side's half of the 640-pixel image; weights 1/N. */
@@ -54,6 +63,11 @@ int pf2d_get_particles(pf2d_handle *h, double *particles, double *weights);
  * on the degenerate re-initialisation. */
 int pf2d_update(pf2d_handle *h, const double *meas, uint32_t u, const double *normals,
                 const double *reinit_unif, float *lw, int32_t *anc);
+
+/* ParticleFilter::predict() (pf2D.cpp:90-102) on its own: the random walk
+ * x += 5 n on the current particles, normals (N x 8) consumed per particle
+ * for columns 2..5, 6, 7, 0, 1 as in pf2d_update; the weights are kept. */
+int pf2d_predict(pf2d_handle *h, const double *normals);
 
 /* ParticleFilter::getEstimator (pf2D.cpp:79-87): sum_i weights_i particles_i. */
 int pf2d_estimate(pf2d_handle *h, double *out);

@@ -23,37 +23,6 @@ def sources():
     return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
-ROOT = os.path.dirname(HERE)
-CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
-CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_host_api")
-
-
-def build_cpp_test() -> str:
-    """Compile the C++ drop-in check (plain g++, linked against libmkfbt.so)."""
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC,
-           "-L", HERE, "-lmkfbt", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../mkfbodytracker_pdaf_amd",
-           "-o", CPP_TEST_BIN]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError("g++ failed:\n" + res.stderr[-4000:])
-    return CPP_TEST_BIN
-
-
-ADJ_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_adjoint_local.cpp")
-ADJ_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_adjoint_local")
-
-
-def build_adjoint_local_test() -> str:
-    """Compile the host-only check of the joint-local reverse (hipcc: the
-    header is host- and device-callable)."""
-    cmd = [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-           "-packed-fp32-ops", ADJ_TEST_SRC, "-o", ADJ_TEST_BIN]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + res.stderr[-4000:])
-    return ADJ_TEST_BIN
-
-
 def build(verbose: bool = False, jobs: int = 0) -> str:
     """Compile every source under csrc/ into libmkfbt.so (always rebuilds):
     one hipcc per source in parallel, then one link."""
@@ -83,10 +52,6 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
         if res.returncode != 0:
             raise RuntimeError("link failed:\n" + res.stderr[-8000:])
     os.replace(tmp, LIB)
-    if LIB.endswith("libmkfbt.so") and os.path.exists(CPP_TEST_SRC):
-        build_cpp_test()
-    if LIB.endswith("libmkfbt.so") and os.path.exists(ADJ_TEST_SRC):
-        build_adjoint_local_test()
     return LIB
 
 

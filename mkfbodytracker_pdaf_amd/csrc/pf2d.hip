@@ -93,8 +93,9 @@ __global__ void __launch_bounds__(256) k_pf2d_weight(const float *xs, int N, int
     }
 }
 
-// Resampling gather or re-initialisation, then predict (pf2D.cpp:90-102):
-// x += 5 n, normals of particle i consumed for columns 2, 3, 4, 5, 6, 7, 0, 1.
+// Resampling gather (anc; the identity when null) or re-initialisation, then
+// predict (pf2D.cpp:90-102): x += 5 n, normals of particle i consumed for
+// columns 2, 3, 4, 5, 6, 7, 0, 1.
 __global__ void __launch_bounds__(256) k_pf2d_move(const float *xin, float *xout, const int *anc,
                                                    const double *normals, const double *reinit, int side,
                                                    int N, int ld)
@@ -110,7 +111,7 @@ __global__ void __launch_bounds__(256) k_pf2d_move(const float *xin, float *xout
             base[c] = (float)(lo + (double)(hi - lo) * reinit[(size_t)j * kD + c]);
         }
     } else {
-        const int s = anc[j];
+        const int s = anc ? anc[j] : j;
 #pragma unroll
         for (int c = 0; c < kD; ++c) base[c] = xin[(size_t)c * ld + s];
     }
@@ -289,20 +290,47 @@ int pf2d_create(pf2d_handle **out, int32_t n_particles, int32_t n_dims, int32_t 
     return PF_OK;
 }
 
+int pf2d_load_gmm_precision(pf2d_handle *h, int32_t K, const double *means, const double *sigma_i,
+                            const double *det_s, const double *weights)
+{
+    if (!h || !means || !sigma_i || !det_s || !weights) return set_error(PF_ERR_ARG, "null argument");
+    if (K <= 0 || K > 256) return set_error(PF_ERR_ARG, "K=%d out of range", K);
+    constexpr int CS = 1 + kD + kD * kD;
+    std::vector<float> tab((size_t)K * CS);
+    for (int k = 0; k < K; ++k) {
+        // weight_k det_s_k exp(-(x - mu_k) sigma_i_k (x - mu_k)^T / 2) (pf2D.cpp:162)
+        const double c = weights[k] * det_s[k];
+        if (!(c > 0.0) || !std::isfinite(c))
+            return set_error(PF_ERR_ARG, "component %d: weight * det_s must be finite and > 0", k);
+        const double *Si = sigma_i + (size_t)k * kD * kD;
+        float *g = &tab[(size_t)k * CS];
+        g[0] = (float)std::log(c);
+        for (int i = 0; i < kD; ++i) g[1 + i] = (float)means[(size_t)k * kD + i];
+        for (int i = 0; i < kD * kD; ++i) g[1 + kD + i] = (float)(0.5 * (Si[i] + Si[(i % kD) * kD + i / kD]));
+    }
+    HIP_TRY2(hipSetDevice(h->device));
+    (void)hipFree(h->table);
+    h->table = nullptr;
+    HIP_TRY2(dalloc2(&h->table, tab.size()));
+    HIP_TRY2(hipMemcpyAsync(h->table, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY2(hipStreamSynchronize(h->stream));
+    h->K = K;
+    h->gmm = true;
+    return PF_OK;
+}
+
 int pf2d_load_gmm(pf2d_handle *h, int32_t K, const double *means, const double *covs, const double *weights)
 {
     if (!h || !means || !covs || !weights) return set_error(PF_ERR_ARG, "null argument");
     if (K <= 0 || K > 256) return set_error(PF_ERR_ARG, "K=%d out of range", K);
-    HIP_TRY2(hipSetDevice(h->device));
-    constexpr int CS = 1 + kD + kD * kD;
-    std::vector<float> tab((size_t)K * CS);
+    std::vector<double> Sinv((size_t)K * kD * kD), dets((size_t)K);
     for (int k = 0; k < K; ++k) {
-        // Cholesky of Sigma_k, its inverse and log det (my_gmm::loadGaussian, pf2D.cpp:33-35)
+        // Cholesky of Sigma_k, its inverse and determinant (my_gmm::loadGaussian, pf2D.cpp:33-35)
         double L[kD * kD];
         const double *S = covs + (size_t)k * kD * kD;
         for (int i = 0; i < kD * kD; ++i) L[i] = 0.0;
         for (int j = 0; j < kD; ++j) {
-            double s = 0.5 * (S[j * kD + j] + S[j * kD + j]);
+            double s = S[j * kD + j];
             for (int q = 0; q < j; ++q) s -= L[j * kD + q] * L[j * kD + q];
             if (!(s > 0.0)) return set_error(PF_ERR_NOT_SPD, "covariance of component %d is not SPD", k);
             L[j * kD + j] = std::sqrt(s);
@@ -314,7 +342,7 @@ int pf2d_load_gmm(pf2d_handle *h, int32_t K, const double *means, const double *
         }
         double logdet = 0.0;
         for (int i = 0; i < kD; ++i) logdet += 2.0 * std::log(L[i * kD + i]);
-        double Si[kD * kD];
+        double *Si = &Sinv[(size_t)k * kD * kD];
         for (int c = 0; c < kD; ++c) {
             double y[kD], xx[kD];
             for (int i = 0; i < kD; ++i) {
@@ -330,19 +358,9 @@ int pf2d_load_gmm(pf2d_handle *h, int32_t K, const double *means, const double *
             for (int i = 0; i < kD; ++i) Si[i * kD + c] = xx[i];
         }
         if (!(weights[k] > 0.0)) return set_error(PF_ERR_ARG, "component weight %d must be > 0", k);
-        float *g = &tab[(size_t)k * CS];
-        g[0] = (float)(std::log(weights[k]) - 0.5 * kD * std::log(6.283185307179586) - 0.5 * logdet);
-        for (int c = 0; c < kD; ++c) g[1 + c] = (float)means[(size_t)k * kD + c];
-        for (int i = 0; i < kD * kD; ++i) g[1 + kD + i] = (float)(0.5 * (Si[i] + Si[(i % kD) * kD + i / kD]));
+        dets[k] = std::exp(-0.5 * kD * std::log(6.283185307179586) - 0.5 * logdet);
     }
-    (void)hipFree(h->table);
-    h->table = nullptr;
-    HIP_TRY2(dalloc2(&h->table, tab.size()));
-    HIP_TRY2(hipMemcpyAsync(h->table, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY2(hipStreamSynchronize(h->stream));
-    h->K = K;
-    h->gmm = true;
-    return PF_OK;
+    return pf2d_load_gmm_precision(h, K, means, Sinv.data(), dets.data(), weights);
 }
 
 int pf2d_init(pf2d_handle *h, const double *unif)
@@ -453,6 +471,20 @@ int pf2d_update(pf2d_handle *h, const double *meas, uint32_t u, const double *no
     if (anc) HIP_TRY2(hipMemcpyAsync(anc, h->anc, sizeof(int) * N, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY2(hipStreamSynchronize(h->stream));
     return degenerate ? PF_WARN_DEGENERATE : PF_OK;
+}
+
+int pf2d_predict(pf2d_handle *h, const double *normals)
+{
+    if (!h || !normals) return set_error(PF_ERR_ARG, "null argument");
+    HIP_TRY2(hipSetDevice(h->device));
+    const int N = h->N;
+    HIP_TRY2(hipMemcpyAsync(h->nbuf, normals, sizeof(double) * (size_t)N * kD, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_pf2d_move, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->xA, h->xB,
+                       (const int *)nullptr, h->nbuf, (const double *)nullptr, h->side, N, h->ld);
+    HIP_TRY2(hipGetLastError());
+    HIP_TRY2(hipStreamSynchronize(h->stream));
+    std::swap(h->xA, h->xB);
+    return PF_OK;
 }
 
 int pf2d_estimate(pf2d_handle *h, double *out)

@@ -4,6 +4,7 @@
 // keeps the reference's object model (public members, method names, argument
 // meaning), draws the host-supplied random numbers and turns negative status
 // codes into exceptions.
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <mutex>
@@ -84,7 +85,7 @@ void KF_model::update(Mat measurement, Mat &state, Mat &cov)
 }
 
 // --------------------------------------------------------------- my_gmm
-my_gmm::my_gmm() {}
+my_gmm::my_gmm() { tracks.gmm_ = this; }
 my_gmm::~my_gmm() {}
 
 void my_gmm::loadGaussian(Mat u, Mat s, Mat &H, Mat &m, double w, double g)
@@ -143,20 +144,41 @@ void my_gmm::resetTracker(std::vector<int> bins)
     if ((int)bins.size() != nParticles) throw PfException(PF_ERR_ARG, "resetTracker: need one bin per particle");
     std::vector<int32_t> b(bins.begin(), bins.end());
     ok(pf_reset(pf_->h_, b.data()));
+    device_changed();
 }
 
-std::vector<state_params> my_gmm::tracks_snapshot() const
+void my_gmm::push()
 {
-    const int d = cov.empty() ? 0 : cov[0].rows, N = nParticles;
+    upload();
+    if (!tracks.dirty_) return;
+    const int d = dims(), N = nParticles;
+    const std::vector<state_params> &v = tracks.v_;
+    if ((int)v.size() != N) throw PfException(PF_ERR_STATE, "gmm.tracks must hold nParticles tracks");
     std::vector<double> x((size_t)N * d), P((size_t)N * d * d);
-    ok(pf_get_tracks(pf_->h_, x.data(), P.data()));
-    std::vector<state_params> out((size_t)N);
     for (int j = 0; j < N; ++j) {
-        out[j].state = Mat(d, 1, x.data() + (size_t)j * d);
-        out[j].cov = Mat(d, d, P.data() + (size_t)j * d * d);
-        out[j].weight = 1.0 / N;
+        const state_params &t = v[j];
+        if (t.state.rows * t.state.cols != d || t.cov.rows != d || t.cov.cols != d)
+            throw PfException(PF_ERR_ARG, "gmm.tracks[" + std::to_string(j) + "] has the wrong shape");
+        std::copy(t.state.data.begin(), t.state.data.end(), x.begin() + (size_t)j * d);
+        std::copy(t.cov.data.begin(), t.cov.data.end(), P.begin() + (size_t)j * d * d);
     }
-    return out;
+    ok(pf_set_tracks(pf_->h_, x.data(), P.data()));
+    tracks.dirty_ = false;
+}
+
+void TrackList::pull() const
+{
+    if (!stale_) return;
+    const int d = gmm_->dims(), N = gmm_->nParticles;
+    std::vector<double> x((size_t)N * d), P((size_t)N * d * d);
+    ok(pf_get_tracks(gmm_->pf_->handle(), x.data(), P.data()));
+    v_.assign((size_t)N, state_params());
+    for (int j = 0; j < N; ++j) {
+        v_[j].state = Mat(d, 1, x.data() + (size_t)j * d);
+        v_[j].cov = Mat(d, d, P.data() + (size_t)j * d * d);
+        v_[j].weight = 1.0 / N;
+    }
+    stale_ = false;
 }
 
 // -------------------------------------------------------- ParticleFilter
@@ -185,37 +207,40 @@ uint32_t ParticleFilter::draw_u32()
 
 void ParticleFilter::setJointClutter(const std::vector<int> &clutter)
 {
-    gmm.upload();
+    gmm.push();
     std::vector<int32_t> c(clutter.begin(), clutter.end());
     ok(pf_set_joint_clutter(h_, c.data()));
 }
 
 void ParticleFilter::update(Mat measurement)
 {
-    gmm.upload();
+    gmm.push();
     if (measurement.cols != gmm.nParticles) throw PfException(PF_ERR_ARG, "update: measurement must be m x N");
     const uint32_t u1 = draw_u32(), u2 = draw_u32();
     last_status_ = ok(pf_step(h_, measurement.ptr(), u1, u2, nullptr, nullptr, nullptr, nullptr));
+    gmm.device_changed();
 }
 
 void ParticleFilter::update_candidates(const std::vector<int> &count, const std::vector<double> &cand)
 {
-    gmm.upload();
+    gmm.push();
     std::vector<int32_t> c(count.begin(), count.end());
     const uint32_t u1 = draw_u32(), u2 = draw_u32();
     last_status_ = ok(pf_step_candidates(h_, c.data(), cand.data(), u1, u2, nullptr, nullptr, nullptr, nullptr));
+    gmm.device_changed();
 }
 
 Mat ParticleFilter::getEstimator()
 {
-    const int d = gmm.cov.empty() ? 0 : gmm.cov[0].rows;
-    Mat e(d, 1);
+    gmm.push();
+    Mat e(gmm.dims(), 1);
     ok(pf_estimate(h_, e.ptr()));
     return e;
 }
 
 Mat ParticleFilter::getSamples(Mat H, Mat M, int N, double scale)
 {
+    gmm.push();
     std::vector<double> n((size_t)2 * N);
     normals_from(seed_, 1, 2 * draws_, n.size(), n.data());
     draws_ += (uint64_t)N;
@@ -228,6 +253,7 @@ void ParticleFilter::getSampleProb(Mat H, Mat M, Mat input1, Mat input2, std::ve
                                    std::vector<double> &weight2, double scale)
 {
     if (input1.cols != input2.cols) throw PfException(PF_ERR_ARG, "getSampleProb: inputs differ in size");
+    gmm.push();
     weight1.assign((size_t)input1.cols, 0.0);
     weight2.assign((size_t)input2.cols, 0.0);
     ok(pf_get_sample_prob(h_, H.ptr(), M.ptr(), H.rows, input1.ptr(), input2.ptr(), input1.cols, scale,
@@ -247,31 +273,116 @@ namespace pf2d {
 
 void my_gmm::loadGaussian(Mat mean_, Mat sigma_, double weight_)
 {
-    mean.push_back(mean_);
-    sigma.push_back(sigma_);
-    weight.push_back(weight_);
-    ++N;
     const int d = sigma_.rows;
-    std::vector<double> mu((size_t)N * d), S((size_t)N * d * d);
-    for (int k = 0; k < N; ++k) {
-        for (int i = 0; i < d; ++i) mu[(size_t)k * d + i] = mean[k].data[i];
-        for (int i = 0; i < d * d; ++i) S[(size_t)k * d * d + i] = sigma[k].data[i];
+    if (sigma_.cols != d || mean_.rows * mean_.cols != d)
+        throw PfException(PF_ERR_ARG, "pf2D loadGaussian: mean 1 x d and sigma d x d");
+    // invert(s, sigma_i, DECOMP_CHOLESKY) and det(s) from the same factor
+    std::vector<double> L((size_t)d * d, 0.0);
+    for (int j = 0; j < d; ++j) {
+        double s = sigma_(j, j);
+        for (int q = 0; q < j; ++q) s -= L[(size_t)j * d + q] * L[(size_t)j * d + q];
+        if (!(s > 0.0)) throw PfException(PF_ERR_NOT_SPD, "pf2D loadGaussian: sigma is not positive definite");
+        L[(size_t)j * d + j] = std::sqrt(s);
+        for (int i = j + 1; i < d; ++i) {
+            double t = 0.5 * (sigma_(i, j) + sigma_(j, i));
+            for (int q = 0; q < j; ++q) t -= L[(size_t)i * d + q] * L[(size_t)j * d + q];
+            L[(size_t)i * d + j] = t / L[(size_t)j * d + j];
+        }
     }
-    ok(pf2d_load_gmm(pf_->h_, N, mu.data(), S.data(), weight.data()));
+    Mat inv(d, d);
+    std::vector<double> y((size_t)d), xx((size_t)d);
+    for (int c = 0; c < d; ++c) {
+        for (int i = 0; i < d; ++i) {
+            double t = (i == c) ? 1.0 : 0.0;
+            for (int q = 0; q < i; ++q) t -= L[(size_t)i * d + q] * y[q];
+            y[i] = t / L[(size_t)i * d + i];
+        }
+        for (int i = d - 1; i >= 0; --i) {
+            double t = y[i];
+            for (int q = i + 1; q < d; ++q) t -= L[(size_t)q * d + i] * xx[q];
+            xx[i] = t / L[(size_t)i * d + i];
+        }
+        for (int i = 0; i < d; ++i) inv(i, c) = xx[i];
+    }
+    double logdet = 0.0;
+    for (int i = 0; i < d; ++i) logdet += 2.0 * std::log(L[(size_t)i * d + i]);
+    ++N;
+    mean.push_back(mean_);
+    sigma_i.push_back(inv);
+    det_s.push_back(std::exp(-0.5 * d * std::log(6.283185307179586) - 0.5 * logdet));
+    weight.push_back(weight_);
 }
+
+ParticleFilter::ParticleFilter() {}
 
 ParticleFilter::ParticleFilter(int numParticles, int numDims, bool side1, uint64_t seed, int device)
     : N_(numParticles), d_(numDims), seed_(seed)
 {
     ok(pf2d_create(&h_, numParticles, numDims, side1 ? 1 : 0, device));
-    gmm.pf_ = this;
     std::vector<double> u = uniforms((size_t)N_ * d_);
     ok(pf2d_init(h_, u.data()));
+}
+
+ParticleFilter::ParticleFilter(ParticleFilter &&o) noexcept
+    : gmm(std::move(o.gmm)), h_(o.h_), N_(o.N_), d_(o.d_), seed_(o.seed_), draws_(o.draws_),
+      gmm_sent_(std::move(o.gmm_sent_))
+{
+    o.h_ = nullptr;
+}
+
+ParticleFilter &ParticleFilter::operator=(ParticleFilter &&o) noexcept
+{
+    if (this != &o) {
+        if (h_) pf2d_destroy(h_);
+        gmm = std::move(o.gmm);
+        h_ = o.h_;
+        N_ = o.N_;
+        d_ = o.d_;
+        seed_ = o.seed_;
+        draws_ = o.draws_;
+        gmm_sent_ = std::move(o.gmm_sent_);
+        o.h_ = nullptr;
+    }
+    return *this;
 }
 
 ParticleFilter::~ParticleFilter()
 {
     if (h_) pf2d_destroy(h_);
+}
+
+void ParticleFilter::need_handle(const char *what) const
+{
+    if (!h_) throw PfException(PF_ERR_STATE, std::string("pf2D ") + what + ": the filter has no particles");
+}
+
+// Uploads the gmm members when they differ from the last upload.
+void ParticleFilter::sync_gmm()
+{
+    const int K = gmm.N;
+    if (K <= 0) throw PfException(PF_ERR_STATE, "pf2D update: no GMM components loaded");
+    if ((int)gmm.mean.size() < K || (int)gmm.sigma_i.size() < K || (int)gmm.det_s.size() < K ||
+        (int)gmm.weight.size() < K)
+        throw PfException(PF_ERR_ARG, "pf2D update: gmm.N exceeds its members");
+    std::vector<double> mu((size_t)K * d_), Si((size_t)K * d_ * d_), ds(gmm.det_s.begin(), gmm.det_s.begin() + K),
+        w(gmm.weight.begin(), gmm.weight.begin() + K);
+    for (int k = 0; k < K; ++k) {
+        const Mat &m = gmm.mean[k], &S = gmm.sigma_i[k];
+        if (m.rows * m.cols != d_ || S.rows != d_ || S.cols != d_)
+            throw PfException(PF_ERR_ARG, "pf2D update: gmm component " + std::to_string(k) + " has the wrong shape");
+        std::copy(m.data.begin(), m.data.end(), mu.begin() + (size_t)k * d_);
+        std::copy(S.data.begin(), S.data.end(), Si.begin() + (size_t)k * d_ * d_);
+    }
+    std::vector<double> key;
+    key.reserve(1 + mu.size() + Si.size() + 2 * (size_t)K);
+    key.push_back((double)K);
+    key.insert(key.end(), mu.begin(), mu.end());
+    key.insert(key.end(), Si.begin(), Si.end());
+    key.insert(key.end(), ds.begin(), ds.end());
+    key.insert(key.end(), w.begin(), w.end());
+    if (key == gmm_sent_) return;
+    ok(pf2d_load_gmm_precision(h_, K, mu.data(), Si.data(), ds.data(), w.data()));
+    gmm_sent_.swap(key);
 }
 
 std::vector<double> ParticleFilter::uniforms(size_t n)
@@ -292,9 +403,18 @@ std::vector<double> ParticleFilter::normals(size_t n)
     return out;
 }
 
+void ParticleFilter::predict()
+{
+    need_handle("predict");
+    std::vector<double> n = normals((size_t)N_ * 8);
+    ok(pf2d_predict(h_, n.data()));
+}
+
 void ParticleFilter::update(Mat measurement)
 {
+    need_handle("update");
     if (measurement.rows != 2 || measurement.cols != 2) throw PfException(PF_ERR_ARG, "pf2D update: measurement is 2 x 2");
+    sync_gmm();
     uint32_t u;
     philox_u32(seed_, 2, draws_++, 1, &u);
     std::vector<double> n = normals((size_t)N_ * 8), r = uniforms((size_t)N_ * d_);
@@ -303,6 +423,7 @@ void ParticleFilter::update(Mat measurement)
 
 Mat ParticleFilter::getEstimator()
 {
+    need_handle("getEstimator");
     Mat e(1, d_);
     ok(pf2d_estimate(h_, e.ptr()));
     return e;
@@ -367,7 +488,13 @@ PoseFrame PFTracker::run(const uint8_t *likelihood, const int32_t *roi, bool upd
     f.tf = Mat::zeros(8, 3);
     f.pose2d = Mat::zeros(8, 2);
     pf_tracker_outputs o = {f.lml, pose.data(), p3.data(), f.tf.ptr(), f.pose2d.ptr()};
+    pf1_->gmm.push();
+    pf2_->gmm.push();
     f.status = update ? ok(pf_tracker_frame(tr_, likelihood, roi, &o)) : ok(pf_tracker_pose(tr_, &o));
+    if (update) {
+        pf1_->gmm.device_changed();
+        pf2_->gmm.device_changed();
+    }
     f.e1 = Mat(1, npose_, pose.data());
     f.e2 = Mat(1, npose_, pose.data() + npose_);
     f.p3D1 = Mat(3, 5, p3.data());

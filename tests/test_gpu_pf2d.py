@@ -98,3 +98,77 @@ def test_pf2d_reference_class_runs(gpu):
         pf.update(np.array([[400.0, 200.0], [100.0, 300.0]]))
     e = pf.getEstimator()
     assert e.shape == (1, 8) and np.all(np.isfinite(e))
+
+
+def test_pf2d_predict_is_the_reference_random_walk(gpu):
+    """pf2d_predict on its own (ParticleFilter::predict, pf2D.cpp:90-102):
+    x_i += 5 n_i with the normals consumed for columns 2..5, 6, 7, 0, 1; the
+    weights are kept."""
+    N = 3000
+    mu, S, w = _gmm(4)
+    g = Pf2d(N, 8, side=0)
+    g.load_gmm(mu, S, w)
+    rng = np.random.default_rng(4)
+    g.init(rng.random((N, 8)))
+    st, _ = g.update(np.array([[400.0, 200.0], [100.0, 300.0]]), 99, rng.standard_normal((N, 8)), rng.random((N, 8)))
+    pin, win = g.get_particles()
+    normals = rng.standard_normal((N, 8))
+    g.predict(normals)
+    pg, wg = g.get_particles()
+    expect = pin.copy()
+    for tt, c in enumerate([2, 3, 4, 5, 6, 7, 0, 1]):
+        expect[:, c] += 5.0 * normals[:, tt]
+    assert np.allclose(pg, expect, rtol=1e-6, atol=1e-4)
+    assert np.array_equal(wg, win)
+    g.close()
+
+
+def test_pf2d_precision_form_loads_the_same_prior(gpu):
+    """The GMM given as the members the reference's update reads (mean,
+    sigma_i, det_s, weight; pf2D.cpp:159-163) weighs the particles as the
+    covariance form does, and both match the oracle."""
+    N = 2000
+    mu, S, w = _gmm(5)
+    og = _oracle_gmm(mu, S, w)
+    rng = np.random.default_rng(5)
+    unif = rng.random((N, 8))
+    meas = np.array([[420.0, 180.0], [200.0, 260.0]])
+    normals = rng.standard_normal((N, 8)); reinit = rng.random((N, 8))
+    outs = []
+    for form in ("cov", "precision"):
+        g = Pf2d(N, 8, side=1)
+        if form == "cov":
+            g.load_gmm(mu, S, w)
+        else:
+            g.load_gmm_precision(mu, og.Sinv, og.cnorm, w)
+        g.init(unif)
+        st, out = g.update(meas, 1234, normals, reinit)
+        outs.append(out["lw"].astype(np.float64))
+        g.close()
+    ref = og.update(oracle.pf2d_init(N, 8, 1, unif), meas, 1234, normals, reinit, side=1)
+    for lw in outs:
+        assert np.all(np.abs(lw - ref["lw"]) <= 1e-3 * np.maximum(1.0, np.abs(ref["lw"]) / 100))
+    assert np.all(np.abs(outs[0] - outs[1]) <= 1e-5 * np.maximum(1.0, np.abs(outs[0])))
+
+
+def test_pf2d_default_constructor_and_predict(gpu):
+    """ParticleFilter() (pf2D.h:29) has no particles; the sized filter's
+    predict() and update() read the gmm members as they are at the call."""
+    from mkfbodytracker_pdaf_amd import capi
+    pf0 = ParticleFilter()
+    with pytest.raises(capi.PfError) as e:
+        pf0.update(np.zeros((2, 2)))
+    assert e.value.code == capi.PF_ERR_STATE
+    pf = ParticleFilter(1000, 8, False, seed=8)
+    mu, S, w = _gmm(6, K=3)
+    for k in range(3):
+        pf.gmm.loadGaussian(mu[k], S[k], w[k])
+    pf.update(np.array([[300.0, 200.0], [100.0, 300.0]]))
+    p0, _ = pf._h.get_particles()
+    pf.predict()
+    p1, _ = pf._h.get_particles()
+    dlt = p1 - p0
+    assert np.all(np.abs(dlt.mean(axis=0)) < 0.8) and np.all(np.abs(dlt.std(axis=0) - 5.0) < 0.6)
+    pf.gmm.det_s[0] *= 2.0
+    pf.update(np.array([[300.0, 200.0], [100.0, 300.0]]))
+    assert np.all(np.isfinite(pf.getEstimator()))
