@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="default workload: skip the adjoint measurement")
     ap.add_argument("--block", type=int, default=0, help="step kernel block size (0 = default)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this size (PMC calibration)")
@@ -76,6 +77,8 @@ def main():
                          "sequences in one persistent step per frame, tuned vs default schedule")
     ap.add_argument("--sequences", type=int, default=256, help="batched: number of sequences")
     ap.add_argument("--tune-frames", type=int, default=32, help="batched: frames timed per candidate schedule")
+    ap.add_argument("--scaled-variants", type=str, default="valu,quad_valu,mfma",
+                    help="scaled: kernels to run, of valu, quad_valu, mfma (the line's value is mfma's)")
     ap.add_argument("--scaled-shape", type=str, default="32,31250,1000",
                     help="scaled: sequences,particles,frames (the measured workload is 32,31250,1000)")
     ap.add_argument("--tracker-shape", type=str, default="100000,400",
@@ -158,12 +161,7 @@ def main():
     bpp = frame_bytes_per_particle(mdl.d)
     achieved = bpp * N / step_avg_s / 1e9
 
-    if args.calib_copy_mb:
-        nbytes = args.calib_copy_mb << 20
-        a = DeviceArray(np.zeros(nbytes // 4, np.float32))
-        b = DeviceArray.empty(nbytes // 4, np.float32)
-        b.copy_from(a)
-        synchronize()
+    calib_copy(args.calib_copy_mb)
 
     traffic = measured_traffic(N, mdl.d, mdl.K)
     evals = N * K
@@ -208,7 +206,33 @@ def main():
                                                         threads=host_cores())
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
         out["speedup_vs_cpu_all_cores"] = out["value"] / out["cpu_baseline"]["all_cores"]["value"]
+    if not args.no_adjoint:
+        # the adjoint workload, measured after the headline's timed region
+        rb.close()
+        m = measure_adjoint(args.adj_shape)
+        out["adjoint_over_forward"] = m["t_grad"] / m["t_fwd"]
+        out["adjoint"] = {"workload": "gradient of the summed log-marginal-likelihood: %d sequences x %d frames x "
+                                      "N=%d, K=15, d=12, checkpoint every %d frames" % (m["S"], m["T"], m["N"], m["C"]),
+                          "forward_s": m["t_fwd"], "gradient_s": m["t_grad"],
+                          "gradient_recorded_forward_ms": m["rec_fwd_ms"], "gradient_backward_ms": m["bwd_ms"],
+                          "lml_identical_in_gradient_run": bool(np.array_equal(m["lml_fwd"], m["lml_grad"])),
+                          "timing": "wall clock of the second of two runs each, after the headline's timed region"}
     print(json.dumps(out))
+
+
+def calib_copy(mb: int) -> None:
+    """One device-to-device copy of mb MiB (the byte counters' calibration
+    under rocprofv3 --pmc); nothing when mb is 0."""
+    if not mb:
+        return
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray, synchronize
+    nbytes = mb << 20
+    a = DeviceArray(np.zeros(nbytes // 4, np.float32))
+    b = DeviceArray.empty(nbytes // 4, np.float32)
+    b.copy_from(a)
+    synchronize()
+    a.close()
+    b.close()
 
 
 def host_cores() -> int:
@@ -330,17 +354,18 @@ def run_batched(args):
     print(json.dumps(out))
 
 
-def run_adjoint(args):
-    """Gradient of the summed log-marginal-likelihood of 64 sequences x 500
-    frames x N = 16,384 particles (K=15, d=12, own GMM per sequence) w.r.t.
-    means, logit weights, reversion rates and Cholesky factors, checkpointing
-    every 50 frames; timed against the plain (hipGraph) forward pass."""
+def measure_adjoint(shape: str) -> dict:
+    """Gradient of the summed log-marginal-likelihood of S sequences x T
+    frames x N particles (K=15, d=12, own GMM per sequence) w.r.t. means,
+    logit weights, reversion rates and Cholesky factors, checkpointing every
+    50 frames, and the plain (hipGraph) forward pass of the same inputs; the
+    second of two runs of each is timed."""
     from mkfbodytracker_pdaf_amd import synthetic
     from mkfbodytracker_pdaf_amd.batch import RbpfBatch
     from mkfbodytracker_pdaf_amd.devmem import DeviceArray, synchronize
     from mkfbodytracker_pdaf_amd.rbpf import Rbpf
 
-    S, N, T = (int(v) for v in args.adj_shape.split(","))
+    S, N, T = (int(v) for v in shape.split(","))
     seed, C, cap, cpj = 11, 50, 16, 8
     lam = (cpj - 1) / (640.0 * 480.0)
     models = [synthetic.make_model(seed=seed * 1000 + s) for s in range(S)]
@@ -382,6 +407,17 @@ def run_adjoint(args):
         t_grad = time.perf_counter() - t0
     rec_fwd_ms, bwd_ms = b.last_adjoint_ms()
     lml_grad = d_lml.get()
+    b.close()
+    return dict(S=S, N=N, T=T, C=C, t_fwd=t_fwd, t_grad=t_grad, rec_fwd_ms=rec_fwd_ms, bwd_ms=bwd_ms,
+                lml_fwd=lml_fwd, lml_grad=lml_grad, g=g)
+
+
+def run_adjoint(args):
+    """The adjoint workload (measure_adjoint) as its own bench line."""
+    m = measure_adjoint(args.adj_shape)
+    S, N, T, C = m["S"], m["N"], m["T"], m["C"]
+    t_fwd, t_grad, rec_fwd_ms, bwd_ms = m["t_fwd"], m["t_grad"], m["rec_fwd_ms"], m["bwd_ms"]
+    lml_fwd, lml_grad, g = m["lml_fwd"], m["lml_grad"], m["g"]
     evals = float(S) * N * T
     out = {
         "metric": "adjoint gradient time / forward time (log-marginal-likelihood, batched)",
@@ -446,7 +482,9 @@ def run_scaled(args):
     evals = float(S) * N * T
     bpp = step_bytes_per_particle(d)
     variants = {}
-    for v in (capi.PF_BATCH_VALU, capi.PF_BATCH_QUAD_VALU, capi.PF_BATCH_MFMA):
+    by_name = {n: v for v, n in names.items()}
+    chosen = [by_name[n] for n in args.scaled_variants.split(",")]
+    for v in chosen:
         b.variant = v
         for rep in range(2):                      # warm-up run, then the timed run
             for s in range(S):
@@ -459,7 +497,9 @@ def run_scaled(args):
         variants[names[v]] = {"evals_per_s": evals / dt, "wall_s": dt, "device_s": b.last_run_ms() / 1e3,
                               "ms_per_frame": dt / T * 1e3, "hbm_frac": bpp * evals / dt / 1e9 / HBM_PEAK_GBS,
                               "lml_total": float(np.sum(d_lml.get()))}
-    dt = variants["mfma"]["wall_s"]
+    calib_copy(args.calib_copy_mb)
+    head = "mfma" if "mfma" in variants else next(iter(variants))
+    dt = variants[head]["wall_s"]
     achieved = bpp * evals / dt / 1e9
     out = {
         "metric": "likelihood evals/s, scaled state (d=24, K=64, 5 PDAF joints), batched",
@@ -479,13 +519,17 @@ def run_scaled(args):
                                "(v_mfma_f32_4x4x1f32, four lanes per particle)" % (S, N, T),
                    "sequences": S, "particles": N, "frames": T, "state_dim": d, "components": K},
         "frames_per_s": T / dt,
-        "lml_total": variants["mfma"]["lml_total"],
+        "lml_total": variants[head]["lml_total"],
         "variants": variants,
-        "mfma_over_valu": variants["valu"]["wall_s"] / variants["mfma"]["wall_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_batch_step_q24<5, mfma> + k_batch_weights (whole run)", "bytes_per_particle": bpp},
+                     "kernel": "k_batch_step_q24<5, %s> + k_batch_weights (whole run)" % head,
+                     "bytes_per_particle": bpp},
     }
+    if "mfma" in variants:
+        for other in ("valu", "quad_valu"):
+            if other in variants:
+                out["mfma_over_" + other] = variants[other]["wall_s"] / variants["mfma"]["wall_s"]
     print(json.dumps(out))
 
 

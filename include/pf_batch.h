@@ -37,6 +37,19 @@ int pf_batch_create(pf_batch **out, int32_t S, const int32_t *n_particles, const
                     int32_t K, int32_t d, int32_t m, const pf_config *cfg);
 int pf_batch_destroy(pf_batch *b);
 
+/* The layout pf_batch_create would build for these sequences and a schedule
+ * of block threads x ppl particles per tile, validated without a GPU: the
+ * particle, block and frame offsets and the tile and weight-block tables
+ * with their per-frame active counts are checked against the invariants the
+ * kernels' indexing relies on (disjoint aligned ranges inside the buffers,
+ * every tile and block inside its sequence, only sequences still running
+ * active at a frame).  summary (optional, 6 values): particle slots (ld),
+ * resampler blocks, tiles, weight blocks, largest per-sequence block count,
+ * longest sequence.  pf_batch_create and pf_batch_set_schedule run the same
+ * check.  Host only. */
+int pf_batch_plan(int32_t S, const int32_t *n_particles, const int32_t *n_frames, int32_t block, int32_t ppl,
+                  int64_t *summary);
+
 /* my_gmm::loadGaussian x K for sequence s (arguments as pf_load_model). */
 int pf_batch_load_model(pf_batch *b, int32_t s, int32_t npose, const int32_t *sel, const double *means,
                         const double *covs, const double *weights, const double *gamma,

@@ -156,3 +156,13 @@ class RbpfBatch:
         f, b = C.c_double(0.0), C.c_double(0.0)
         check(self.lib.pf_batch_last_adjoint_ms(self.h, C.byref(f), C.byref(b)))
         return f.value, b.value
+
+
+def plan(n_particles, n_frames, block: int = 256, ppl: int = 1) -> dict:
+    """The validated layout of a batch of these sequences under a schedule
+    (pf_batch_plan; host only, no GPU needed)."""
+    N = capi.i32(n_particles)
+    T = capi.i32(n_frames)
+    out = np.zeros(6, np.int64)
+    check(capi.lib().pf_batch_plan(len(N), ptr(N), ptr(T), int(block), int(ppl), ptr(out)))
+    return dict(zip(("ld", "blocks", "tiles", "weight_blocks", "nb_max", "Tmax"), (int(v) for v in out)))

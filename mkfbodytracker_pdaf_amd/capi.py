@@ -34,7 +34,7 @@ EXPORTS = [
     "pf_yml_read_matrix", "pf_read_camera_info", "pf_load_model_yml", "pf_philox_u32",
     "pf2d_create", "pf2d_destroy", "pf2d_load_gmm", "pf2d_load_gmm_precision", "pf2d_init", "pf2d_set_particles",
     "pf2d_get_particles", "pf2d_update", "pf2d_predict", "pf2d_estimate",
-    "pf_batch_create", "pf_batch_destroy", "pf_batch_load_model", "pf_batch_set_joint_clutter",
+    "pf_batch_create", "pf_batch_destroy", "pf_batch_plan", "pf_batch_load_model", "pf_batch_set_joint_clutter",
     "pf_batch_reset", "pf_batch_run_dev", "pf_batch_get_tracks", "pf_batch_estimate",
     "pf_batch_set_tracks", "pf_batch_get_last_frame", "pf_batch_set_variant", "pf_batch_get_variant",
     "pf_batch_get_status",
@@ -138,6 +138,7 @@ def lib():
     _sig(L, "pf2d_estimate", C.c_int, _vp, _vp)
     _sig(L, "pf_batch_create", C.c_int, C.POINTER(_vp), i32, _vp, _vp, i32, i32, i32, C.POINTER(Config))
     _sig(L, "pf_batch_destroy", C.c_int, _vp)
+    _sig(L, "pf_batch_plan", C.c_int, i32, _vp, _vp, i32, i32, _vp)
     _sig(L, "pf_batch_load_model", C.c_int, _vp, i32, i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
     _sig(L, "pf_batch_set_joint_clutter", C.c_int, _vp, _vp)
     _sig(L, "pf_batch_reset", C.c_int, _vp, i32, _vp)

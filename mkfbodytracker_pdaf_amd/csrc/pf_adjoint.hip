@@ -65,6 +65,7 @@ struct AdjArgs {
     const int *bins;                   // initial components (reset)
     double *tacc;                      // per tile, K x kGV
     int t, t0;
+    int kc;                            // components per group of the deterministic term reduction
 };
 
 // Sum over the 16 lanes of each row of the wave, in every lane of the row,
@@ -79,50 +80,73 @@ __device__ __forceinline__ float row_sum16(float v)
     return v;
 }
 
-// Reduction of the per-particle component terms into the block's LDS
-// accumulator acc (K x kGV, zeroed by the caller; mask = components present):
-// per distinct component of the wave, row sums by DPP, then one LDS atomic
-// add per row (4 per value and wave).
-__device__ __forceinline__ void reduce_terms(bool live, int k, const float (&term)[kGV], float *acc,
-                                             unsigned long long *mask)
+// LDS floats of the term reduction's wave slots for groups of kc components
+// (nwaves x kc x kGV values, then nwaves 64-bit masks).
+__host__ __device__ constexpr int term_slot_floats(int kc, int nwaves)
 {
-    const int lane = threadIdx.x & 63;
-    const bool row_lead = (lane & 15) == 0;
-    unsigned long long rem = __ballot(live);
-    while (rem) {
-        const int leader = __ffsll((long long)rem) - 1;
-        const int kk = __shfl(k, leader, 64);
-        const bool sel = live && k == kk;
-        rem &= ~__ballot(sel);
-        float *dst = acc + kk * kGV;
-        constexpr int kChunk = 8;
+    return ((nwaves * kc * kGV + 1) & ~1) + 2 * nwaves;
+}
+
+// Sum of v over the wave by a fixed tree (rows of 16 lanes by DPP, then the
+// four rows), the same in every lane and in every run.
+__device__ __forceinline__ float wave_sum_fixed(float v)
+{
+    v = row_sum16(v);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// The per-particle component terms of a tile added to its FP64 accumulator
+// tacc (K x kGV) in an order fixed by the tile alone, so gradients are
+// bit-reproducible: components in groups of kc; per distinct component of
+// the group in a wave, the wave's sums (fixed tree) go to that wave's LDS
+// slot; after a barrier the slots are added in wave order, in FP64, to tacc.
+// Every thread of the block calls it.
+__device__ __forceinline__ void reduce_terms(bool live, int k, const float (&term)[kGV], float *slots, double *tacc,
+                                             int K, int kc)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    unsigned long long *wmask = reinterpret_cast<unsigned long long *>(slots + ((nw * kc * kGV + 1) & ~1));
+    for (int g0 = 0; g0 < K; g0 += kc) {
+        const int g1 = min(K, g0 + kc);
+        unsigned long long rem = __ballot(live && k >= g0 && k < g1), mine = 0;
+        while (rem) {
+            const int leader = __ffsll((long long)rem) - 1;
+            const int kk = __shfl(k, leader, 64);
+            const bool sel = live && k == kk;
+            rem &= ~__ballot(sel);
+            float *dst = slots + ((size_t)wid * kc + (kk - g0)) * kGV;
+            constexpr int kChunk = 8;
 #pragma unroll
-        for (int v0 = 0; v0 < kGV; v0 += kChunk) {
-            float r[kChunk];
-#pragma unroll
-            for (int i = 0; i < kChunk; ++i)
-                if (v0 + i < kGV) r[i] = row_sum16(sel ? term[v0 + i] : 0.f);
-            if (row_lead) {
+            for (int v0 = 0; v0 < kGV; v0 += kChunk) {
+                float r[kChunk];
 #pragma unroll
                 for (int i = 0; i < kChunk; ++i)
-                    if (v0 + i < kGV) atomicAdd(dst + v0 + i, r[i]);
+                    if (v0 + i < kGV) r[i] = wave_sum_fixed(sel ? term[v0 + i] : 0.f);
+                if (lane == 0) {
+#pragma unroll
+                    for (int i = 0; i < kChunk; ++i)
+                        if (v0 + i < kGV) dst[v0 + i] = r[i];
+                }
             }
+            mine |= 1ull << (kk - g0);
         }
-        if (lane == 0) atomicOr(mask, 1ull << kk);
+        if (lane == 0) wmask[wid] = mine;
+        __syncthreads();
+        for (int i = threadIdx.x; i < (g1 - g0) * kGV; i += blockDim.x) {
+            const int c = i / kGV;
+            double sum = 0.0;
+            bool any = false;
+            for (int w = 0; w < nw; ++w)
+                if ((wmask[w] >> c) & 1ull) {
+                    sum += (double)slots[((size_t)w * kc + c) * kGV + (i - c * kGV)];
+                    any = true;
+                }
+            if (any) tacc[(size_t)g0 * kGV + i] += sum;
+        }
+        __syncthreads();
     }
-}
-
-__device__ __forceinline__ void flush_terms(const float *acc, const unsigned long long *mask, double *tacc, int K)
-{
-    const unsigned long long m = *mask;
-    for (int i = threadIdx.x; i < K * kGV; i += blockDim.x)
-        if ((m >> (i / kGV)) & 1ull) tacc[i] += (double)acc[i];
-}
-
-__device__ __forceinline__ void clear_terms(float *acc, unsigned long long *mask, int K)
-{
-    for (int i = threadIdx.x; i < K * kGV; i += blockDim.x) acc[i] = 0.f;
-    if (threadIdx.x == 0) *mask = 0ull;
 }
 
 __device__ __forceinline__ void load_state(const float *xs, const float *Ps, int ld, int col, float (&x)[D],
@@ -217,8 +241,7 @@ __global__ void __launch_bounds__(kAdjTile) k_adj_step(AdjArgs a)
     float *sm = reinterpret_cast<float *>(sm4);
     float *smu = sm + ba.stage_floats;
     float *ssg = smu + K * D;
-    float *acc = ssg + K * NP;
-    unsigned long long *mask = reinterpret_cast<unsigned long long *>(acc + ((K * kGV + 1) & ~1));
+    float *slots = ssg + K * NP;
     const StepArgs v = seq_view(ba, sd);
     const float *fcand = v.cand + (size_t)a.t * ba.J * ba.cap * 2;
     CandPrefetch cpf;
@@ -227,7 +250,6 @@ __global__ void __launch_bounds__(kAdjTile) k_adj_step(AdjArgs a)
     cand_store(cpf, sm + ba.table_floats, fcand, ba.J, ba.cap);
     for (int i = threadIdx.x; i < K * D; i += blockDim.x) smu[i] = a.mus[(size_t)s * K * D + i];
     for (int i = threadIdx.x; i < K * NP; i += blockDim.x) ssg[i] = a.sigs[(size_t)s * K * NP + i];
-    clear_terms(acc, mask, K);
     __syncthreads();
 
     const int t = a.t;
@@ -401,9 +423,7 @@ __global__ void __launch_bounds__(kAdjTile) k_adj_step(AdjArgs a)
         term[D + NP + 1] = wbar;
         store_state(a.adj_out_x, a.adj_out_P, ba.ld, sd.pofs + j, xb, Pb);
     }
-    reduce_terms(live, k, term, acc, mask);
-    __syncthreads();
-    flush_terms(acc, mask, a.tacc + (size_t)tile * K * kGV, K);
+    reduce_terms(live, k, term, slots, a.tacc + (size_t)tile * K * kGV, K, a.kc);
 }
 
 // Initial states x_0j = mu'_{bins_j}, P_0j = Sigma'_{bins_j}: the adjoint of
@@ -417,10 +437,7 @@ __global__ void __launch_bounds__(kAdjTile) k_adj_reset(AdjArgs a)
     const int s = a.t_seq[tile];
     const int j0 = a.t_j0[tile];
     const SeqDesc sd = ba.seqs[s];
-    float *acc = reinterpret_cast<float *>(sm4);
-    unsigned long long *mask = reinterpret_cast<unsigned long long *>(acc + ((K * kGV + 1) & ~1));
-    clear_terms(acc, mask, K);
-    __syncthreads();
+    float *slots = reinterpret_cast<float *>(sm4);
     const int j = j0 + (int)threadIdx.x;
     const bool live = j < sd.N;
     float term[kGV];
@@ -436,9 +453,7 @@ __global__ void __launch_bounds__(kAdjTile) k_adj_reset(AdjArgs a)
 #pragma unroll
         for (int p = 0; p < NP; ++p) term[D + p] = Pb[p];
     }
-    reduce_terms(live, k, term, acc, mask);
-    __syncthreads();
-    flush_terms(acc, mask, a.tacc + (size_t)tile * K * kGV, K);
+    reduce_terms(live, k, term, slots, a.tacc + (size_t)tile * K * kGV, K, a.kc);
 }
 
 // Per-sequence sums of the tile accumulators (fixed order, FP64).
@@ -543,7 +558,8 @@ int launch_adj(pf_batch *b, const AdjArgs &a, int grid, bool replay, hipStream_t
 {
     if (grid <= 0) return PF_OK;
     const size_t lds_replay = 4 * (size_t)b->stage_floats();
-    const size_t lds_step = 4 * ((size_t)b->stage_floats() + (size_t)b->K * (D + NP) + (((size_t)b->K * kGV + 1) & ~1)) + 8;
+    const size_t lds_step = 4 * ((size_t)b->stage_floats() + (size_t)b->K * (D + NP) +
+                                 (size_t)term_slot_floats(a.kc, kAdjTile / 64));
     switch (b->J) {
     case 1: launch_adj_t<1>(a, grid, lds_replay, lds_step, replay, st); break;
     case 2: launch_adj_t<2>(a, grid, lds_replay, lds_step, replay, st); break;
@@ -641,6 +657,11 @@ int pf_batch_lml_grad(pf_batch *b, const int32_t *count, const float *cand, cons
     a.t_j0 = b->t256_j0;
     a.bins = b->rec_bins;
     a.tacc = b->tacc;
+    {
+        // components per reduction group: the step kernel's LDS within 64 KB
+        const long avail = 16384L - (long)b->stage_floats() - (long)b->K * (D + NP) - 2 * (kAdjTile / 64) - 1;
+        a.kc = (int)std::max(1L, std::min<long>({(long)b->K, 64L, avail / ((kAdjTile / 64) * kGV)}));
+    }
     // Segments, last first.  The replay of segment s-1 (and its child ranges)
     // runs on the second stream while the reverse kernels of segment s run on
     // the first; segment buffers alternate by segment parity.
@@ -711,7 +732,7 @@ int pf_batch_lml_grad(pf_batch *b, const int32_t *count, const float *cand, cons
     }
     a.adj_in_x = b->adj_x[0];
     a.adj_in_P = b->adj_P[0];
-    const size_t lds_reset = 4 * (((size_t)b->K * kGV + 1) & ~1) + 8;
+    const size_t lds_reset = 4 * (size_t)term_slot_floats(a.kc, kAdjTile / 64);
     hipLaunchKernelGGL(k_adj_reset, dim3(b->t256_total), dim3(kAdjTile), lds_reset, b->stream, a);
     HIP_TRYB(hipGetLastError());
     const int ng = b->S * b->K * kGV;

@@ -244,28 +244,137 @@ int bcheck(pf_batch *b)
     return PF_OK;
 }
 
+// Host-side layout of a batch: sequences in rank order (longest first),
+// their particle, block and frame offsets, and for a tile size the tile and
+// weight-block tables with the active counts of every frame.
+struct BatchLayout {
+    std::vector<int> order;                 // rank -> sequence
+    std::vector<SeqDesc> seqs;              // by rank
+    int ld = 0, nb_max = 0, Tmax = 0, nblocks = 0;
+    std::vector<int> ts, tj, wbs, wbb, nt, nw;
+};
+
+int make_layout(int S, const int32_t *N, const int32_t *T, BatchLayout &L)
+{
+    if (S <= 0 || S > (1 << 16)) return set_error(PF_ERR_ARG, "S=%d out of range", S);
+    for (int s = 0; s < S; ++s)
+        if (N[s] <= 0 || N[s] > kMaxScanBlocks * kWeightsBlock || T[s] <= 0)
+            return set_error(PF_ERR_ARG, "sequence %d: bad size (N=%d, T=%d)", s, N[s], T[s]);
+    L.order.resize(S);
+    std::iota(L.order.begin(), L.order.end(), 0);
+    std::stable_sort(L.order.begin(), L.order.end(), [&](int p, int q) { return T[p] > T[q]; });
+    std::vector<int> tofs(S);
+    for (int s = 0, acc = 0; s < S; ++s) { tofs[s] = acc; acc += T[s]; }
+    L.seqs.assign(S, SeqDesc{});
+    long long pofs = 0, bofs = 0;
+    for (int i = 0; i < S; ++i) {
+        const int s = L.order[i];
+        SeqDesc &sd = L.seqs[i];
+        sd.pofs = (int)pofs;
+        sd.N = N[s];
+        sd.T = T[s];
+        sd.tofs = tofs[s];
+        sd.nb = (sd.N + kWeightsBlock - 1) / kWeightsBlock;
+        sd.bofs = (int)bofs;
+        pofs += (sd.N + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+        bofs += sd.nb;
+        L.nb_max = std::max(L.nb_max, sd.nb);
+        L.Tmax = std::max(L.Tmax, sd.T);
+    }
+    if (pofs > (1LL << 28)) return set_error(PF_ERR_ARG, "batch too large (%lld particles)", pofs);
+    L.ld = (int)pofs;
+    L.nblocks = (int)bofs;
+    return PF_OK;
+}
+
+void make_tiles(BatchLayout &L, int tile)
+{
+    const int S = (int)L.seqs.size();
+    L.ts.clear(); L.tj.clear(); L.wbs.clear(); L.wbb.clear();
+    std::vector<int> first_tile(S + 1), first_wb(S + 1);
+    for (int i = 0; i < S; ++i) {
+        first_tile[i] = (int)L.ts.size();
+        first_wb[i] = (int)L.wbs.size();
+        const int n = L.seqs[i].N;
+        for (int j0 = 0; j0 < n; j0 += tile) { L.ts.push_back(i); L.tj.push_back(j0); }
+        for (int k = 0; k * kWeightsBlock < n; ++k) { L.wbs.push_back(i); L.wbb.push_back(k); }
+    }
+    first_tile[S] = (int)L.ts.size();
+    first_wb[S] = (int)L.wbs.size();
+    L.nt.assign((size_t)L.Tmax + 1, 0);
+    L.nw.assign((size_t)L.Tmax + 1, 0);
+    for (int t = 0; t <= L.Tmax; ++t) {
+        int act = 0;                       // sequences are sorted by length, longest first
+        while (act < S && L.seqs[act].T > t) ++act;
+        L.nt[t] = first_tile[act];
+        L.nw[t] = first_wb[act];
+    }
+}
+
+// The invariants the kernels' indexing relies on (every index they form from
+// these tables stays inside the buffers pf_batch_create allocates): disjoint,
+// aligned particle ranges inside ld; disjoint block ranges; every tile and
+// weight block inside its sequence, each particle and block covered once; at
+// frame t only tiles and blocks of sequences with T > t are active.
+int check_layout(const BatchLayout &L, int tile)
+{
+    const int S = (int)L.seqs.size();
+    long long pend = 0, bend = 0, tsum = 0;
+    for (int i = 0; i < S; ++i) {
+        const SeqDesc &sd = L.seqs[i];
+        if (sd.pofs % kSeqAlign || sd.pofs < pend || (long long)sd.pofs + sd.N > L.ld)
+            return set_error(PF_ERR_STATE, "layout: particle range of rank %d", i);
+        if (sd.bofs != bend || sd.nb != (sd.N + kWeightsBlock - 1) / kWeightsBlock || sd.nb > L.nb_max ||
+            sd.bofs + sd.nb > L.nblocks)
+            return set_error(PF_ERR_STATE, "layout: block range of rank %d", i);
+        if (i > 0 && sd.T > L.seqs[i - 1].T) return set_error(PF_ERR_STATE, "layout: ranks not by length");
+        pend = (long long)sd.pofs + sd.N;
+        bend = (long long)sd.bofs + sd.nb;
+        tsum += sd.T;
+    }
+    for (int i = 0; i < S; ++i)
+        if ((long long)L.seqs[i].tofs + L.seqs[i].T > tsum) return set_error(PF_ERR_STATE, "layout: frame offsets");
+    std::vector<long long> covered(S, 0), bcovered(S, 0);
+    for (size_t k = 0; k < L.ts.size(); ++k) {
+        const int i = L.ts[k], j0 = L.tj[k];
+        if (i < 0 || i >= S || (k > 0 && i < L.ts[k - 1])) return set_error(PF_ERR_STATE, "layout: tile %zu sequence", k);
+        if (j0 != covered[i] || j0 % tile || j0 >= L.seqs[i].N) return set_error(PF_ERR_STATE, "layout: tile %zu", k);
+        covered[i] = std::min<long long>(L.seqs[i].N, (long long)j0 + tile);
+    }
+    for (size_t k = 0; k < L.wbs.size(); ++k) {
+        const int i = L.wbs[k], bb = L.wbb[k];
+        if (i < 0 || i >= S || bb != bcovered[i] || bb >= L.seqs[i].nb)
+            return set_error(PF_ERR_STATE, "layout: weight block %zu", k);
+        ++bcovered[i];
+    }
+    for (int i = 0; i < S; ++i)
+        if (covered[i] != L.seqs[i].N || bcovered[i] != L.seqs[i].nb)
+            return set_error(PF_ERR_STATE, "layout: rank %d not covered", i);
+    for (int t = 0; t <= L.Tmax; ++t) {
+        const int nt = L.nt[t], nw = L.nw[t];
+        if (nt < 0 || nt > (int)L.ts.size() || nw < 0 || nw > (int)L.wbs.size() ||
+            (t > 0 && (nt > L.nt[t - 1] || nw > L.nw[t - 1])))
+            return set_error(PF_ERR_STATE, "layout: active counts at frame %d", t);
+        if (nt > 0 && L.seqs[L.ts[nt - 1]].T <= t) return set_error(PF_ERR_STATE, "layout: inactive tile at %d", t);
+        if (nw > 0 && L.seqs[L.wbs[nw - 1]].T <= t) return set_error(PF_ERR_STATE, "layout: inactive block at %d", t);
+        if (nt < (int)L.ts.size() && L.seqs[L.ts[nt]].T > t) return set_error(PF_ERR_STATE, "layout: active tile skipped at %d", t);
+        if (nw < (int)L.wbs.size() && L.seqs[L.wbs[nw]].T > t)
+            return set_error(PF_ERR_STATE, "layout: active block skipped at %d", t);
+    }
+    return PF_OK;
+}
+
 // Tile tables for the current schedule (tile = block * ppl particles).
 int build_tiles(pf_batch *b)
 {
     const int tile = b->sch.block * b->sch.ppl;
-    std::vector<int> ts, tj, wbs, wbb;
-    std::vector<int> first_tile(b->S + 1), first_wb(b->S + 1);
-    for (int i = 0; i < b->S; ++i) {
-        first_tile[i] = (int)ts.size();
-        first_wb[i] = (int)wbs.size();
-        const int n = b->N[b->order[i]];
-        for (int j0 = 0; j0 < n; j0 += tile) { ts.push_back(i); tj.push_back(j0); }
-        for (int k = 0; k * kWeightsBlock < n; ++k) { wbs.push_back(i); wbb.push_back(k); }
-    }
-    first_tile[b->S] = (int)ts.size();
-    first_wb[b->S] = (int)wbs.size();
-    std::vector<int> nt((size_t)b->Tmax + 1), nw((size_t)b->Tmax + 1);
-    for (int t = 0; t <= b->Tmax; ++t) {
-        int act = 0;                       // sequences are sorted by length, longest first
-        while (act < b->S && b->T[b->order[act]] > t) ++act;
-        nt[t] = first_tile[act];
-        nw[t] = first_wb[act];
-    }
+    BatchLayout L;
+    int st = make_layout(b->S, b->N.data(), b->T.data(), L);
+    if (st) return st;
+    make_tiles(L, tile);
+    st = check_layout(L, tile);
+    if (st) return st;
+    const std::vector<int> &ts = L.ts, &tj = L.tj, &wbs = L.wbs, &wbb = L.wbb, &nt = L.nt, &nw = L.nw;
     b->all_tiles = (int)ts.size();
     b->h_ntiles = nt;
     (void)hipFree(b->tile_seq); (void)hipFree(b->tile_j0); (void)hipFree(b->n_tiles);
@@ -600,41 +709,24 @@ int pf_batch_create(pf_batch **out, int32_t S, const int32_t *n_particles, const
     b->S = S; b->K = K; b->D = d; b->m = m; b->J = m / 2;
     b->cap = c.cand_capacity > 0 ? c.cand_capacity : 16;
     b->pd = c.p_detect; b->gate = c.gate_chi2; b->lambda = c.clutter_density;
+    BatchLayout L;
+    int lst = make_layout(S, n_particles, n_frames, L);
+    if (lst) { delete b; return lst; }
     b->N.assign(n_particles, n_particles + S);
     b->T.assign(n_frames, n_frames + S);
-    for (int s = 0; s < S; ++s)
-        if (b->N[s] <= 0 || b->N[s] > kMaxScanBlocks * kWeightsBlock || b->T[s] <= 0) {
-            delete b;
-            return set_error(PF_ERR_ARG, "sequence %d: bad size (N=%d, T=%d)", s, n_particles[s], n_frames[s]);
-        }
     int cus = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device);
     b->cus = cus > 0 ? cus : 256;
-    b->order.resize(S);
-    std::iota(b->order.begin(), b->order.end(), 0);
-    std::stable_sort(b->order.begin(), b->order.end(), [&](int p, int q) { return b->T[p] > b->T[q]; });
+    b->order = L.order;
     b->rank.resize(S);
     for (int i = 0; i < S; ++i) b->rank[b->order[i]] = i;
     b->tofs_user.resize(S);
     for (int s = 0, acc = 0; s < S; ++s) { b->tofs_user[s] = acc; acc += b->T[s]; }
-    b->h_seqs.resize(S);
-    long long pofs = 0, bofs = 0;
-    for (int i = 0; i < S; ++i) {
-        const int s = b->order[i];
-        SeqDesc &sd = b->h_seqs[i];
-        sd.pofs = (int)pofs;
-        sd.N = b->N[s];
-        sd.T = b->T[s];
-        sd.tofs = b->tofs_user[s];
-        sd.nb = (sd.N + kWeightsBlock - 1) / kWeightsBlock;
-        sd.bofs = (int)bofs;
-        pofs += (sd.N + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
-        bofs += sd.nb;
-        b->nb_max = std::max(b->nb_max, sd.nb);
-        b->Tmax = std::max(b->Tmax, sd.T);
-    }
-    if (pofs > (1LL << 28)) { delete b; return set_error(PF_ERR_ARG, "batch too large (%lld particles)", pofs); }
-    b->ld = (int)pofs;
+    b->h_seqs = L.seqs;
+    b->nb_max = L.nb_max;
+    b->Tmax = L.Tmax;
+    const long long bofs = L.nblocks;
+    b->ld = L.ld;
     b->hm.resize(S);
     b->loaded.assign(S, 0);
     b->sel_ok.assign(S, 0);
@@ -672,6 +764,29 @@ int pf_batch_create(pf_batch **out, int32_t S, const int32_t *n_particles, const
     int st = build_tiles(b);
     if (st) { pf_batch_destroy(b); return st; }
     *out = b;
+    return PF_OK;
+}
+
+int pf_batch_plan(int32_t S, const int32_t *n_particles, const int32_t *n_frames, int32_t block, int32_t ppl,
+                  int64_t *summary)
+{
+    if (!n_particles || !n_frames) return set_error(PF_ERR_ARG, "null argument");
+    if ((block != 64 && block != 128 && block != 256) || (ppl != 1 && ppl != 2))
+        return set_error(PF_ERR_ARG, "bad schedule");
+    BatchLayout L;
+    int st = make_layout(S, n_particles, n_frames, L);
+    if (st) return st;
+    make_tiles(L, block * ppl);
+    st = check_layout(L, block * ppl);
+    if (st) return st;
+    if (summary) {
+        summary[0] = L.ld;
+        summary[1] = L.nblocks;
+        summary[2] = (int64_t)L.ts.size();
+        summary[3] = (int64_t)L.wbs.size();
+        summary[4] = L.nb_max;
+        summary[5] = L.Tmax;
+    }
     return PF_OK;
 }
 

@@ -59,6 +59,7 @@ def lib():
         L.orc_adj_n_theta.argtypes = [C.c_void_p]
         L.orc_adj_value.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_adj_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_adj_dirderiv.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_philox_u32.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_int64, _up]
         L.orc_det_exp.argtypes = [C.c_double]
         L.orc_det_exp.restype = C.c_double
@@ -375,6 +376,17 @@ class AdjProblem:
         rc = lib().orc_adj_grad(C.byref(self.s), theta.ctypes.data, C.byref(F), g.ctypes.data)
         assert rc == 0
         return F.value, g
+
+    def dirderiv(self, theta, V):
+        """grad F(theta) . V[c] for the rows of V (at most 8), one forward-mode pass."""
+        theta = _f64(theta)
+        V = np.ascontiguousarray(_f64(V).reshape(-1, self.n_theta))
+        F = C.c_double(0.0)
+        dd = np.zeros(V.shape[0])
+        rc = lib().orc_adj_dirderiv(C.byref(self.s), theta.ctypes.data, V.shape[0], V.ctypes.data, C.byref(F),
+                                    dd.ctypes.data)
+        assert rc == 0
+        return F.value, dd
 
 
 # ------------------------------------------------- pose tracker stage

@@ -205,6 +205,8 @@ T replay(const orc_adj_problem *p, const Params<T> &q, const double *logpi0, dou
     T F(0.0);
     std::vector<T> lw(N);
     for (int t = 0; t < p->T; ++t) {
+        // particles are independent within a frame (the sums below are sequential)
+#pragma omp parallel for schedule(static)
         for (int j = 0; j < N; ++j) {
             const int k = p->comp[(size_t)t * N + j];
             std::vector<T> xj = x[j], Pj = P[j];
@@ -260,6 +262,25 @@ int orc_adj_value(const orc_adj_problem *p, const double *theta, const double *l
     if (check(p)) return -1;
     Params<double> q = derive<double>(p, theta);
     *F = replay<double>(p, q, logpi0, lml);
+    return 0;
+}
+
+int orc_adj_dirderiv(const orc_adj_problem *p, const double *theta, int nd, const double *V, double *F, double *dd)
+{
+    constexpr int NT = 8;
+    if (check(p) || nd <= 0 || nd > NT) return -1;
+    const int n = n_theta(p);
+    Params<double> q0 = derive<double>(p, theta);
+    std::vector<double> logpi0(q0.logpi);
+    std::vector<Dual<NT>> th(n);
+    for (int i = 0; i < n; ++i) {
+        th[i] = Dual<NT>(theta[i]);
+        for (int c = 0; c < nd; ++c) th[i].t[c] = V[(size_t)c * n + i];
+    }
+    Params<Dual<NT>> q = derive<Dual<NT>>(p, th.data());
+    const Dual<NT> f = replay<Dual<NT>>(p, q, logpi0.data(), nullptr);
+    for (int c = 0; c < nd; ++c) dd[c] = f.t[c];
+    *F = f.v;
     return 0;
 }
 

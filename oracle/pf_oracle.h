@@ -207,6 +207,9 @@ int orc_adj_value(const orc_adj_problem *p, const double *theta, const double *l
 /* F and dF/dtheta at theta (pi0 = softmax(logits of theta)), by forward-mode
  * dual numbers, 8 directions per pass. */
 int orc_adj_grad(const orc_adj_problem *p, const double *theta, double *F, double *grad);
+/* Directional derivatives of F at theta along nd <= 8 directions (rows of V,
+ * nd x n_theta), in one forward-mode pass: dd[c] = grad F . V[c]. */
+int orc_adj_dirderiv(const orc_adj_problem *p, const double *theta, int nd, const double *V, double *F, double *dd);
 
 /* ------------------------------------------------- pose tracker stage */
 /* (pf_pose_ref.c) IEEE-exact-operation log, sin/cos of 2 pi w 2^-32, and the

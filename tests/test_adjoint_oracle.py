@@ -92,3 +92,14 @@ def test_logit_gradient_closed_form(run):
         for k in range(K):
             expect[k] += wb[comp[t] == k].sum() - pi[k] * wb.sum()
     assert np.allclose(g[K * d:K * d + K], expect, rtol=1e-9, atol=1e-10)
+
+
+def test_directional_derivatives_are_gradient_projections(run):
+    """orc_adj_dirderiv (one forward pass over up to 8 directions) equals the
+    full forward-mode gradient projected on those directions."""
+    mdl, prob, theta, lml, lw, comp = run
+    F, g = prob.grad(theta)
+    V = np.random.default_rng(11).standard_normal((8, prob.n_theta))
+    F2, dd = prob.dirderiv(theta, V)
+    assert F2 == F
+    assert np.allclose(dd, V @ g, rtol=1e-10, atol=1e-10 * np.abs(V @ g).max())
