@@ -72,7 +72,8 @@ __device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc
 // which keeps the 24-dimensional state's register footprint free of the
 // loop's live values.
 template <int D, int SELJ, int PPL, bool PERSIST = true>
-__global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((D > 16 || PPL > 1) ? 1 : 2, 8)))
+k_batch_step(BatchArgs a)
 {
     constexpr int NP = npacked(D);
     extern __shared__ float4 sm4[];
@@ -154,19 +155,6 @@ __global__ void __launch_bounds__(256) k_batch_step(BatchArgs a)
                     }
                 };
                 store();
-                if (f & kNeedFp64) {
-                    // redo this particle with the ill-conditioned joints in FP64
-                    f = 0;
-                    const unsigned voff = (unsigned)(sd.pofs + src[r]) * 4u;
-#pragma unroll
-                    for (int i = 0; i < D; ++i) x[r][i] = soa_ld(rx, voff + rowb * i, 0);
-#pragma unroll
-                    for (int p = 0; p < NP; ++p) P[r][p] = soa_ld(rP, voff + rowb * p, 0);
-                    lwj = particle_update<D, kModeCand, SELJ, true>(a.base, t, j, U1,
-                                                                    reinterpret_cast<const float *>(sm4), x[r], P[r],
-                                                                    sd.pi_total, comp_global, comp_out, f);
-                    store();
-                }
                 pflags |= f;
                 a.lw[sd.pofs + j] = lwj;
             }

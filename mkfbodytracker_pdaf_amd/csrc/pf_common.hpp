@@ -149,9 +149,6 @@ __host__ __device__ inline bool innovation_ill(float s00, float s01, float s11)
 constexpr unsigned kStatusDegenerate = 1u;
 constexpr unsigned kStatusIllCond = 2u;
 constexpr unsigned kStatusNotSpd = 4u;
-// Per-particle flag of a step's FP32 pass: a joint needs the FP64 factor
-// (the particle is redone; never reported).
-constexpr unsigned kNeedFp64 = 0x100u;
 // LDS budget of the staged image: above it only the head is staged and the
 // component rows are read from global memory.
 constexpr int kImageLdsBytes = 48 * 1024;

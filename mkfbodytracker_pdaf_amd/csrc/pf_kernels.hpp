@@ -921,13 +921,10 @@ __host__ __device__ constexpr int step_lds_bytes(int stage_floats, int nb_pendin
 // per-sequence values pi_total (component thresholds' total), comp_global
 // (component rows outside LDS, or null) and comp_out (optional component
 // record) are passed explicitly.
-// F64 = false (the step's own pass): every joint is factored in FP32; a
-// joint whose S is ill-conditioned (innovation_ill) sets kNeedFp64 in flags
-// and the particle's results are then not used: the step redoes that particle
-// from its ancestor's state with F64 = true, where such joints are factored
-// in FP64 (joint_factor64).  Keeping the FP64 code out of the main pass keeps
-// its register allocation unchanged.
-template <int D, int MODE, int SELJ, bool F64 = false>
+// A joint whose innovation covariance FP32 cannot factor reliably
+// (innovation_ill) is factored in FP64 from the particle's current state
+// (joint_factor64, a per-lane branch taken only for those joints).
+template <int D, int MODE, int SELJ>
 __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j, uint32_t U1, const float *sm,
                                                  float (&x)[D], float (&P)[npacked(D)], unsigned long long pi_total,
                                                  const float *comp_global, int *comp_out, unsigned &flags)
@@ -963,8 +960,7 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
                                                   Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0, y1, il00, l10, il11,
                                                   logl);
             float v0, v1, m00, m01, m11;
-            if (!F64 || ok) {
-                if (!ok) flags |= kNeedFp64;
+            if (ok) {
                 acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
                                            v1, m00, m01, m11);
             } else {
@@ -988,8 +984,7 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
                                               BHs[2 * b + 1], Rs[3 * b], Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0,
                                               y1, il00, l10, il11, logl);
             float v0, v1, m00, m01, m11;
-            if (!F64 || ok) {
-                if (!ok) flags |= kNeedFp64;
+            if (ok) {
                 acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
                                            v1, m00, m01, m11);
             } else {
@@ -1067,21 +1062,6 @@ __device__ __forceinline__ void step_body(const StepArgs &ain)
         for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[i]);
 #pragma unroll
         for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[p]);
-        if (pflags & kNeedFp64) {
-            // redo this particle with the ill-conditioned joints in FP64
-            pflags = 0;
-            const __amdgpu_buffer_rsrc_t rx = soa_rsrc(a.xin, rowb * D), rP = soa_rsrc(a.Pin, rowb * NP);
-#pragma unroll
-            for (int i = 0; i < D; ++i) x[i] = soa_ld(rx, voff, rowb * i);
-#pragma unroll
-            for (int p = 0; p < NP; ++p) P[p] = soa_ld(rP, voff, rowb * p);
-            lwj = particle_update<D, MODE, SELJ, true>(a, t, j, U1, reinterpret_cast<const float *>(sm4), x, P,
-                                                       a.pi_total, a.comp_global, a.comp_out, pflags);
-#pragma unroll
-            for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[i]);
-#pragma unroll
-            for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[p]);
-        }
         a.lw[j] = lwj;
     }
     report_flags(pflags, &a.fs->status);
@@ -1100,8 +1080,11 @@ __device__ __forceinline__ void step_body(const StepArgs &ain)
     }
 }
 
+// At least two waves per SIMD for states up to d = 16 in the selection
+// basis: the reference-mode kernels would otherwise take more than 256
+// registers (their FP64 joint branch) and run one.
 template <int D, int MODE, int SELJ>
-__global__ void __launch_bounds__(256) k_step(StepArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((D > 16 || SELJ == 0) ? 1 : 2, 8))) k_step(StepArgs a)
 {
     step_body<D, MODE, SELJ>(a);
 }
@@ -1109,7 +1092,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
 // Two filters of the same shape in one launch (grid.y = filter): the pose
 // tracker advances its two arms together.
 template <int D, int MODE, int SELJ>
-__global__ void __launch_bounds__(256) k_step_pair(StepArgs a0, StepArgs a1)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((D > 16 || SELJ == 0) ? 1 : 2, 8))) k_step_pair(StepArgs a0, StepArgs a1)
 {
     step_body<D, MODE, SELJ>(blockIdx.y ? a1 : a0);
 }
