@@ -879,18 +879,6 @@ __device__ __forceinline__ bool joint_factor64(const float (&x)[D], const float 
     return true;
 }
 
-// Likelihood and update parameters of a joint that is not positive definite
-// in FP64: zero weight (-inf) and no update.
-template <int D>
-__device__ __forceinline__ float joint_reject(float (&w0)[D], float (&w1)[D], float &v0, float &v1, float &m00,
-                                              float &m01, float &m11)
-{
-#pragma unroll
-    for (int c = 0; c < D; ++c) w0[c] = w1[c] = 0.f;
-    v0 = v1 = m00 = m01 = m11 = 0.f;
-    return -INFINITY;
-}
-
 // ORs the wave's particle flags (kStatusIllCond, kStatusNotSpd) into *status
 // with one atomic per wave, only when a flag is raised.  Every lane of the
 // wave calls it.
@@ -921,6 +909,18 @@ __host__ __device__ constexpr int step_lds_bytes(int stage_floats, int nb_pendin
 // per-sequence values pi_total (component thresholds' total), comp_global
 // (component rows outside LDS, or null) and comp_out (optional component
 // record) are passed explicitly.
+// Likelihood and update parameters of a joint that is not positive definite
+// in FP64: zero weight (-inf) and no update.
+template <int D>
+__device__ __forceinline__ float joint_reject(float (&w0)[D], float (&w1)[D], float &v0, float &v1, float &m00,
+                                              float &m01, float &m11)
+{
+#pragma unroll
+    for (int c = 0; c < D; ++c) w0[c] = w1[c] = 0.f;
+    v0 = v1 = m00 = m01 = m11 = 0.f;
+    return -INFINITY;
+}
+
 // A joint whose innovation covariance FP32 cannot factor reliably
 // (innovation_ill) is factored in FP64 from the particle's current state
 // (joint_factor64, a per-lane branch taken only for those joints).
@@ -959,18 +959,26 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
             const bool ok = Joint<D>::prepare_sel(x, P, 2 * oi, BHs[2 * b], BHs[2 * b + 1], Rs[3 * b],
                                                   Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0, y1, il00, l10, il11,
                                                   logl);
+            // States up to d = 16 apply the update inside each branch: on the
+            // FP32 path the compiler may then start it before the likelihood
+            // is complete (a certain joint's M is the identity), which keeps W
+            // short-lived; the 24-dimensional state keeps one update after
+            // the branches (fewer values to spill).
             float v0, v1, m00, m01, m11;
             if (ok) {
                 acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
                                            v1, m00, m01, m11);
+                if constexpr (D <= 16) Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
             } else {
                 Whiten64 wh;
-                if (joint_factor64<D, true>(x, P, 2 * oi, nullptr, nullptr, J64 + 5 * b, wh, logl, w0, w1, flags))
+                if (joint_factor64<D, true>(x, P, 2 * oi, nullptr, nullptr, J64 + 5 * b, wh, logl, w0, w1, flags)) {
                     acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, wh, logl, v0, v1, m00, m01, m11);
-                else
-                    acc += joint_reject<D>(w0, w1, v0, v1, m00, m01, m11);
+                    if constexpr (D <= 16) Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
+                } else {
+                    acc += joint_reject<D>(w0, w1, v0, v1, m00, m01, m11);   // zero weight, no update
+                }
             }
-            Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
+            if constexpr (D > 16) Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
         }
     } else {
         for (int oi = 0; oi < a.J; ++oi) {
@@ -987,15 +995,18 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
             if (ok) {
                 acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, y0, y1, il00, l10, il11, logl, v0,
                                            v1, m00, m01, m11);
+                if constexpr (D <= 16) Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
             } else {
                 Whiten64 wh;
                 if (joint_factor64<D, false>(x, P, 0, Hs + (2 * b) * D, Hs + (2 * b + 1) * D, J64 + 5 * b, wh, logl,
-                                             w0, w1, flags))
+                                             w0, w1, flags)) {
                     acc += joint_measure<MODE>(a, t, j, b, clutter, cnt, cz, wh, logl, v0, v1, m00, m01, m11);
-                else
-                    acc += joint_reject<D>(w0, w1, v0, v1, m00, m01, m11);
+                    if constexpr (D <= 16) Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
+                } else {
+                    acc += joint_reject<D>(w0, w1, v0, v1, m00, m01, m11);   // zero weight, no update
+                }
             }
-            Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
+            if constexpr (D > 16) Joint<D>::apply(x, P, w0, w1, v0, v1, m00, m01, m11);
         }
     }
     return acc;
