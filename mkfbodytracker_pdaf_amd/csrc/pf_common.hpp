@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 namespace mkfbt {
 
 // fma(a, b, c) for the deterministic polynomials.  On the device it is the
@@ -73,7 +75,7 @@ __host__ __device__ inline uint64_t fixed32(double v)
 // Fixed-point weight of a log-weight relative to the maximum.
 __host__ __device__ inline uint64_t qlog(double lw, double lmax)
 {
-    return (isfinite(lw) && isfinite(lmax)) ? fixed32(det_exp(lw - lmax)) : 0;
+    return (std::isfinite(lw) && std::isfinite(lmax)) ? fixed32(det_exp(lw - lmax)) : 0;
 }
 
 // Block-scaled log-weights (the resampler of log-weights).  A block of
@@ -97,7 +99,7 @@ __host__ __device__ inline double pow_ref(int e) { return (double)e * 0.69314718
 
 __host__ __device__ inline uint64_t qlog_pow(double lw, int e)
 {
-    return (isfinite(lw) && e != kPowEmpty) ? fixed32(det_exp(lw - pow_ref(e))) : 0;
+    return (std::isfinite(lw) && e != kPowEmpty) ? fixed32(det_exp(lw - pow_ref(e))) : 0;
 }
 
 // Right shift aligning a block of exponent e to the largest exponent E (64:
@@ -152,27 +154,5 @@ constexpr unsigned kStatusNotSpd = 4u;
 // LDS budget of the staged image: above it only the head is staged and the
 // component rows are read from global memory.
 constexpr int kImageLdsBytes = 48 * 1024;
-
-// Ordered 64-bit key of a double (unsigned comparison = numeric order), for
-// maxima taken with 64-bit atomicMax, and its inverse.
-__device__ __forceinline__ unsigned long long order_key(double v)
-{
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
-__device__ __forceinline__ double key_value(unsigned long long k)
-{
-    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
-    return __longlong_as_double((long long)b);
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_max(T v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
 
 }  // namespace mkfbt

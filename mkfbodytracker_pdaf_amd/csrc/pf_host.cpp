@@ -91,6 +91,10 @@ my_gmm::~my_gmm() {}
 void my_gmm::loadGaussian(Mat u, Mat s, Mat &H, Mat &m, double w, double g)
 {
     const int d = s.rows, m6 = (int)sel.size(), npose = H.cols;
+    if (s.cols != d || u.rows * u.cols != d || H.rows != d || m.rows * m.cols != npose)
+        throw PfException(PF_ERR_ARG, "loadGaussian: u 1 x d, s d x d, H d x npose, m 1 x npose");
+    for (int r = 0; r < m6; ++r)
+        if (sel[r] < 0 || sel[r] >= npose) throw PfException(PF_ERR_ARG, "loadGaussian: sel outside the pose rows");
     mean.push_back(u);
     cov.push_back(s);
     weight.push_back(w);

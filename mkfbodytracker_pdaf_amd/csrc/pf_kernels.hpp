@@ -29,6 +29,28 @@
 
 namespace mkfbt {
 
+// Ordered 64-bit key of a double (unsigned comparison = numeric order), for
+// maxima taken with 64-bit atomicMax, and its inverse.
+__device__ __forceinline__ unsigned long long order_key(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double key_value(unsigned long long k)
+{
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)b);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 constexpr int kMaxJoints = 8;
 constexpr int kWeightsBlock = 256;     // input block of the resampler scans
 constexpr int kStageBlocks = 12;       // input blocks the ancestor search stages in LDS
