@@ -10,8 +10,10 @@
 // one captured frame pair replayable for the whole run.
 //
 // Per frame: k_batch_step (ancestors from the previous frame's weights, the
-// same per-particle update as k_step, per-sequence block maxima) and
-// k_batch_weights (per-sequence fixed-point weights and block scans).
+// same per-particle update as k_step) and the frame's per-sequence
+// fixed-point weights and block scans: written by the step itself when its
+// tiles are resampler blocks (256 threads, one particle each), else by
+// k_batch_weights.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,24 +33,28 @@ using namespace mkfbt::batch;
 
 namespace {
 
+// Ancestors of a tile's outputs from the resampling of frame tw's weights
+// (set tw & 1, uniform uni[tw][1]); the sequence's first tile also writes
+// the frame's log mean weight and the degenerate flag.
 template <int PPL>
-__device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc &sd, int s, int t, int jt0,
+__device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc &sd, int s, int tw, int jt0,
                                                unsigned long long *wsp, int (&src)[PPL], int *anc_out,
-                                               int *anc_out2, int stats_parity, double *lml_slot)
+                                               int *anc_out2, double *lml_slot)
 {
     const int bd = blockDim.x;
+    const WeightSet ws = weight_set(a, tw);
     Pending pd;
-    pd.cloc = a.cloc + sd.pofs;
-    pd.bsum = a.bsum + sd.bofs;
-    pd.bexp = a.bexp + sd.bofs;
-    pd.bpow = a.bpow + sd.bofs;
+    pd.cloc = ws.cloc + sd.pofs;
+    pd.bsum = ws.bsum + sd.bofs;
+    pd.bexp = ws.bexp + sd.bofs;
+    pd.bpow = ws.bpow + sd.bofs;
     pd.L = sd.N;
     pd.nb = sd.nb;
     double es = 0.0;
     const bool first = (jt0 == 0);
     int E;
     const unsigned long long total = resolve_prefix(pd, wsp, first, es, E);
-    const uint32_t U = a.Upend[s];
+    const uint32_t U = a.uni[(size_t)(sd.tofs + tw) * 2 + 1];
 #pragma unroll
     for (int r = 0; r < PPL; ++r) {
         src[r] = 0;
@@ -64,7 +70,6 @@ __device__ __forceinline__ void tile_ancestors(const BatchArgs &a, const SeqDesc
         if (total == 0ull) atomicOr(&a.status[s], kStatusDegenerate);
         if (lml_slot) *lml_slot = (total == 0ull) ? -INFINITY : pow_ref(E) + log(es) - log((double)sd.N);
     }
-    (void)stats_parity;
 }
 
 // PERSIST: a grid of blocks_per_cu x CUs walks the active tiles; otherwise
@@ -85,14 +90,16 @@ k_batch_step(BatchArgs a)
     float *xout = odd ? a.buf_x[0] : a.buf_x[1], *Pout = odd ? a.buf_P[0] : a.buf_P[1];
     const unsigned rowb = (unsigned)a.ld * 4u;
     unsigned long long *wsp = reinterpret_cast<unsigned long long *>(reinterpret_cast<float *>(sm4) + a.stage_floats);
+    if (a.wfused && blockIdx.x == 0 && threadIdx.x == 0) *a.frame_next = t + 1;
     for (int tile = blockIdx.x; tile < ntiles; tile += PERSIST ? gridDim.x : ntiles) {
         const int s = uni_int(a.tile_seq[tile]);
         const int jt0 = uni_int(a.tile_j0[tile]);
         const SeqDesc sd = uniform_seq(a.seqs, s);
         int src[PPL];
         if (a.fused) {
-            tile_ancestors<PPL>(a, sd, s, t, jt0, wsp, src, a.rec_anc ? a.rec_anc + (size_t)(t - 1) * a.ld : nullptr,
-                                nullptr, a.parity ^ 1, a.lml + sd.tofs + (t - 1));
+            tile_ancestors<PPL>(a, sd, s, t - 1, jt0, wsp, src,
+                                a.rec_anc ? a.rec_anc + (size_t)(t - 1) * a.ld : nullptr, nullptr,
+                                a.lml + sd.tofs + (t - 1));
         } else {
 #pragma unroll
             for (int r = 0; r < PPL; ++r) {
@@ -133,6 +140,7 @@ k_batch_step(BatchArgs a)
         const uint32_t U1 = a.uni[(size_t)(sd.tofs + t) * 2];
         const __amdgpu_buffer_rsrc_t wx = soa_rsrc(xout, rowb * D), wP = soa_rsrc(Pout, rowb * NP);
         unsigned pflags = 0;
+        float lw_last = -INFINITY;
 #pragma unroll
         for (int r = 0; r < PPL; ++r) {
             const int j = jt0 + r * bd + (int)threadIdx.x;
@@ -157,9 +165,21 @@ k_batch_step(BatchArgs a)
                 store();
                 pflags |= f;
                 a.lw[sd.pofs + j] = lwj;
+                lw_last = lwj;
             }
         }
         report_flags(pflags, &a.status[s]);
+        if constexpr (PPL == 1) {
+            if (a.wfused) {
+                // the tile is resampler block jt0 / 256 of its sequence
+                const int j = jt0 + (int)threadIdx.x;
+                const bool valid = j < sd.N;
+                const WeightSet ws = weight_set(a, t);
+                const int wb = jt0 / kWeightsBlock;
+                (void)block_weights_log((double)lw_last, valid, ws.cloc + sd.pofs + (valid ? j : 0),
+                                        ws.bsum + sd.bofs + wb, ws.bexp + sd.bofs + wb, ws.bpow + sd.bofs + wb);
+            }
+        }
     }
 }
 
@@ -176,9 +196,9 @@ __global__ void __launch_bounds__(kWeightsBlock) k_batch_weights(BatchArgs a)
         const int i = b * kWeightsBlock + (int)threadIdx.x;
         const bool valid = i < sd.N;
         const double x = valid ? (double)a.lw[sd.pofs + i] : -INFINITY;
-        block_weights_log(x, valid, a.cloc + sd.pofs + (valid ? i : 0), a.bsum + sd.bofs + b, a.bexp + sd.bofs + b,
-                          a.bpow + sd.bofs + b);
-        if (b == 0 && threadIdx.x == 0) a.Upend[s] = a.uni[(size_t)(sd.tofs + t) * 2 + 1];
+        const WeightSet ws = weight_set(a, t);
+        block_weights_log(x, valid, ws.cloc + sd.pofs + (valid ? i : 0), ws.bsum + sd.bofs + b, ws.bexp + sd.bofs + b,
+                          ws.bpow + sd.bofs + b);
     }
 }
 
@@ -195,8 +215,7 @@ __global__ void __launch_bounds__(256) k_batch_finish(BatchArgs a)
         const SeqDesc sd = uniform_seq(a.seqs, s);
         int src[PPL];
         tile_ancestors<PPL>(a, sd, s, sd.T - 1, jt0, wsp, src, a.anc,
-                            a.rec_anc ? a.rec_anc + (size_t)(sd.T - 1) * a.ld : nullptr, (sd.T - 1) & 1,
-                            a.lml + sd.tofs + sd.T - 1);
+                            a.rec_anc ? a.rec_anc + (size_t)(sd.T - 1) * a.ld : nullptr, a.lml + sd.tofs + sd.T - 1);
     }
 }
 
@@ -436,11 +455,15 @@ BatchArgs batch_args(pf_batch *b, const int32_t *count, const float *cand, const
     a.lw = b->lw;
     a.maxkey = b->maxkey;
     a.status = b->status;
-    a.Upend = b->Upend;
     a.cloc = b->cloc;
     a.bsum = b->bsum;
     a.bexp = b->bexp;
     a.bpow = b->bpow;
+    a.cloc1 = b->cloc1;
+    a.bsum1 = b->bsum1;
+    a.bexp1 = b->bexp1;
+    a.bpow1 = b->bpow1;
+    a.wfused = step_writes_weights(b) ? 1 : 0;
     a.tables = b->tables;
     a.seqs = b->seqs;
     a.tile_seq = b->tile_seq;
@@ -599,7 +622,7 @@ int run_all(pf_batch *b, const int32_t *count, const float *cand, const uint32_t
         }
         int st = launch_batch_step(b, a);
         if (st) return st;
-        return launch_batch_weights(b, a);
+        return a.wfused ? PF_OK : launch_batch_weights(b, a);
     };
     int st = one(0);
     if (st) return st;
@@ -659,7 +682,7 @@ int pf_batch_destroy(pf_batch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     void *ptrs[] = {b->bx[0], b->bx[1], b->bP[0], b->bP[1], b->lw, b->tables, b->mus, b->sigs, b->Tis, b->Ts,
                     b->bexp, b->bpow, b->partial, b->est, b->gbuf, b->anc, b->frame, b->maxkey, b->cloc, b->bsum,
-                    b->status, b->Upend, b->seqs, b->tile_seq, b->tile_j0, b->n_tiles, b->wb_seq, b->wb_b,
+                    b->cloc1, b->bsum1, b->bexp1, b->bpow1, b->status, b->seqs, b->tile_seq, b->tile_j0, b->n_tiles, b->wb_seq, b->wb_b,
                     b->n_wblocks, b->fs, b->rec_comp, b->rec_anc, b->rec_bins, b->ck_x, b->ck_P, b->seg_x[0],
                     b->seg_x[1], b->seg_P[0], b->seg_P[1], b->adj_x[0], b->adj_x[1], b->adj_P[0], b->adj_P[1], b->first, b->tacc, b->gsum,
                     b->t256_seq, b->t256_j0, b->t256_first};
@@ -731,9 +754,12 @@ int pf_batch_create(pf_batch **out, int32_t S, const int32_t *n_particles, const
     e = e ? e : dallocb(&b->bsum, (size_t)bofs);
     e = e ? e : dallocb(&b->bexp, (size_t)bofs);
     e = e ? e : dallocb(&b->bpow, (size_t)bofs);
+    e = e ? e : dallocb(&b->cloc1, (size_t)b->ld);
+    e = e ? e : dallocb(&b->bsum1, (size_t)bofs);
+    e = e ? e : dallocb(&b->bexp1, (size_t)bofs);
+    e = e ? e : dallocb(&b->bpow1, (size_t)bofs);
     e = e ? e : dallocb(&b->maxkey, 2 * (size_t)S);
     e = e ? e : dallocb(&b->status, (size_t)S);
-    e = e ? e : dallocb(&b->Upend, (size_t)S);
     e = e ? e : dallocb(&b->seqs, (size_t)S);
     e = e ? e : dallocb(&b->frame, 2);
     e = e ? e : dallocb(&b->rec_bins, (size_t)b->ld);

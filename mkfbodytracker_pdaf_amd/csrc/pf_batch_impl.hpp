@@ -36,10 +36,12 @@ struct BatchArgs {
     float *lw;
     unsigned long long *maxkey;   // 2 x S (parity-major)
     unsigned int *status;         // S
-    uint32_t *Upend;              // S
-    unsigned long long *cloc, *bsum;
-    double *bexp;
-    int *bpow;
+    // resampler weights of frame t in set t & 1 (frame t + 1 resolves them
+    // while the weights of its own frame go to the other set): block-local
+    // scans (ld), block sums, FP64 block sums and exponents (per block)
+    unsigned long long *cloc, *bsum, *cloc1, *bsum1;
+    double *bexp, *bexp1;
+    int *bpow, *bpow1;
     const float *tables;
     const SeqDesc *seqs;
     const int *tile_seq, *tile_j0, *n_tiles;   // tiles of active sequences first
@@ -54,6 +56,7 @@ struct BatchArgs {
     int *rec_comp;                // optional record: components of frame t at rec_comp[t * ld + column]
     int *rec_anc;                 // optional record: ancestors of frame t's resampling, same layout
     int S, ld, K, m, J, cap, table_floats, stage_floats, parity, fused, nb_max, all_tiles;
+    int wfused;                   // the step writes its frame's resampler weights (256-particle tiles, one per lane)
     unsigned int order_packed, clutter_mask;
     float pdl, b0, gate;
 };
@@ -78,6 +81,18 @@ __device__ __forceinline__ SeqDesc uniform_seq(const SeqDesc *seqs, int s)
     const unsigned hi = (unsigned)uni_int((int)(unsigned)(d.pi_total >> 32));
     r.pi_total = ((unsigned long long)hi << 32) | lo;
     return r;
+}
+
+// The resampler weights of frame t (set t & 1).
+struct WeightSet {
+    unsigned long long *cloc, *bsum;
+    double *bexp;
+    int *bpow;
+};
+__device__ __forceinline__ WeightSet weight_set(const BatchArgs &a, int t)
+{
+    const bool o = (t & 1) != 0;     // selects, not indexes
+    return {o ? a.cloc1 : a.cloc, o ? a.bsum1 : a.bsum, o ? a.bexp1 : a.bexp, o ? a.bpow1 : a.bpow};
 }
 
 __device__ __forceinline__ const float *seq_comp_global(const BatchArgs &a, const SeqDesc &sd)
@@ -138,13 +153,13 @@ struct pf_batch {
     hipStream_t stream = nullptr;
     float *bx[2] = {nullptr, nullptr}, *bP[2] = {nullptr, nullptr};
     float *lw = nullptr, *tables = nullptr, *mus = nullptr, *sigs = nullptr;
-    double *Tis = nullptr, *Ts = nullptr, *bexp = nullptr, *partial = nullptr, *est = nullptr, *gbuf = nullptr;
-    int *bpow = nullptr;
+    double *Tis = nullptr, *Ts = nullptr, *bexp = nullptr, *bexp1 = nullptr, *partial = nullptr, *est = nullptr;
+    double *gbuf = nullptr;
+    int *bpow = nullptr, *bpow1 = nullptr;
     size_t gbuf_bytes = 0;
     int *anc = nullptr, *frame = nullptr;
-    unsigned long long *maxkey = nullptr, *cloc = nullptr, *bsum = nullptr;
+    unsigned long long *maxkey = nullptr, *cloc = nullptr, *bsum = nullptr, *cloc1 = nullptr, *bsum1 = nullptr;
     unsigned int *status = nullptr;
-    uint32_t *Upend = nullptr;
     mkfbt::batch::SeqDesc *seqs = nullptr;
     std::vector<mkfbt::batch::SeqDesc> h_seqs;
     int *tile_seq = nullptr, *tile_j0 = nullptr, *n_tiles = nullptr, all_tiles = 0;
@@ -181,6 +196,13 @@ int build_tiles(pf_batch *b);
 BatchArgs batch_args(pf_batch *b, const int32_t *count, const float *cand, const uint32_t *uni, double *lml,
                      int parity, int fused);
 int launch_batch_step(pf_batch *b, const BatchArgs &a);
+// Whether the d = 12 step writes its frame's resampler weights itself: its
+// tiles are resampler blocks (256 threads, one particle per lane); else
+// k_batch_weights runs after it.
+inline bool step_writes_weights(const pf_batch *b)
+{
+    return b->D == kBD && b->sch.block == kWeightsBlock && b->sch.ppl == 1;
+}
 int launch_batch_weights(pf_batch *b, const BatchArgs &a);
 int launch_batch_finish(pf_batch *b, const BatchArgs &a);
 // Frames [0, frames) of every sequence from its reset state.  ckpt: copy the

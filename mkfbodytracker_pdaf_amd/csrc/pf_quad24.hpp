@@ -103,8 +103,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))
     const SeqDesc sd = uniform_seq(a.seqs, s);
     int src[1];
     if (a.fused) {
-        tile_ancestors<1>(a, sd, s, t, jt0, wsp, src, a.rec_anc ? a.rec_anc + (size_t)(t - 1) * a.ld : nullptr,
-                          nullptr, a.parity ^ 1, a.lml + sd.tofs + (t - 1));
+        tile_ancestors<1>(a, sd, s, t - 1, jt0, wsp, src, a.rec_anc ? a.rec_anc + (size_t)(t - 1) * a.ld : nullptr,
+                          nullptr, a.lml + sd.tofs + (t - 1));
     } else {
         const int j = jt0 + (int)threadIdx.x;
         src[0] = j < sd.N ? a.anc[sd.pofs + j] : 0;
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))
         if (live) {
             const unsigned vo = (unsigned)(sd.pofs + j) * 4u;
 #pragma unroll
-            for (int K = 0; K < 6; ++K) soa_st(wx, vo + crow, rowb * (unsigned)(4 * K), x[K]);
+            for (int K = 0; K < 6; ++K) soa_st_partial(wx, vo + crow, rowb * (unsigned)(4 * K), x[K]);
 #pragma unroll
             for (int I = 0; I < 6; ++I)
 #pragma unroll
@@ -353,9 +353,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         if (K > I)
-                            soa_st(wP, vo + crow, rowb * (unsigned)pk(24, 4 * I + r, 4 * K), Pt[qtile(I, K)][r]);
+                            soa_st_partial(wP, vo + crow, rowb * (unsigned)pk(24, 4 * I + r, 4 * K), Pt[qtile(I, K)][r]);
                         else if (r <= c)
-                            soa_st(wP, vo + (unsigned)drow[I][r] * rowb, 0, Pt[qtile(I, K)][r]);
+                            soa_st_partial(wP, vo + (unsigned)drow[I][r] * rowb, 0, Pt[qtile(I, K)][r]);
                     }
             if (c == 0) {
                 a.lw[sd.pofs + j] = acc;
