@@ -1064,6 +1064,8 @@ int pf_get_sample_prob(pf_handle *h, const double *Hp, const double *M, int32_t 
 }
 
 // ------------------------------------------------ device-resident sequences
+}  // extern "C"
+
 namespace {
 
 // The step of a device-resident run for frames of parity p (the frame
@@ -1164,6 +1166,8 @@ int run_frames_split(pf_handle *h, int T, int mode, const float *Z, const int *c
 }
 
 }  // namespace
+
+extern "C" {
 
 static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *count,
                       const float *cand, const uint32_t *uniforms, double *lml, int flags)

@@ -56,13 +56,17 @@ class BatchProblem:
 class LmlFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means, logits, rates, chol, problem: BatchProblem):
-        f, g = problem.evaluate(*(t.detach().cpu().double().numpy() for t in (means, logits, rates, chol)))
-        ctx.grads = tuple(torch.from_numpy(g[k]).to(dtype=means.dtype) for k in ("means", "logits", "rates", "chol"))
-        return torch.tensor(f, dtype=torch.float64)
+        inputs = (means, logits, rates, chol)
+        f, g = problem.evaluate(*(t.detach().cpu().double().numpy() for t in inputs))
+        # each gradient on its input's device and in its dtype; the value on
+        # the device of the means
+        ctx.grads = tuple(torch.from_numpy(np.ascontiguousarray(g[k])).to(device=t.device, dtype=t.dtype)
+                          for k, t in zip(("means", "logits", "rates", "chol"), inputs))
+        return torch.tensor(f, dtype=torch.float64, device=means.device)
 
     @staticmethod
     def backward(ctx, gout):
-        return tuple(gout.to(g.dtype) * g for g in ctx.grads) + (None,)
+        return tuple(gout.to(device=g.device, dtype=g.dtype) * g for g in ctx.grads) + (None,)
 
 
 def lml(means, logits, rates, chol, problem: BatchProblem):

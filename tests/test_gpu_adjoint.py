@@ -146,6 +146,17 @@ def test_torch_function_backward(gpu, run):
     for p, k in zip(params, ("means", "logits", "rates", "chol")):
         assert _rel(p.grad.numpy(), gd[k].sum(axis=0)) <= 1e-6, k
     b.close()
+    # inputs on a torch GPU device in FP32: the value and every gradient come
+    # back on that device in the input's dtype (the gradient of the
+    # FP32-rounded parameters)
+    dev = torch.device("cuda", 0)
+    params32 = [p.detach().to(device=dev, dtype=torch.float32).requires_grad_(True) for p in params]
+    out32 = lml_fn(*params32, prob)
+    out32.backward()
+    assert out32.device == dev
+    for p32, p, k in zip(params32, params, ("means", "logits", "rates", "chol")):
+        assert p32.grad.device == dev and p32.grad.dtype == torch.float32, k
+        assert _rel(p32.grad.double().cpu().numpy(), p.grad.numpy()) <= 1e-3, k
 
 
 def test_adjoint_workload_shape(gpu):
@@ -153,9 +164,9 @@ def test_adjoint_workload_shape(gpu):
     frames (own model per sequence, seeded as the bench draws them),
     checkpoint every 50 frames.  Size-independent properties: the recorded
     forward gives the plain run's log mean weights bit for bit, the gradient
-    does not depend on the checkpoint interval (50 vs 125; within 1e-5: the
-    float atomics of the component sums add in a varying order over 500
-    frames) and every entry is finite."""
+    does not depend on the checkpoint interval (50 vs 125, bit for bit: the
+    component sums are added in an order fixed by the tiles) and every entry
+    is finite."""
     from mkfbodytracker_pdaf_amd.batch import RbpfBatch
     from mkfbodytracker_pdaf_amd.devmem import DeviceArray
     S, N, T, seed, cap, cpj = 64, 16_384, 500, 11, 16, 8
@@ -187,7 +198,7 @@ def test_adjoint_workload_shape(gpu):
         assert np.array_equal(d_lml.get(), lml_plain), C
     for k in grads[0]:
         assert np.all(np.isfinite(grads[0][k])), k
-        assert _rel(grads[1][k], grads[0][k]) <= 1e-5, (k, _rel(grads[1][k], grads[0][k]))
+        assert np.array_equal(grads[1][k], grads[0][k]), (k, _rel(grads[1][k], grads[0][k]))
     b.close()
 
 
