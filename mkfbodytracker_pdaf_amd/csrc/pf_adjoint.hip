@@ -175,8 +175,11 @@ __device__ __forceinline__ void store_state(float *xs, float *Ps, int ld, int co
 
 // Replay of frame t (recorded ancestors, same component draw and update as
 // the forward): writes the updated states of frame t into the segment slot.
+// Three waves per SIMD: it then holds ~150 registers without spills (177
+// when the compiler may use two waves' budget), and it shares the GPU with
+// the reverse kernels on the other stream.
 template <int SELJ>
-__global__ void __launch_bounds__(kAdjTile) k_adj_replay(AdjArgs a)
+__global__ void __launch_bounds__(kAdjTile) __attribute__((amdgpu_waves_per_eu(3, 8))) k_adj_replay(AdjArgs a)
 {
     extern __shared__ float4 sm4[];
     const BatchArgs &ba = a.b;

@@ -143,14 +143,14 @@ def test_step_candidate_mode_teacher_forced(gpu, N):
 def test_headline_shape_against_oracle_and_device_run(gpu):
     """The single-sequence workload's own shape (N = 100,000 particles, K = 15,
     d = 12, 8 candidates per hand of capacity 16, clutter density 7 / (640 x
-    480)): two teacher-forced frames against the FP64 oracle (components and
-    ancestors bit-exact, states 1e-4), then the same three frames run resident
+    480)): eight teacher-forced frames against the FP64 oracle (components and
+    ancestors bit-exact, states 1e-4), then the same nine frames run resident
     on the device with graph replay, equal bit for bit to host-driven steps."""
     from mkfbodytracker_pdaf_amd.devmem import DeviceArray
-    N, cap, cpj = 100_000, 16, 8
+    N, cap, cpj, TF = 100_000, 16, 8, 8
     lam = (cpj - 1) / (640.0 * 480.0)
     mdl = synthetic.make_model(seed=2024)
-    seq = synthetic.simulate(mdl, 3, seed=2024, cand_per_joint=cpj, cap=cap)
+    seq = synthetic.simulate(mdl, TF + 1, seed=2024, cand_per_joint=cpj, cap=cap)
     om = oracle.Model(mdl.weights, mdl.gamma, mdl.means, mdl.Sigma, mdl.H(), mdl.BH(), mdl.R)
     rbs = []
     for _ in range(2):
@@ -165,11 +165,12 @@ def test_headline_shape_against_oracle_and_device_run(gpu):
     rb.set_tracks(x0, P0)
     rb2.set_tracks(x0, P0)
     lml_host = []
-    for t in range(3):
+    oracle.set_threads(8)
+    for t in range(TF + 1):
         U1, U2 = int(seq.uniforms[t, 0]), int(seq.uniforms[t, 1])
         st, out = rb.step_candidates(seq.count[t], seq.cand[t], U1, U2)
         lml_host.append(out["lml"])
-        if t < 2:
+        if t < TF:
             ref = om.update_pdaf(x0, P0, mdl.clutter, seq.count[t], seq.cand[t], U1, U2,
                                  0.95, 9.210340371976184, lam)
             assert np.array_equal(out["comp"], ref["comp"])
@@ -181,12 +182,13 @@ def test_headline_shape_against_oracle_and_device_run(gpu):
             assert _relerr(x, ref["xu"][out["anc"]]).max() <= 1e-4
             assert _relerr(P, ref["Pu"][out["anc"]]).max() <= 1e-4
             x0, P0 = x, P
+    oracle.set_threads(1)
     x1, P1 = rb.get_tracks()
     d_uni = DeviceArray(seq.uniforms.astype(np.uint32))
-    d_lml = DeviceArray.empty(3, np.float64)
+    d_lml = DeviceArray.empty(TF + 1, np.float64)
     d_cnt = DeviceArray(seq.count.astype(np.int32))
     d_cand = DeviceArray(seq.cand.astype(np.float32))
-    rb2.run_candidates_dev(3, d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, 1)
+    rb2.run_candidates_dev(TF + 1, d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, 1)
     rb2.synchronize()
     x2, P2 = rb2.get_tracks()
     assert np.array_equal(x1, x2) and np.array_equal(P1, P2)
