@@ -19,8 +19,8 @@
  * FP64: that particle got zero weight and kept its predicted state (the
  * reference's Cholesky failure, src/pf2DRao.cpp:36-52).  Host matrices are
  * row-major double, i.e. the memory of a continuous CV_64F cv::Mat.  Device
- * state is FP32 structure-of-arrays with packed symmetric covariances;
- * log-likelihood accumulation is FP64.
+ * state is FP32 with packed symmetric covariances, in rows or per-particle
+ * records (pf_set_state_layout); log-likelihood accumulation is FP64.
  *
  * Randomness is never drawn inside the library: every entry point that the
  * reference implements with cv::RNG / randn (src/pf2DRao.cpp:96,179-197)
@@ -181,6 +181,22 @@ int pf_get_stream(pf_handle *h, void **stream);
 int pf_last_run_kernel_time(pf_handle *h, double *step_ms, double *frame_ms, int32_t *launches);
 /* Frames of the last pf_run_*_dev call that were replayed from graphs (0 or T). */
 int pf_last_run_graph_frames(pf_handle *h, int32_t *frames);
+
+/* Device layout of the filter's state; no counterpart in the reference (a
+ * cv::Mat per track).  PF_LAYOUT_ROWS: each state component is a row over
+ * the particles; PF_LAYOUT_RECORDS: one record per particle (x, then the
+ * packed covariance; d <= 12); PF_LAYOUT_AUTO (the default): records when the
+ * step's 256-particle blocks fit at two waves per SIMD at once (N <= 512 x
+ * compute units), where a frame is bound by latency and by the gather of
+ * thinly spread ancestors, rows above (and for the arms of a pose tracker,
+ * whose paired launches have twice the blocks).  Every layout gives the same
+ * results bit for bit; changing it converts the current tracks.
+ * pf_get_state_layout returns the layout in use (ROWS or RECORDS). */
+#define PF_LAYOUT_AUTO 0
+#define PF_LAYOUT_ROWS 1
+#define PF_LAYOUT_RECORDS 2
+int pf_set_state_layout(pf_handle *h, int32_t layout);
+int pf_get_state_layout(pf_handle *h, int32_t *layout);
 
 /* ---------------------------------------------------------- KF_model */
 /* KF_model::predict (KF_model.cpp:11-15) for n independent states sharing

@@ -22,6 +22,7 @@ PF_ERR_STATE = -4
 PF_ERR_UNSUPPORTED = -5
 PF_ERR_IO = -6
 PF_BATCH_VALU, PF_BATCH_MFMA, PF_BATCH_QUAD_VALU = 0, 1, 2   # include/pf_batch.h
+PF_LAYOUT_AUTO, PF_LAYOUT_ROWS, PF_LAYOUT_RECORDS = 0, 1, 2   # include/pf_rbpf.h
 PF_NOALLOC_MAX_WEIGHTS = 524288   # include/pf_rbpf.h
 
 # Every symbol declared in include/pf_rbpf.h, pf2d.h, pf_batch.h and pf_tracker.h.
@@ -30,7 +31,8 @@ EXPORTS = [
     "pf_set_joint_clutter", "pf_reset", "pf_set_tracks", "pf_get_tracks", "pf_step",
     "pf_step_candidates", "pf_estimate", "pf_resample", "pf_resample_log", "pf_get_samples",
     "pf_resample_log_noalloc", "pf_get_sample_prob", "pf_run_ref_dev", "pf_run_candidates_dev", "pf_synchronize",
-    "pf_get_stream", "pf_last_run_kernel_time", "pf_last_run_graph_frames", "pf_kf_predict", "pf_kf_update",
+    "pf_get_stream", "pf_last_run_kernel_time", "pf_last_run_graph_frames", "pf_set_state_layout",
+    "pf_get_state_layout", "pf_kf_predict", "pf_kf_update",
     "pf_yml_read_matrix", "pf_read_camera_info", "pf_load_model_yml", "pf_philox_u32",
     "pf2d_create", "pf2d_destroy", "pf2d_load_gmm", "pf2d_load_gmm_precision", "pf2d_init", "pf2d_set_particles",
     "pf2d_get_particles", "pf2d_update", "pf2d_predict", "pf2d_estimate",
@@ -120,6 +122,8 @@ def lib():
     _sig(L, "pf_get_stream", C.c_int, _vp, C.POINTER(_vp))
     _sig(L, "pf_last_run_kernel_time", C.c_int, _vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(i32))
     _sig(L, "pf_last_run_graph_frames", C.c_int, _vp, C.POINTER(i32))
+    _sig(L, "pf_set_state_layout", C.c_int, _vp, i32)
+    _sig(L, "pf_get_state_layout", C.c_int, _vp, C.POINTER(i32))
     _sig(L, "pf_kf_predict", C.c_int, _vp, i32, i32, _vp, _vp, _vp, _vp, _vp)
     _sig(L, "pf_kf_update", C.c_int, _vp, i32, i32, i32, _vp, _vp, _vp, _vp, _vp, _vp)
     _sig(L, "pf_yml_read_matrix", C.c_int, C.c_char_p, C.c_char_p, C.POINTER(i32), C.POINTER(i32), _vp, i64)

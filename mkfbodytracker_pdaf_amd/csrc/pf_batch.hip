@@ -850,7 +850,7 @@ int pf_batch_reset(pf_batch *b, int32_t s, const int32_t *bins)
     auto reset_kernel = b->D == 24 ? k_reset<24> : k_reset<kBD>;
     hipLaunchKernelGGL(reset_kernel, dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->rec_bins + sd.pofs,
                        b->mus + (size_t)i * b->K * b->D, b->sigs + (size_t)i * b->K * npacked(b->D),
-                       b->bx[0] + sd.pofs, b->bP[0] + sd.pofs, sd.N, b->ld);
+                       b->bx[0] + sd.pofs, b->bP[0] + sd.pofs, sd.N, Lay{b->ld, 1});
     hipLaunchKernelGGL(k_identity_seq, dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->anc, sd.pofs, sd.N);
     HIP_TRYB(hipGetLastError());
     HIP_TRYB(hipStreamSynchronize(b->stream));
@@ -923,11 +923,11 @@ int pf_batch_get_tracks(pf_batch *b, int32_t s, double *x, double *P)
     if (d == 24)
         hipLaunchKernelGGL((k_export_rows<24>), dim3((unsigned)(((long long)sd.N * 24 + 255) / 256)), dim3(256), 0,
                            b->stream, b->bx[buf] + sd.pofs, b->bP[buf] + sd.pofs, b->anc + sd.pofs,
-                           b->Tis + (size_t)i * d * d, b->gbuf, b->gbuf + nx, sd.N, b->ld);
+                           b->Tis + (size_t)i * d * d, b->gbuf, b->gbuf + nx, sd.N, Lay{b->ld, 1});
     else
         hipLaunchKernelGGL((k_export<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->bx[buf] + sd.pofs,
                            b->bP[buf] + sd.pofs, b->anc + sd.pofs, b->Tis + (size_t)i * d * d, b->gbuf,
-                           b->gbuf + nx, sd.N, b->ld);
+                           b->gbuf + nx, sd.N, Lay{b->ld, 1});
     HIP_TRYB(hipGetLastError());
     if (x) HIP_TRYB(hipMemcpyAsync(x, b->gbuf, sizeof(double) * nx, hipMemcpyDeviceToHost, b->stream));
     if (P) HIP_TRYB(hipMemcpyAsync(P, b->gbuf + nx, sizeof(double) * np, hipMemcpyDeviceToHost, b->stream));
@@ -959,12 +959,12 @@ int pf_batch_set_tracks(pf_batch *b, int32_t s, const double *x, const double *P
         hipLaunchKernelGGL((k_import_rows<24>), dim3((unsigned)(((long long)sd.N * 24 + 255) / 256)), dim3(256), 0,
                            b->stream, (const double *)b->gbuf, (const double *)(b->gbuf + nx),
                            (const double *)(b->Ts + (size_t)i * d * d), b->bx[0] + sd.pofs, b->bP[0] + sd.pofs, sd.N,
-                           b->ld);
+                           Lay{b->ld, 1});
     else
         hipLaunchKernelGGL((k_import<kBD>), dim3((sd.N + 255) / 256), dim3(256), 0, b->stream,
                            (const double *)b->gbuf, (const double *)(b->gbuf + nx),
                            (const double *)(b->Ts + (size_t)i * d * d), b->bx[0] + sd.pofs, b->bP[0] + sd.pofs, sd.N,
-                           b->ld);
+                           Lay{b->ld, 1});
     hipLaunchKernelGGL(k_identity_seq, dim3((sd.N + 255) / 256), dim3(256), 0, b->stream, b->anc, sd.pofs, sd.N);
     HIP_TRYB(hipGetLastError());
     HIP_TRYB(hipStreamSynchronize(b->stream));
@@ -994,10 +994,10 @@ int pf_batch_estimate(pf_batch *b, int32_t s, double *xbar)
     const SeqDesc &sd = b->h_seqs[i];
     const int buf = sd.T & 1;
     if (b->D == 24)
-        launch_estimate<24>(b->stream, b->bx[buf] + sd.pofs, b->anc + sd.pofs, sd.N, b->ld, b->partial,
+        launch_estimate<24>(b->stream, b->bx[buf] + sd.pofs, b->anc + sd.pofs, sd.N, Lay{b->ld, 1}, b->partial,
                             b->Tis + (size_t)i * b->D * b->D, b->est);
     else
-        launch_estimate<kBD>(b->stream, b->bx[buf] + sd.pofs, b->anc + sd.pofs, sd.N, b->ld, b->partial,
+        launch_estimate<kBD>(b->stream, b->bx[buf] + sd.pofs, b->anc + sd.pofs, sd.N, Lay{b->ld, 1}, b->partial,
                              b->Tis + (size_t)i * b->D * b->D, b->est);
     HIP_TRYB(hipGetLastError());
     HIP_TRYB(hipMemcpyAsync(xbar, b->est, sizeof(double) * b->D, hipMemcpyDeviceToHost, b->stream));

@@ -60,7 +60,14 @@ struct pf_handle {
     bool model = false, tracks = false;
     unsigned int clutter_mask = 0, order_packed = 0;
     hipStream_t stream = nullptr;
-    // state (FP32 SoA, packed covariance), double buffered
+    // state (FP32, packed covariance), double buffered: buffer q is one
+    // allocation rec[q] holding x and P in layout lay (per-particle records
+    // for d <= 12, rows otherwise); xA / PA view the buffer read by the next
+    // step, xB / PB the other
+    float *rec[2] = {nullptr, nullptr};
+    Lay lay{0, 1};
+    int layout_req = PF_LAYOUT_AUTO;   // pf_set_state_layout
+    int cus = 0;                       // compute units of the device
     float *xA = nullptr, *PA = nullptr, *xB = nullptr, *PB = nullptr;
     int *anc = nullptr, *comp = nullptr, *bins = nullptr;
     float *lw = nullptr;
@@ -202,8 +209,18 @@ int check_launch(const char *what)
 template <int D, int SELJ>
 int launch_step_dj(pf_handle *h, const StepArgs &a, int mode, int block)
 {
-    const size_t lds = (size_t)step_lds_bytes(a.stage_floats, a.anc_mode == kAncFused ? a.pend.nb : 0);
+    const bool rec = h->lay.row == 1;
+    const size_t lds = (size_t)step_lds_bytes_rec(rec, a.stage_floats, a.anc_mode == kAncFused ? a.pend.nb : 0);
     const int grid = (h->N + block - 1) / block;
+    if constexpr (rec_layout(D)) {
+        if (rec) {
+            if (mode == kModeRef)
+                hipLaunchKernelGGL((k_step<D, kModeRef, SELJ, true>), dim3(grid), dim3(block), lds, h->stream, a);
+            else
+                hipLaunchKernelGGL((k_step<D, kModeCand, SELJ, true>), dim3(grid), dim3(block), lds, h->stream, a);
+            return check_launch("k_step");
+        }
+    }
     if (mode == kModeRef)
         hipLaunchKernelGGL((k_step<D, kModeRef, SELJ>), dim3(grid), dim3(block), lds, h->stream, a);
     else
@@ -272,7 +289,7 @@ struct ResetL {
     static void run(pf_handle *h)
     {
         hipLaunchKernelGGL((k_reset<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->bins,
-                           h->mu, h->sig, h->xA, h->PA, h->N, h->ld);
+                           h->mu, h->sig, h->xA, h->PA, h->N, h->lay);
     }
 };
 
@@ -282,10 +299,10 @@ struct ExportL {
     {
         if constexpr (D > 16)
             hipLaunchKernelGGL((k_export_rows<D>), dim3((unsigned)(((long long)h->N * D + 255) / 256)), dim3(256), 0,
-                               h->stream, h->xA, h->PA, h->anc, h->Tinv, x, P, h->N, h->ld);
+                               h->stream, h->xA, h->PA, h->anc, h->Tinv, x, P, h->N, h->lay);
         else
             hipLaunchKernelGGL((k_export<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, h->xA, h->PA,
-                               h->anc, h->Tinv, x, P, h->N, h->ld);
+                               h->anc, h->Tinv, x, P, h->N, h->lay);
     }
 };
 
@@ -295,10 +312,10 @@ struct ImportL {
     {
         if constexpr (D > 16)
             hipLaunchKernelGGL((k_import_rows<D>), dim3((unsigned)(((long long)h->N * D + 255) / 256)), dim3(256), 0,
-                               h->stream, x, P, h->Tdev, h->xA, h->PA, h->N, h->ld);
+                               h->stream, x, P, h->Tdev, h->xA, h->PA, h->N, h->lay);
         else
             hipLaunchKernelGGL((k_import<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, x, P, h->Tdev,
-                               h->xA, h->PA, h->N, h->ld);
+                               h->xA, h->PA, h->N, h->lay);
     }
 };
 
@@ -306,7 +323,7 @@ template <int D>
 struct EstimateL {
     static void run(pf_handle *h)
     {
-        launch_estimate<D>(h->stream, h->xA, h->anc, h->N, h->ld, h->est_partial, h->Tinv, h->est_out);
+        launch_estimate<D>(h->stream, h->xA, h->anc, h->N, h->lay, h->est_partial, h->Tinv, h->est_out);
     }
 };
 
@@ -424,6 +441,36 @@ void swap_state(pf_handle *h)
     std::swap(h->PA, h->PB);
 }
 
+// The layout pf_set_state_layout asks for, resolved for this filter.
+bool want_records(const pf_handle *h)
+{
+    if (!rec_layout(h->D) || h->layout_req == PF_LAYOUT_ROWS) return false;
+    if (h->layout_req == PF_LAYOUT_RECORDS) return true;
+    return (h->N + 255) / 256 <= 2 * h->cus;
+}
+
+// Views of the two state buffers in the given layout; buffer a (0 or 1)
+// becomes xA / PA (the state the next step reads).
+void set_views(pf_handle *h, bool records, int a)
+{
+    h->lay = records ? Lay{1, rec_floats(h->D)} : Lay{h->ld, 1};
+    for (int q = 0; q < 2; ++q) {
+        h->xbuf[q] = h->rec[q];
+        h->Pbuf[q] = h->rec[q] + (records ? (size_t)h->D : (size_t)h->D * h->ld);
+    }
+    h->xA = h->xbuf[a]; h->PA = h->Pbuf[a];
+    h->xB = h->xbuf[1 - a]; h->PB = h->Pbuf[1 - a];
+}
+
+template <int D>
+struct RelayoutL {
+    static void run(pf_handle *h, const float *xs, const float *Ps, Lay Ls)
+    {
+        hipLaunchKernelGGL((k_relayout<D>), dim3((h->N + 255) / 256), dim3(256), 0, h->stream, xs, Ps, Ls, h->xA,
+                           h->PA, h->lay, h->N);
+    }
+};
+
 // Waits for the handle's stream, then reads and clears its status word
 // (kStatus* bits raised by the device work since the last read).
 int read_status(pf_handle *h, unsigned *word)
@@ -524,7 +571,7 @@ struct AncEstimateL {
         const int nb = (h->N + 255) / 256;
         const size_t lds = (size_t)anc_estimate_lds_bytes(an.pend.nb, D);
         hipLaunchKernelGGL((k_ancestors_estimate<D>), dim3(nb), dim3(256), lds, h->stream, an, (const float *)h->xB,
-                           h->ld, h->est_partial);
+                           h->lay, h->est_partial);
         hipLaunchKernelGGL((k_estimate_final<D>), dim3(1), dim3(256), 0, h->stream, (const double *)h->est_partial, nb,
                            h->N, (const double *)h->Tinv, h->est_out);
     }
@@ -644,7 +691,7 @@ int step_ref_pair(pf_handle *h0, pf_handle *h1, const float *Z0, const float *Z1
         if (fused) e[i].an.pow_frame = h[i]->frame + p;
         e[i].x = h[i]->xB;                      // the updated state, before the swap
         e[i].partial = h[i]->est_partial;
-        e[i].ld = h[i]->ld;
+        e[i].lay = h[i]->lay;
         f[i] = estimate_final_args(h[i]);
     }
     if (a[0].stage_floats != a[1].stage_floats)
@@ -708,6 +755,9 @@ int pf_create(pf_handle **out, const pf_config *cfg)
     h->N = c.n_particles;
     h->ld = (c.n_particles + 63) & ~63;
     h->cap = c.cand_capacity;
+    if (hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess ||
+        h->cus <= 0)
+        h->cus = 256;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return set_error(PF_ERR_HIP, "hipStreamCreate failed");
@@ -745,7 +795,7 @@ int pf_destroy(pf_handle *h)
     (void)hipSetDevice(h->cfg.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
-    void *ptrs[] = {h->run_desc, h->run_ctr, h->xA, h->PA, h->xB, h->PB, h->anc, h->comp, h->bins, h->lw, h->table,
+    void *ptrs[] = {h->run_desc, h->run_ctr, h->rec[0], h->rec[1], h->anc, h->comp, h->bins, h->lw, h->table,
                     h->mu, h->sig, h->fs, h->frame, h->uni, h->lml1, h->cloc, h->bsum, h->bexp, h->bpow,
                     h->cloc1, h->bsum1, h->bexp1, h->bpow1, h->zbuf, h->cntbuf, h->candbuf, h->est_partial, h->est_out, h->gbuf, h->ibuf,
                     h->Tdev, h->Tinv};
@@ -772,18 +822,17 @@ int pf_load_model(pf_handle *h, int32_t K, int32_t d, int32_t npose, int32_t m, 
     h->D = d; h->K = K; h->m = m; h->J = J; h->npose = npose;
 
     // (re)allocate device buffers sized by the model
-    (void)hipFree(h->xA); (void)hipFree(h->PA); (void)hipFree(h->xB); (void)hipFree(h->PB);
+    (void)hipFree(h->rec[0]); (void)hipFree(h->rec[1]);
+    h->rec[0] = h->rec[1] = nullptr;
     h->xA = h->PA = h->xB = h->PB = nullptr;
     (void)hipFree(h->zbuf); (void)hipFree(h->cntbuf); (void)hipFree(h->candbuf);
     h->zbuf = nullptr; h->cntbuf = nullptr; h->candbuf = nullptr;
     h->model = false;
     h->tracks = false;
-    HIP_TRY(dalloc(&h->xA, (size_t)d * h->ld));
-    HIP_TRY(dalloc(&h->xB, (size_t)d * h->ld));
-    HIP_TRY(dalloc(&h->PA, (size_t)NP * h->ld));
-    HIP_TRY(dalloc(&h->PB, (size_t)NP * h->ld));
-    h->xbuf[0] = h->xA; h->xbuf[1] = h->xB;
-    h->Pbuf[0] = h->PA; h->Pbuf[1] = h->PB;
+    // room for either layout (a record is padded to 16 bytes)
+    const size_t per = rec_layout(d) ? (size_t)rec_floats(d) : (size_t)(d + NP);
+    for (int q = 0; q < 2; ++q) HIP_TRY(dalloc(&h->rec[q], per * h->ld));
+    set_views(h, want_records(h), 0);
     h->clutter_mask = 0;
     h->order_packed = 0;
     for (int b = 0; b < J; ++b) h->order_packed |= (unsigned)b << (4 * b);
@@ -1188,7 +1237,7 @@ static int run_frames(pf_handle *h, int T, int mode, const float *Z, const int *
     // own.  Otherwise the frame runs as separate launches, without graphs.
     const int nbp = (h->N + kWeightsBlock - 1) / kWeightsBlock;
     const int stage = h->table_floats + (mode == kModeCand ? cand_lds_floats(h->J, h->cap) : 0);
-    const bool fused = step_lds_bytes(stage, nbp) <= kMaxLdsBytes;
+    const bool fused = step_lds_bytes_rec(h->lay.row == 1, stage, nbp) <= kMaxLdsBytes;
     if (!fused) {
         st = run_frames_split(h, T, mode, Z, count, cand, uniforms, lml, timing, run_timing);
     } else {
@@ -1309,6 +1358,46 @@ int pf_last_run_graph_frames(pf_handle *h, int32_t *frames)
 {
     if (!h || !frames) return set_error(PF_ERR_ARG, "null argument");
     *frames = h->graph_frames;
+    return PF_OK;
+}
+
+int pf_set_state_layout(pf_handle *h, int32_t layout)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (layout != PF_LAYOUT_AUTO && layout != PF_LAYOUT_ROWS && layout != PF_LAYOUT_RECORDS)
+        return set_error(PF_ERR_ARG, "unknown state layout %d", layout);
+    if (layout == PF_LAYOUT_RECORDS && h->model && !rec_layout(h->D))
+        return set_error(PF_ERR_UNSUPPORTED, "per-particle records are compiled for d <= 12");
+    h->layout_req = layout;
+    if (!h->model) return PF_OK;
+    const bool rec = want_records(h);
+    if (rec == (h->lay.row == 1)) return PF_OK;
+    drop_graphs(h);                   // they bake in the step kernel of the layout
+    const int a = h->xA == h->xbuf[1] ? 1 : 0;
+    if (!h->tracks) {
+        set_views(h, rec, a);
+        return PF_OK;
+    }
+    // the current state copied into the other buffer in the new layout, which
+    // becomes the state the next step reads
+    const float *xs = h->xA, *Ps = h->PA;
+    const Lay Ls = h->lay;
+    set_views(h, rec, 1 - a);
+    st = dispatch_d<RelayoutL>(h->D, h, xs, Ps, Ls);
+    if (st) return st;
+    st = check_launch("k_relayout");
+    if (st) return st;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PF_OK;
+}
+
+int pf_get_state_layout(pf_handle *h, int32_t *layout)
+{
+    int st = check_handle(h);
+    if (st) return st;
+    if (!layout) return set_error(PF_ERR_ARG, "null argument");
+    *layout = (h->model ? h->lay.row == 1 : want_records(h)) ? PF_LAYOUT_RECORDS : PF_LAYOUT_ROWS;
     return PF_OK;
 }
 

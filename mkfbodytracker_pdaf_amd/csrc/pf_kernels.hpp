@@ -685,6 +685,125 @@ __device__ __forceinline__ void soa_st_partial(__amdgpu_buffer_rsrc_t r, unsigne
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_int(v), r, (int)voff, (int)soff, 0);
 }
 
+// Layout of a particle state buffer: element r of particle j of x (r < D)
+// or of packed P (r < NP) at base[r * row + j * part], P's base its own.
+// Rows (structure of arrays: row = ld, part = 1, P's base after the D
+// rows of x) or one record per particle (x, then packed P, padded to 16
+// bytes: row = 1, part = rec_floats(D), P's base = x's + D).  A resampled
+// particle's record is ~3 cache lines (d = 12) wherever its ancestor lies,
+// where rows cost one line per row for ancestors spread thinly; the record
+// gather and store go through LDS and hold more registers.  The batched
+// engine keeps rows; a single filter with d <= 12 may use records.
+struct Lay {
+    int row, part;
+};
+__host__ __device__ constexpr int rec_floats(int D) { return (D + npacked(D) + 3) & ~3; }
+__host__ __device__ constexpr bool rec_layout(int D) { return D <= 12; }     // records compiled for this d
+__host__ __device__ inline size_t lidx(const Lay &L, int r, int j) { return (size_t)r * L.row + (size_t)j * L.part; }
+
+// LDS of the record gather and store of the step: per wave 64 particles x 4
+// chunks of 16 bytes (+1 of padding against bank conflicts), 4 waves.
+constexpr int kRecStageBytes = 4 * 64 * 5 * 16;
+
+// Wave-cooperative gather of the records of the wave's 64 particles: lane q
+// of round r loads 16-byte chunk r + (q & 3) of particle q >> 2's ancestor
+// src (so consecutive lanes read consecutive bytes of a record), the chunks
+// pass through the wave's LDS slice gw, and each lane takes its particle's
+// chunks into x and P.  Particles j0w + p >= N read nothing (the buffer
+// resource's bound returns zeros for them).  Every lane of the wave calls it.
+template <int D>
+__device__ __forceinline__ void rec_gather(const float *rec, int N, int j0w, int src, float4 *gw, float (&x)[D],
+                                           float (&P)[npacked(D)])
+{
+    constexpr int NP = npacked(D), RS = rec_floats(D), NCH = RS / 4;
+    const int lane = threadIdx.x & 63;
+    const unsigned lim = (unsigned)N * RS * 4u;
+    const __amdgpu_buffer_rsrc_t ra =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(rec), (short)0, (int)lim, 0x00020000);
+#pragma unroll
+    for (int r0 = 0; r0 < NCH; r0 += 4) {
+        u32x4 c[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int q = i * 64 + lane, p = q >> 2, part = r0 + (q & 3);
+            const int sp = __shfl(src, p, 64);
+            const bool ok = part < NCH && j0w + p < N;
+            c[i] = __builtin_amdgcn_raw_buffer_load_b128(ra, ok ? (int)((unsigned)(sp * RS + part * 4) * 4u) : (int)lim,
+                                                         0, 0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int q = i * 64 + lane;
+            reinterpret_cast<u32x4 *>(gw)[(q >> 2) * 5 + (q & 3)] = c[i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            if (r0 + cc < NCH) {
+                const float4 v = gw[lane * 5 + cc];
+                const float ve[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int f = (r0 + cc) * 4 + e;
+                    if (f < D) x[f] = ve[e];
+                    else if (f < D + NP) P[f - D] = ve[e];
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// The inverse: the wave's 64 records (particles j0w .. j0w + 63) written
+// from x and P through gw, consecutive lanes writing consecutive bytes of a
+// record.  These stores write partial lines per instruction, so they keep
+// the default (write-back) policy.  Every lane of the wave calls it.
+template <int D>
+__device__ __forceinline__ void rec_store(float *rec, int N, int j0w, float4 *gw, const float (&x)[D],
+                                          const float (&P)[npacked(D)])
+{
+    constexpr int NP = npacked(D), RS = rec_floats(D), NCH = RS / 4;
+    const int lane = threadIdx.x & 63;
+    const unsigned lim = (unsigned)N * RS * 4u;
+    const __amdgpu_buffer_rsrc_t wa = __builtin_amdgcn_make_buffer_rsrc(rec, (short)0, (int)lim, 0x00020000);
+#pragma unroll
+    for (int r0 = 0; r0 < NCH; r0 += 4) {
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            if (r0 + cc < NCH) {
+                float ve[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int f = (r0 + cc) * 4 + e;
+                    ve[e] = f < D ? x[f] : (f < D + NP ? P[f - D] : 0.f);
+                }
+                gw[lane * 5 + cc] = make_float4(ve[0], ve[1], ve[2], ve[3]);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int q = i * 64 + lane, p = q >> 2, part = r0 + (q & 3);
+            const bool ok = part < NCH && j0w + p < N;
+            const u32x4 v = reinterpret_cast<const u32x4 *>(gw)[p * 5 + (q & 3)];
+            __builtin_amdgcn_raw_buffer_store_b128(v, wa, ok ? (int)((unsigned)((j0w + p) * RS + part * 4) * 4u)
+                                                           : (int)lim, 0, 0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
 __device__ __forceinline__ int draw_component(const unsigned long long *thr, int K, uint32_t j,
                                               uint32_t U, unsigned long long total)
 {
@@ -933,6 +1052,18 @@ __host__ __device__ constexpr int step_lds_bytes(int stage_floats, int nb_pendin
     return 4 * stage_floats + (nb_pending > 0 ? 8 * resolve_lds_words(nb_pending) : 0);
 }
 
+// The single filter's step with per-particle records: the record gather's
+// LDS follows the staged image, 16-byte aligned, sharing the resolver's
+// workspace.
+__host__ __device__ constexpr int step_lds_bytes_rec(bool rec, int stage_floats, int nb_pending)
+{
+    return !rec ? step_lds_bytes(stage_floats, nb_pending)
+                : 4 * ((stage_floats + 3) & ~3) +
+                      (nb_pending > 0 && 8 * resolve_lds_words(nb_pending) > kRecStageBytes
+                           ? 8 * resolve_lds_words(nb_pending)
+                           : kRecStageBytes);
+}
+
 // Model image layout: [component thresholds, 4K floats][H (m x D), BH (m),
 // R 2x2 blocks (3J), padded to 4][component rows, K x (2 + D + D(D+1)/2)].
 // The head is always staged in LDS; the component rows are staged with it
@@ -1049,7 +1180,8 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
     return acc;
 }
 
-template <int D, int MODE, int SELJ>
+// REC: the state buffers hold per-particle records (rows otherwise).
+template <int D, int MODE, int SELJ, bool REC>
 __device__ __forceinline__ void step_body(const StepArgs &ain)
 {
     constexpr int NP = npacked(D);
@@ -1084,7 +1216,16 @@ __device__ __forceinline__ void step_body(const StepArgs &ain)
     const unsigned rowb = (unsigned)a.ld * 4u;
     const unsigned voff = (unsigned)src * 4u;
     float x[D], P[NP];
-    if (live) {
+    // records: gathered by the wave through its LDS slice after the
+    // resolver's workspace (free once every thread has its ancestor)
+    const int j0w = j0 + (int)(threadIdx.x & ~63u);
+    float4 *gw = reinterpret_cast<float4 *>(reinterpret_cast<float *>(sm4) + ((a.stage_floats + 3) & ~3)) +
+                 (threadIdx.x >> 6) * 64 * 5;
+    static_assert(!REC || rec_layout(D), "records are compiled for d <= 12");
+    if constexpr (REC) {
+        if (a.anc_mode == kAncFused) __syncthreads();
+        rec_gather<D>(a.xin, a.N, j0w, src, gw, x, P);
+    } else if (live) {
         const __amdgpu_buffer_rsrc_t rx = soa_rsrc(a.xin, rowb * D), rP = soa_rsrc(a.Pin, rowb * NP);
 #pragma unroll
         for (int i = 0; i < D; ++i) x[i] = soa_ld(rx, voff, rowb * i);
@@ -1104,14 +1245,17 @@ __device__ __forceinline__ void step_body(const StepArgs &ain)
     if (live) {
         lwj = particle_update<D, MODE, SELJ>(a, t, j, U1, reinterpret_cast<const float *>(sm4), x, P, a.pi_total,
                                              a.comp_global, a.comp_out, pflags);
-        const unsigned doff = (unsigned)j * 4u;
-        const __amdgpu_buffer_rsrc_t wx = soa_rsrc(a.xout, rowb * D), wP = soa_rsrc(a.Pout, rowb * NP);
+        if constexpr (!REC) {
+            const unsigned doff = (unsigned)j * 4u;
+            const __amdgpu_buffer_rsrc_t wx = soa_rsrc(a.xout, rowb * D), wP = soa_rsrc(a.Pout, rowb * NP);
 #pragma unroll
-        for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[i]);
+            for (int i = 0; i < D; ++i) soa_st(wx, doff, rowb * i, x[i]);
 #pragma unroll
-        for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[p]);
+            for (int p = 0; p < NP; ++p) soa_st(wP, doff, rowb * p, P[p]);
+        }
         a.lw[j] = lwj;
     }
+    if constexpr (REC) rec_store<D>(a.xout, a.N, j0w, gw, x, P);
     report_flags(pflags, &a.fs->status);
     if (a.w_cloc) {
         // this frame's resampler weights: the block is one resampler block;
@@ -1131,18 +1275,33 @@ __device__ __forceinline__ void step_body(const StepArgs &ain)
 // At least two waves per SIMD for states up to d = 16 in the selection
 // basis: the reference-mode kernels would otherwise take more than 256
 // registers (their FP64 joint branch) and run one.
-template <int D, int MODE, int SELJ>
+template <int D, int MODE, int SELJ, bool REC = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((D > 16 || SELJ == 0) ? 1 : 2, 8))) k_step(StepArgs a)
 {
-    step_body<D, MODE, SELJ>(a);
+    step_body<D, MODE, SELJ, REC>(a);
 }
 
 // Two filters of the same shape in one launch (grid.y = filter): the pose
-// tracker advances its two arms together.
+// tracker advances its two arms together (state in rows: the paired launch
+// has twice the blocks, where rows are the faster layout).
 template <int D, int MODE, int SELJ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((D > 16 || SELJ == 0) ? 1 : 2, 8))) k_step_pair(StepArgs a0, StepArgs a1)
 {
-    step_body<D, MODE, SELJ>(blockIdx.y ? a1 : a0);
+    step_body<D, MODE, SELJ, false>(blockIdx.y ? a1 : a0);
+}
+
+// A state buffer copied into another layout (the same particles).
+template <int D>
+__global__ void __launch_bounds__(256) k_relayout(const float *xs, const float *Ps, Lay Ls, float *xd, float *Pd,
+                                                  Lay Ld, int N)
+{
+    constexpr int NP = npacked(D);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+#pragma unroll
+    for (int i = 0; i < D; ++i) xd[lidx(Ld, i, j)] = xs[lidx(Ls, i, j)];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) Pd[lidx(Ld, p, j)] = Ps[lidx(Ls, p, j)];
 }
 
 // Weights of the resampler: q_i and per-block inclusive scans.
@@ -1290,22 +1449,22 @@ static __global__ void k_iota(int *p, int n)
 // my_gmm::resetTracker (my_gmm.cpp:30-42) from the device copy of mu / Sigma.
 template <int D>
 __global__ void __launch_bounds__(256) k_reset(const int *bins, const float *mu, const float *sig,
-                                               float *x, float *P, int N, int ld)
+                                               float *x, float *P, int N, Lay L)
 {
     constexpr int NP = npacked(D);
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     const int b = bins[j];
 #pragma unroll
-    for (int i = 0; i < D; ++i) x[(size_t)i * ld + j] = mu[b * D + i];
+    for (int i = 0; i < D; ++i) x[lidx(L, i, j)] = mu[b * D + i];
 #pragma unroll
-    for (int p = 0; p < NP; ++p) P[(size_t)p * ld + j] = sig[b * NP + p];
+    for (int p = 0; p < NP; ++p) P[lidx(L, p, j)] = sig[b * NP + p];
 }
 
 // Tracks in: x' = T x, P' = T sym(P) T^T (FP64), stored FP32 SoA packed.
 template <int D>
 __global__ void __launch_bounds__(256) k_import(const double *x, const double *P, const double *T, float *xs,
-                                                float *Ps, int N, int ld)
+                                                float *Ps, int N, Lay L)
 {
     __shared__ double sT[D * D];
     for (int i = threadIdx.x; i < D * D; i += blockDim.x) sT[i] = T[i];
@@ -1316,7 +1475,7 @@ __global__ void __launch_bounds__(256) k_import(const double *x, const double *P
     for (int i = 0; i < D; ++i) {
         double acc = 0.0;
         for (int k = 0; k < D; ++k) acc = fma(sT[i * D + k], xj[k], acc);
-        xs[(size_t)i * ld + j] = (float)acc;
+        xs[lidx(L, i, j)] = (float)acc;
     }
     for (int i = 0; i < D; ++i) {
         double r[D];
@@ -1328,7 +1487,7 @@ __global__ void __launch_bounds__(256) k_import(const double *x, const double *P
         for (int q = i; q < D; ++q) {
             double acc = 0.0;
             for (int k = 0; k < D; ++k) acc = fma(r[k], sT[q * D + k], acc);
-            Ps[(size_t)pk(D, i, q) * ld + j] = (float)acc;
+            Ps[lidx(L, pk(D, i, q), j)] = (float)acc;
         }
     }
 }
@@ -1337,7 +1496,7 @@ __global__ void __launch_bounds__(256) k_import(const double *x, const double *P
 // back in the model basis: x = Ti x', P = Ti P' Ti^T (FP64, row-major).
 template <int D>
 __global__ void __launch_bounds__(256) k_export(const float *xs, const float *Ps, const int *anc,
-                                                const double *Ti, double *x, double *P, int N, int ld)
+                                                const double *Ti, double *x, double *P, int N, Lay L)
 {
     constexpr int NP = npacked(D);
     __shared__ double sT[D * D];
@@ -1347,8 +1506,8 @@ __global__ void __launch_bounds__(256) k_export(const float *xs, const float *Ps
     if (j >= N) return;
     const int s = anc ? anc[j] : j;
     double xp[D], pp[NP];
-    for (int i = 0; i < D; ++i) xp[i] = (double)xs[(size_t)i * ld + s];
-    for (int p = 0; p < NP; ++p) pp[p] = (double)Ps[(size_t)p * ld + s];
+    for (int i = 0; i < D; ++i) xp[i] = (double)xs[lidx(L, i, s)];
+    for (int p = 0; p < NP; ++p) pp[p] = (double)Ps[lidx(L, p, s)];
     double *xo = x + (size_t)j * D, *Po = P + (size_t)j * D * D;
     for (int i = 0; i < D; ++i) {
         double acc = 0.0;
@@ -1375,7 +1534,7 @@ __global__ void __launch_bounds__(256) k_export(const float *xs, const float *Ps
 // (particle, row i), FP64, the packed state read from global memory.
 template <int D>
 __global__ void __launch_bounds__(256) k_import_rows(const double *x, const double *P, const double *T, float *xs,
-                                                     float *Ps, int N, int ld)
+                                                     float *Ps, int N, Lay L)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long long)N * D) return;
@@ -1383,7 +1542,7 @@ __global__ void __launch_bounds__(256) k_import_rows(const double *x, const doub
     const double *xj = x + (size_t)j * D, *Pj = P + (size_t)j * D * D;
     double acc = 0.0;
     for (int k = 0; k < D; ++k) acc = fma(T[i * D + k], xj[k], acc);
-    xs[(size_t)i * ld + j] = (float)acc;
+    xs[lidx(L, i, j)] = (float)acc;
     double r[D];
     for (int k = 0; k < D; ++k) {
         double a = 0.0;
@@ -1393,26 +1552,26 @@ __global__ void __launch_bounds__(256) k_import_rows(const double *x, const doub
     for (int q = i; q < D; ++q) {
         double a = 0.0;
         for (int k = 0; k < D; ++k) a = fma(r[k], T[q * D + k], a);
-        Ps[(size_t)pk(D, i, q) * ld + j] = (float)a;
+        Ps[lidx(L, pk(D, i, q), j)] = (float)a;
     }
 }
 
 template <int D>
 __global__ void __launch_bounds__(256) k_export_rows(const float *xs, const float *Ps, const int *anc,
-                                                     const double *Ti, double *x, double *P, int N, int ld)
+                                                     const double *Ti, double *x, double *P, int N, Lay L)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long long)N * D) return;
     const int j = (int)(g / D), i = (int)(g % D);
     const int s = anc ? anc[j] : j;
     double acc = 0.0;
-    for (int k = 0; k < D; ++k) acc = fma(Ti[i * D + k], (double)xs[(size_t)k * ld + s], acc);
+    for (int k = 0; k < D; ++k) acc = fma(Ti[i * D + k], (double)xs[lidx(L, k, s)], acc);
     x[(size_t)j * D + i] = acc;
     double r[D];
     for (int k = 0; k < D; ++k) {
         double a = 0.0;
         for (int l = 0; l < D; ++l)
-            a = fma(Ti[i * D + l], (double)Ps[(size_t)pk(D, l < k ? l : k, l < k ? k : l) * ld + s], a);
+            a = fma(Ti[i * D + l], (double)Ps[lidx(L, pk(D, l < k ? l : k, l < k ? k : l), s)], a);
         r[k] = a;
     }
     double *Po = P + (size_t)j * D * D;
@@ -1456,8 +1615,31 @@ __device__ __forceinline__ void block_sum256(const double (&v)[D], double (*s_v)
 // accumulation in a fixed order: block sums (block_sum256) into partial,
 // then k_estimate_final adds them (thread t takes blocks t, t + 256, ... in
 // order, then block_sum256).  256 threads per block.
+// x of particle s as FP64 (zeros if !ok): 16-byte loads from a record
+// (row stride 1, D a multiple of 4), else one load per row.
 template <int D>
-__global__ void __launch_bounds__(256) k_estimate(const float *x, const int *anc, int N, int ld, double *partial)
+__device__ __forceinline__ void load_x64(const float *x, const Lay &L, int s, bool ok, double (&v)[D])
+{
+    if constexpr (D % 4 == 0) {
+        if (L.row == 1) {
+            const float4 *r = reinterpret_cast<const float4 *>(x + (size_t)s * L.part);
+#pragma unroll
+            for (int c = 0; c < D / 4; ++c) {
+                const float4 q = ok ? r[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+                v[4 * c] = q.x;
+                v[4 * c + 1] = q.y;
+                v[4 * c + 2] = q.z;
+                v[4 * c + 3] = q.w;
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) v[i] = ok ? (double)x[lidx(L, i, s)] : 0.0;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_estimate(const float *x, const int *anc, int N, Lay L, double *partial)
 {
     __shared__ double s_v[D][16][17];
     __shared__ double s_run[D][16];
@@ -1465,8 +1647,7 @@ __global__ void __launch_bounds__(256) k_estimate(const float *x, const int *anc
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     double v[D];
     const int s = (j < N) ? (anc ? anc[j] : j) : 0;
-#pragma unroll
-    for (int i = 0; i < D; ++i) v[i] = (j < N) ? (double)x[(size_t)i * ld + s] : 0.0;
+    load_x64<D>(x, L, s, j < N, v);
     block_sum256<D>(v, s_v, s_run, s_sum);
     if (threadIdx.x < D) partial[(size_t)blockIdx.x * D + threadIdx.x] = s_sum[threadIdx.x];
 }
@@ -1521,7 +1702,7 @@ __global__ void __launch_bounds__(256) k_estimate_final_pair(EstFinalArgs e0, Es
 // reduction).  Followed by k_estimate_final.  Dynamic LDS:
 // anc_estimate_lds_bytes(nb, D).
 template <int D>
-__device__ __forceinline__ void ancestors_estimate_body(const AncArgs &a, const float *x, int ld, double *partial)
+__device__ __forceinline__ void ancestors_estimate_body(const AncArgs &a, const float *x, Lay L, double *partial)
 {
     extern __shared__ unsigned long long wsp[];
     const int j0 = blockIdx.x * blockDim.x;
@@ -1534,8 +1715,7 @@ __device__ __forceinline__ void ancestors_estimate_body(const AncArgs &a, const 
         finish_resampling(a.fs, a.parity, es, rs, a.N, a.kind_linear,
                           (a.lml && a.lml_frame) ? a.lml + *a.lml_frame : a.lml);
     double v[D];
-#pragma unroll
-    for (int i = 0; i < D; ++i) v[i] = (j < a.N) ? (double)x[(size_t)i * ld + src] : 0.0;
+    load_x64<D>(x, L, src, j < a.N, v);
     __syncthreads();                     // every lane is done with the resolver's LDS
     double *ws = reinterpret_cast<double *>(wsp);
     block_sum256<D>(v, reinterpret_cast<double (*)[16][17]>(ws), reinterpret_cast<double (*)[16]>(ws + D * 16 * 17),
@@ -1544,23 +1724,23 @@ __device__ __forceinline__ void ancestors_estimate_body(const AncArgs &a, const 
 }
 
 template <int D>
-__global__ void __launch_bounds__(256) k_ancestors_estimate(AncArgs a, const float *x, int ld, double *partial)
+__global__ void __launch_bounds__(256) k_ancestors_estimate(AncArgs a, const float *x, Lay L, double *partial)
 {
-    ancestors_estimate_body<D>(a, x, ld, partial);
+    ancestors_estimate_body<D>(a, x, L, partial);
 }
 
 struct AncEstArgs {
     AncArgs an;
-    const float *x;     // the resampled filter's updated state (x rows, leading dimension ld)
+    const float *x;     // the resampled filter's updated state (layout lay)
     double *partial;    // block sums of the estimate
-    int ld;
+    Lay lay;
 };
 
 template <int D>
 __global__ void __launch_bounds__(256) k_ancestors_estimate_pair(AncEstArgs e0, AncEstArgs e1)
 {
     const AncEstArgs &e = blockIdx.y ? e1 : e0;
-    ancestors_estimate_body<D>(e.an, e.x, e.ld, e.partial);
+    ancestors_estimate_body<D>(e.an, e.x, e.lay, e.partial);
 }
 
 __host__ __device__ constexpr int anc_estimate_lds_bytes(int nb, int D)
@@ -1571,11 +1751,11 @@ __host__ __device__ constexpr int anc_estimate_lds_bytes(int nb, int D)
 
 // Both estimate kernels on one stream.
 template <int D>
-inline void launch_estimate(hipStream_t st, const float *x, const int *anc, int N, int ld, double *partial,
+inline void launch_estimate(hipStream_t st, const float *x, const int *anc, int N, Lay L, double *partial,
                             const double *Ti, double *out)
 {
     const int nb = (N + 255) / 256;
-    hipLaunchKernelGGL((k_estimate<D>), dim3(nb), dim3(256), 0, st, x, anc, N, ld, partial);
+    hipLaunchKernelGGL((k_estimate<D>), dim3(nb), dim3(256), 0, st, x, anc, N, L, partial);
     hipLaunchKernelGGL((k_estimate_final<D>), dim3(1), dim3(256), 0, st, (const double *)partial, nb, N, Ti, out);
 }
 

@@ -958,6 +958,12 @@ int pf_tracker_create(pf_tracker **out, const pf_tracker_config *cfg, pf_handle 
     if (!(c.p_assoc >= 0.0 && c.p_assoc < 0.5) || !(c.z_floor > 0.0) || !(c.spread > 0.0))
         return set_error(PF_ERR_ARG, "bad association constants");
     if (!(c.fx != 0.0 && c.fy != 0.0)) return set_error(PF_ERR_ARG, "zero focal length");
+    // the arms' state in rows: their paired step launches have twice the
+    // blocks of one filter, where rows are the faster layout
+    for (pf_handle *arm : {arm1, arm2}) {
+        const int st = pf_set_state_layout(arm, PF_LAYOUT_ROWS);
+        if (st) return st;
+    }
 
     pf_tracker *tr = new pf_tracker();
     tr->cfg = c;

@@ -177,6 +177,20 @@ class Rbpf:
         check(self.lib.pf_last_run_graph_frames(self.h, C.byref(n)))
         return n.value
 
+    @property
+    def state_layout(self) -> int:
+        """Device layout of the state in use: capi.PF_LAYOUT_ROWS or
+        PF_LAYOUT_RECORDS (pf_get_state_layout)."""
+        n = C.c_int32(0)
+        check(self.lib.pf_get_state_layout(self.h, C.byref(n)))
+        return n.value
+
+    @state_layout.setter
+    def state_layout(self, layout: int) -> None:
+        """PF_LAYOUT_AUTO, PF_LAYOUT_ROWS or PF_LAYOUT_RECORDS; the current
+        tracks are converted (pf_set_state_layout)."""
+        check(self.lib.pf_set_state_layout(self.h, int(layout)))
+
 
 # ============================================ reference-named mirror classes
 class KF_model:

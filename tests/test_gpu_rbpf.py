@@ -427,3 +427,88 @@ def test_graph_groups_serve_consecutive_runs(gpu):
     for (xa, Pa), (xb, Pb) in zip(res[0][0], res[1][0]):
         assert np.array_equal(xa, xb) and np.array_equal(Pa, Pb)
     assert np.array_equal(res[0][1], res[1][1])
+
+
+# ------------------------------------------------------------- state layouts
+@pytest.mark.parametrize("mode", ["reference", "candidates"])
+def test_state_layouts_give_identical_runs(gpu, mode):
+    """Rows and per-particle records hold the same filter: host-driven steps
+    and device-resident runs (with graph replay) give bit-identical tracks,
+    ancestors, log-weights and log mean weights in either layout, and a
+    layout change between frames converts the tracks exactly."""
+    from mkfbodytracker_pdaf_amd import capi
+    from mkfbodytracker_pdaf_amd.devmem import DeviceArray
+    N, T = 3000, 12
+    handles = []
+    for layout in (capi.PF_LAYOUT_ROWS, capi.PF_LAYOUT_RECORDS):
+        mdl, rb, _ = _setup(N)
+        rb.state_layout = layout
+        assert rb.state_layout == layout
+        if mode == "candidates":
+            rb.set_joint_clutter(mdl.clutter)
+        handles.append(rb)
+    seq = synthetic.simulate(mdl, 2 * T, seed=31, cand_per_joint=4, cap=8)
+    bins, _ = oracle.resample_linear(mdl.weights, N, 99)
+    outs = []
+    for i, rb in enumerate(handles):
+        rb.reset(bins)
+        frames = []
+        for t in range(T):
+            U1, U2 = int(seq.uniforms[t, 0]), int(seq.uniforms[t, 1])
+            if t == T // 2:        # switch the layout between frames: the tracks are converted
+                rb.state_layout = capi.PF_LAYOUT_RECORDS if i == 0 else capi.PF_LAYOUT_ROWS
+            if mode == "candidates":
+                st, out = rb.step_candidates(seq.count[t], seq.cand[t], U1, U2)
+            else:
+                st, out = rb.step(synthetic.per_particle_measurements(seq.z[t], N, seed=t), U1, U2)
+            frames.append(out)
+        x, P = rb.get_tracks()
+        d_uni = DeviceArray(seq.uniforms[T:].astype(np.uint32))
+        d_lml = DeviceArray.empty(T, np.float64)
+        if mode == "candidates":
+            d_cnt = DeviceArray(seq.count[T:].astype(np.int32))
+            d_cand = DeviceArray(seq.cand[T:].astype(np.float32))
+            rb.run_candidates_dev(T, d_cnt.ptr, d_cand.ptr, d_uni.ptr, d_lml.ptr, 1)
+        else:
+            Zs = np.stack([synthetic.per_particle_measurements(seq.z[T + t], N, seed=T + t) for t in range(T)])
+            d_Z = DeviceArray(Zs.astype(np.float32))
+            rb.run_ref_dev(T, d_Z.ptr, d_uni.ptr, d_lml.ptr, 1)
+        rb.synchronize()
+        assert rb.last_graph_frames() == T
+        x2, P2 = rb.get_tracks()
+        outs.append((frames, x, P, d_lml.get(), x2, P2, rb.estimate()))
+    (fa, xa, Pa, la, x2a, P2a, ea), (fb, xb, Pb, lb, x2b, P2b, eb) = outs
+    for t in range(T):
+        for k in ("lw", "anc", "comp"):
+            assert np.array_equal(fa[t][k], fb[t][k]), (t, k)
+        assert fa[t]["lml"] == fb[t]["lml"], t
+    assert np.array_equal(xa, xb) and np.array_equal(Pa, Pb)
+    assert np.array_equal(la, lb)
+    assert np.array_equal(x2a, x2b) and np.array_equal(P2a, P2b)
+    assert np.array_equal(ea, eb)
+    for rb in handles:
+        rb.close()
+
+
+def test_state_layout_choice(gpu):
+    """AUTO keeps records while the step's 256-particle blocks fit two waves
+    per SIMD at once (N <= 512 x compute units) and rows above; records are
+    refused for d = 24; an unknown layout is an argument error."""
+    import torch
+    from mkfbodytracker_pdaf_amd import capi
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    for N, want in ((1000, capi.PF_LAYOUT_RECORDS), (512 * cus, capi.PF_LAYOUT_RECORDS),
+                    (512 * cus + 1, capi.PF_LAYOUT_ROWS)):
+        _, rb, _ = _setup(N)
+        assert rb.state_layout == want, N
+        rb.close()
+    mdl24 = synthetic.make_model(seed=5, K=4, d=24, joints=("head", "left_hand", "right_hand"))
+    rb = Rbpf(100)
+    rb.load_model(mdl24.means, mdl24.covs, mdl24.weights, mdl24.gamma, mdl24.pca_proj, mdl24.pca_mean,
+                  sel=mdl24.sel, R=mdl24.R)
+    assert rb.state_layout == capi.PF_LAYOUT_ROWS
+    with pytest.raises(Exception):
+        rb.state_layout = capi.PF_LAYOUT_RECORDS
+    with pytest.raises(Exception):
+        rb.state_layout = 7
+    rb.close()
