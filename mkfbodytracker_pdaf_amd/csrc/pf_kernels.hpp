@@ -701,55 +701,59 @@ __host__ __device__ constexpr int rec_floats(int D) { return (D + npacked(D) + 3
 __host__ __device__ constexpr bool rec_layout(int D) { return D <= 12; }     // records compiled for this d
 __host__ __device__ inline size_t lidx(const Lay &L, int r, int j) { return (size_t)r * L.row + (size_t)j * L.part; }
 
-// LDS of the record gather and store of the step: per wave 64 particles x 4
-// chunks of 16 bytes (+1 of padding against bank conflicts), 4 waves.
-constexpr int kRecStageBytes = 4 * 64 * 5 * 16;
+// LDS of the record gather and store of the step: per wave the records of
+// 16 particles (d <= 12: at most 16 x 23 chunks of 16 bytes), 4 waves.
+constexpr int kRecStageChunks = 16 * 23;
+constexpr int kRecStageBytes = 4 * kRecStageChunks * 16;
 
-// Wave-cooperative gather of the records of the wave's 64 particles: lane q
-// of round r loads 16-byte chunk r + (q & 3) of particle q >> 2's ancestor
-// src (so consecutive lanes read consecutive bytes of a record), the chunks
-// pass through the wave's LDS slice gw, and each lane takes its particle's
-// chunks into x and P.  Particles j0w + p >= N read nothing (the buffer
-// resource's bound returns zeros for them).  Every lane of the wave calls it.
+// Wave-cooperative gather of the records of the wave's 64 particles, in four
+// rounds of 16: lane k of a round loads 16-byte chunk k % NCH of particle
+// k / NCH's ancestor src (consecutive lanes read consecutive bytes of one
+// record, then of the next), the chunks go to the wave's LDS slice gw, and
+// the round's 16 lanes take their particle's chunks into x and P.  Particles
+// j0w + p >= N read nothing (the buffer resource's bound returns zeros for
+// them).  Every lane of the wave calls it.
 template <int D>
 __device__ __forceinline__ void rec_gather(const float *rec, int N, int j0w, int src, float4 *gw, float (&x)[D],
                                            float (&P)[npacked(D)])
 {
     constexpr int NP = npacked(D), RS = rec_floats(D), NCH = RS / 4;
+    static_assert(16 * NCH <= kRecStageChunks, "a round of 16 records exceeds the LDS slice");
     const int lane = threadIdx.x & 63;
     const unsigned lim = (unsigned)N * RS * 4u;
     const __amdgpu_buffer_rsrc_t ra =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(rec), (short)0, (int)lim, 0x00020000);
 #pragma unroll
-    for (int r0 = 0; r0 < NCH; r0 += 4) {
-        u32x4 c[4];
+    for (int g = 0; g < 4; ++g) {
+        constexpr int NI = (16 * NCH + 63) / 64;
+        u32x4 c[NI];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int q = i * 64 + lane, p = q >> 2, part = r0 + (q & 3);
-            const int sp = __shfl(src, p, 64);
-            const bool ok = part < NCH && j0w + p < N;
-            c[i] = __builtin_amdgcn_raw_buffer_load_b128(ra, ok ? (int)((unsigned)(sp * RS + part * 4) * 4u) : (int)lim,
-                                                         0, 0);
+        for (int i = 0; i < NI; ++i) {
+            const int k = i * 64 + lane;
+            const int p = k / NCH, ch = k - p * NCH;
+            const int sp = __shfl(src, 16 * g + (p < 16 ? p : 15), 64);
+            const bool ok = k < 16 * NCH && j0w + 16 * g + p < N;
+            c[i] = __builtin_amdgcn_raw_buffer_load_b128(ra, ok ? (int)((unsigned)(sp * RS + ch * 4) * 4u) : (int)lim, 0, 0);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int q = i * 64 + lane;
-            reinterpret_cast<u32x4 *>(gw)[(q >> 2) * 5 + (q & 3)] = c[i];
+        for (int i = 0; i < NI; ++i) {
+            const int k = i * 64 + lane;
+            if (k < 16 * NCH) reinterpret_cast<u32x4 *>(gw)[k] = c[i];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if ((lane >> 4) == g) {
 #pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-            if (r0 + cc < NCH) {
-                const float4 v = gw[lane * 5 + cc];
+            for (int cc = 0; cc < NCH; ++cc) {
+                const float4 v = gw[(lane & 15) * NCH + cc];
                 const float ve[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int f = (r0 + cc) * 4 + e;
+                    const int f = cc * 4 + e;
                     if (f < D) x[f] = ve[e];
                     else if (f < D + NP) P[f - D] = ve[e];
                 }
@@ -761,42 +765,46 @@ __device__ __forceinline__ void rec_gather(const float *rec, int N, int j0w, int
     }
 }
 
-// The inverse: the wave's 64 records (particles j0w .. j0w + 63) written
-// from x and P through gw, consecutive lanes writing consecutive bytes of a
-// record.  These stores write partial lines per instruction, so they keep
-// the default (write-back) policy.  Every lane of the wave calls it.
+// The inverse: the wave's 64 records (particles j0w .. j0w + 63, contiguous
+// in the buffer) written from x and P through gw in four rounds of 16; each
+// round stores 16 whole records as consecutive 16-byte chunks, so the stores
+// write whole lines and carry device scope like the row stores.  Every lane
+// of the wave calls it.
 template <int D>
 __device__ __forceinline__ void rec_store(float *rec, int N, int j0w, float4 *gw, const float (&x)[D],
                                           const float (&P)[npacked(D)])
 {
     constexpr int NP = npacked(D), RS = rec_floats(D), NCH = RS / 4;
+    static_assert(16 * NCH <= kRecStageChunks, "a round of 16 records exceeds the LDS slice");
     const int lane = threadIdx.x & 63;
     const unsigned lim = (unsigned)N * RS * 4u;
     const __amdgpu_buffer_rsrc_t wa = __builtin_amdgcn_make_buffer_rsrc(rec, (short)0, (int)lim, 0x00020000);
 #pragma unroll
-    for (int r0 = 0; r0 < NCH; r0 += 4) {
+    for (int g = 0; g < 4; ++g) {
+        if ((lane >> 4) == g) {
 #pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-            if (r0 + cc < NCH) {
+            for (int cc = 0; cc < NCH; ++cc) {
                 float ve[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int f = (r0 + cc) * 4 + e;
+                    const int f = cc * 4 + e;
                     ve[e] = f < D ? x[f] : (f < D + NP ? P[f - D] : 0.f);
                 }
-                gw[lane * 5 + cc] = make_float4(ve[0], ve[1], ve[2], ve[3]);
+                gw[(lane & 15) * NCH + cc] = make_float4(ve[0], ve[1], ve[2], ve[3]);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const unsigned base = (unsigned)(j0w + 16 * g) * RS * 4u;
+        const int nvalid = min(16, N - (j0w + 16 * g)) * NCH;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int q = i * 64 + lane, p = q >> 2, part = r0 + (q & 3);
-            const bool ok = part < NCH && j0w + p < N;
-            const u32x4 v = reinterpret_cast<const u32x4 *>(gw)[p * 5 + (q & 3)];
-            __builtin_amdgcn_raw_buffer_store_b128(v, wa, ok ? (int)((unsigned)((j0w + p) * RS + part * 4) * 4u)
-                                                           : (int)lim, 0, 0);
+        for (int i = 0; i < (16 * NCH + 63) / 64; ++i) {
+            const int k = i * 64 + lane;
+            const bool ok = k < 16 * NCH && k < nvalid;
+            const u32x4 v = reinterpret_cast<const u32x4 *>(gw)[k < 16 * NCH ? k : 0];
+            __builtin_amdgcn_raw_buffer_store_b128(v, wa, ok ? (int)(base + (unsigned)k * 16u) : (int)lim, 0,
+                                                   kStoreDeviceScope);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1220,7 +1228,7 @@ __device__ __forceinline__ void step_body(const StepArgs &ain)
     // resolver's workspace (free once every thread has its ancestor)
     const int j0w = j0 + (int)(threadIdx.x & ~63u);
     float4 *gw = reinterpret_cast<float4 *>(reinterpret_cast<float *>(sm4) + ((a.stage_floats + 3) & ~3)) +
-                 (threadIdx.x >> 6) * 64 * 5;
+                 (threadIdx.x >> 6) * kRecStageChunks;
     static_assert(!REC || rec_layout(D), "records are compiled for d <= 12");
     if constexpr (REC) {
         if (a.anc_mode == kAncFused) __syncthreads();
