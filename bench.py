@@ -414,6 +414,7 @@ def measure_adjoint(shape: str) -> dict:
 
 def run_adjoint(args):
     """The adjoint workload (measure_adjoint) as its own bench line."""
+    calib_copy(args.calib_copy_mb)
     m = measure_adjoint(args.adj_shape)
     S, N, T, C = m["S"], m["N"], m["T"], m["C"]
     t_fwd, t_grad, rec_fwd_ms, bwd_ms = m["t_fwd"], m["t_grad"], m["rec_fwd_ms"], m["bwd_ms"]
