@@ -2,7 +2,7 @@
 //
 // The handle is one reference ParticleFilter (src/pf2DRao.h:13-31) with its
 // my_gmm model tables resident in HBM.  Host code here only validates,
-// converts layouts (row-major double <-> FP32 SoA packed) and launches; every
+// converts layouts (row-major double <-> FP32 packed records or rows) and launches; every
 // arithmetic step of the filter runs in the kernels of pf_kernels.hpp.
 #include <hip/hip_runtime.h>
 

@@ -2,11 +2,13 @@
 // (reference: src/pf2DRao.cpp, src/KF_model.cpp, src/my_gmm.cpp).
 //
 // Device layout (one filter):
-//   state   x[D][ld], P[D(D+1)/2][ld]  FP32 structure-of-arrays, packed upper
-//           triangle of the symmetric covariance, ld = particle capacity.
-//           Two copies: the step reads copy A at the ancestor index and writes
-//           copy B at the particle index (the resampling gather of
-//           pf2DRao.cpp:153-156 is folded into the next frame's loads).
+//   state   x (D) and the packed upper triangle of the symmetric
+//           covariance P (D(D+1)/2), FP32, either as per-particle records or
+//           as rows x[D][ld], P[D(D+1)/2][ld] (Lay below; ld = particle
+//           capacity).  Two copies: the step reads copy A at the ancestor
+//           index and writes copy B at the particle index (the resampling
+//           gather of pf2DRao.cpp:153-156 is folded into the next frame's
+//           loads).
 //   table   LDS image of the model: 2K words of component thresholds
 //           C_pi[k] * N * 2^32 (128-bit), then per component k
 //           [g, g^2, (1-g) mu_k (D), (1-g^2) Sigma_k packed], then the shared
@@ -1469,7 +1471,7 @@ __global__ void __launch_bounds__(256) k_reset(const int *bins, const float *mu,
     for (int p = 0; p < NP; ++p) P[lidx(L, p, j)] = sig[b * NP + p];
 }
 
-// Tracks in: x' = T x, P' = T sym(P) T^T (FP64), stored FP32 SoA packed.
+// Tracks in: x' = T x, P' = T sym(P) T^T (FP64), stored FP32 packed in layout L.
 template <int D>
 __global__ void __launch_bounds__(256) k_import(const double *x, const double *P, const double *T, float *xs,
                                                 float *Ps, int N, Lay L)
