@@ -91,7 +91,7 @@ k_batch_step(BatchArgs a)
     const unsigned rowb = (unsigned)a.ld * 4u;
     unsigned long long *wsp = reinterpret_cast<unsigned long long *>(reinterpret_cast<float *>(sm4) + a.stage_floats);
     if (a.wfused && blockIdx.x == 0 && threadIdx.x == 0) *a.frame_next = t + 1;
-    for (int tile = blockIdx.x; tile < ntiles; tile += PERSIST ? gridDim.x : ntiles) {
+    for (int tile = PERSIST ? (int)blockIdx.x : xcd_tile(); tile < ntiles; tile += PERSIST ? gridDim.x : ntiles) {
         const int s = uni_int(a.tile_seq[tile]);
         const int jt0 = uni_int(a.tile_j0[tile]);
         const SeqDesc sd = uniform_seq(a.seqs, s);

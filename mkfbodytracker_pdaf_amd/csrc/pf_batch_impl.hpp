@@ -83,6 +83,17 @@ __device__ __forceinline__ SeqDesc uniform_seq(const SeqDesc *seqs, int s)
     return r;
 }
 
+// Block index -> tile index so that each XCD works on a contiguous range of
+// tiles.  Workgroups are dispatched to the 8 XCDs round robin (block b on XCD
+// b mod 8); with this bijection of [0, gridDim.x) the blocks of one XCD take
+// consecutive tiles, i.e. few sequences, whose model rows then stay in that
+// XCD's L2 (the d = 24, K = 64 images are read from global memory).
+__device__ __forceinline__ int xcd_tile()
+{
+    const int G = (int)gridDim.x, b = (int)blockIdx.x, q = G >> 3, r = G & 7, c = b & 7, i = b >> 3;
+    return c < r ? c * (q + 1) + i : r * (q + 1) + (c - r) * q + i;
+}
+
 // The resampler weights of frame t (set t & 1).
 struct WeightSet {
     unsigned long long *cloc, *bsum;

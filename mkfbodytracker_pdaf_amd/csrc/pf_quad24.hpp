@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))
     constexpr int D = 24, NP = npacked(24), CS = 2 + D + NP;
     extern __shared__ float4 sm4[];
     const int t = *a.frame;
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile();
     if (tile >= a.n_tiles[t]) return;                   // block-uniform: inactive sequences' tiles
     const bool odd = (t & 1) != 0;
     const float *xin = odd ? a.buf_x[1] : a.buf_x[0], *Pin = odd ? a.buf_P[1] : a.buf_P[0];
