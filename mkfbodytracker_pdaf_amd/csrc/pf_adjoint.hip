@@ -87,22 +87,67 @@ __host__ __device__ constexpr int term_slot_floats(int kc, int nwaves)
     return ((nwaves * kc * kGV + 1) & ~1) + 2 * nwaves;
 }
 
-// Sum of v over the wave by a fixed tree (rows of 16 lanes by DPP, then the
-// four rows), the same in every lane and in every run.
-__device__ __forceinline__ float wave_sum_fixed(float v)
+// One reduce-scatter step inside the rows of 16 lanes: the lanes with the
+// mask bit clear keep the first of each pair (lo[i], hi[i]) and the others
+// the second, each adding the value its DPP partner (a lane of the other
+// half) holds for it.
+template <int CTRL>
+__device__ __forceinline__ float rs_step(bool upper, float lo, float hi)
 {
-    v = row_sum16(v);
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    return v;
+    const float keep = upper ? hi : lo, send = upper ? lo : hi;
+    return keep + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(send), CTRL, 0xF, 0xF, true));
+}
+
+// The wave sums of the kGV = 92 terms of the lanes with sel, into dst[0..91],
+// by a fixed reduce-scatter butterfly: the two lane halves by
+// v_permlane32_swap (46 pairs), the rows 0/1 and 2/3 by v_permlane16_swap
+// (23 pairs; row r then holds terms 23 r .. 23 r + 22), then inside each row
+// of 16 by DPP row_mirror, row_half_mirror, quad_perm [2,3,0,1] and
+// quad_perm [1,0,3,2].  Lane l of row r ends with the sums of terms
+// 23 r + o and 23 r + o + 1, o = 12 b3 + 6 b2 + 3 b1 + 2 b0 from the bits of
+// l mod 16 (the second only for b0 = 0; o = 23 is padding).  The tree is
+// fixed by the lane positions, so every run adds in the same order.
+__device__ __forceinline__ void wave_sums_scatter(bool sel, const float (&term)[kGV], float *dst)
+{
+    static_assert(kGV == 4 * 23, "the butterfly splits the terms as 4 rows of 23");
+    const int lane = threadIdx.x & 63, row = lane >> 4;
+    const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0, b1 = (lane & 2) != 0, b0 = (lane & 1) != 0;
+    float u[46];
+#pragma unroll
+    for (int i = 0; i < 46; ++i) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(sel ? term[i] : 0.f),
+                                                        __float_as_uint(sel ? term[i + 46] : 0.f), false, false);
+        u[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    float w[24];
+#pragma unroll
+    for (int i = 0; i < 23; ++i) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(u[i]), __float_as_uint(u[i + 23]), false,
+                                                        false);
+        w[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    w[23] = 0.f;
+    float x[12], y[6], z[4], q[2];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) x[i] = rs_step<0x140>(b3, w[i], w[i + 12]);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) y[i] = rs_step<0x141>(b2, x[i], x[i + 6]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) z[i] = rs_step<0x4E>(b1, y[i], y[i + 3]);
+    z[3] = 0.f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) q[i] = rs_step<0xB1>(b0, z[i], z[i + 2]);
+    const int o = 12 * b3 + 6 * b2 + 3 * b1 + 2 * b0;
+    if (o < 23) dst[23 * row + o] = q[0];
+    if (!b0) dst[23 * row + o + 1] = q[1];
 }
 
 // The per-particle component terms of a tile added to its FP64 accumulator
 // tacc (K x kGV) in an order fixed by the tile alone, so gradients are
 // bit-reproducible: components in groups of kc; per distinct component of
-// the group in a wave, the wave's sums (fixed tree) go to that wave's LDS
-// slot; after a barrier the slots are added in wave order, in FP64, to tacc.
-// Every thread of the block calls it.
+// the group in a wave, the wave's sums (fixed butterfly) go to that wave's
+// LDS slot; after a barrier the slots are added in wave order, in FP64, to
+// tacc.  Every thread of the block calls it.
 __device__ __forceinline__ void reduce_terms(bool live, int k, const float (&term)[kGV], float *slots, double *tacc,
                                              int K, int kc)
 {
@@ -116,20 +161,7 @@ __device__ __forceinline__ void reduce_terms(bool live, int k, const float (&ter
             const int kk = __shfl(k, leader, 64);
             const bool sel = live && k == kk;
             rem &= ~__ballot(sel);
-            float *dst = slots + ((size_t)wid * kc + (kk - g0)) * kGV;
-            constexpr int kChunk = 8;
-#pragma unroll
-            for (int v0 = 0; v0 < kGV; v0 += kChunk) {
-                float r[kChunk];
-#pragma unroll
-                for (int i = 0; i < kChunk; ++i)
-                    if (v0 + i < kGV) r[i] = wave_sum_fixed(sel ? term[v0 + i] : 0.f);
-                if (lane == 0) {
-#pragma unroll
-                    for (int i = 0; i < kChunk; ++i)
-                        if (v0 + i < kGV) dst[v0 + i] = r[i];
-                }
-            }
+            wave_sums_scatter(sel, term, slots + ((size_t)wid * kc + (kk - g0)) * kGV);
             mine |= 1ull << (kk - g0);
         }
         if (lane == 0) wmask[wid] = mine;
@@ -678,10 +710,12 @@ int pf_batch_lml_grad(pf_batch *b, const int32_t *count, const float *cand, cons
     auto seg_input_P = [&](int seg, int t, int t0) -> const float * {
         return t == t0 ? b->ck_P + (size_t)seg * Pw : b->seg_P[seg & 1] + (size_t)(t - 1 - t0) * Pw;
     };
-    auto replay_seg = [&](int seg) -> int {
+    // states = false: the recorded forward already wrote the segment's states
+    // (the last segment), only its child ranges are computed.
+    auto replay_seg = [&](int seg, bool states) -> int {
         const int t0 = seg * C, t1 = std::min(t0 + C, b->Tmax);
         AdjArgs r = a;
-        for (int t = t0; t + 1 < t1; ++t) {          // states of frames t0 .. t1-2
+        for (int t = t0; states && t + 1 < t1; ++t) {   // states of frames t0 .. t1-2
             r.t = t;
             r.t0 = t0;
             r.in_x = seg_input_x(seg, t, t0);
@@ -725,11 +759,11 @@ int pf_batch_lml_grad(pf_batch *b, const int32_t *count, const float *cand, cons
         return PF_OK;
     };
     const int last = b->n_ckpt - 1;
-    if ((st = replay_seg(last))) return st;
+    if ((st = replay_seg(last, false))) return st;
     for (int seg = last; seg >= 0; --seg) {
         if (seg > 0) {
             if (seg + 1 <= last) HIP_TRYB(hipStreamWaitEvent(R, b->ev_back[seg + 1], 0));
-            if ((st = replay_seg(seg - 1))) return st;
+            if ((st = replay_seg(seg - 1, true))) return st;
         }
         if ((st = backward_seg(seg))) return st;
     }

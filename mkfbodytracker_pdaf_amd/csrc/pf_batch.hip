@@ -619,6 +619,19 @@ int run_all(pf_batch *b, const int32_t *count, const float *cand, const uint32_t
                 HIP_TRYB(hipMemcpyAsync(b->ck_P + c * Pw, b->bP[f & 1], sizeof(float) * Pw, hipMemcpyDeviceToDevice,
                                         b->stream));
             }
+            // The frames of the last checkpoint segment write their updated
+            // states into that segment's slots (slot f - t0 for frames t0 ..
+            // frames - 2, where the backward's replay would put them) and
+            // read them from there, so the backward starts without a replay.
+            const int t0 = (b->n_ckpt - 1) * b->ckpt_every, q = (b->n_ckpt - 1) & 1;
+            if (f - 1 >= t0 && f - 1 <= frames - 2) {
+                a.buf_x[f & 1] = b->seg_x[q] + (size_t)(f - 1 - t0) * xw;
+                a.buf_P[f & 1] = b->seg_P[q] + (size_t)(f - 1 - t0) * Pw;
+            }
+            if (f >= t0 && f <= frames - 2) {
+                a.buf_x[(f + 1) & 1] = b->seg_x[q] + (size_t)(f - t0) * xw;
+                a.buf_P[(f + 1) & 1] = b->seg_P[q] + (size_t)(f - t0) * Pw;
+            }
         }
         int st = launch_batch_step(b, a);
         if (st) return st;
