@@ -225,13 +225,28 @@ __host__ __device__ constexpr int resolve_lds_words(int nb, int stage = kStageBl
     return nb + 1 + stage + 16 + (nb + 1) / 2;
 }
 
+// A load through a pointer known to address global memory (a global rather
+// than a flat instruction: it counts only against the vector-memory counter,
+// so later waits need not cover it unless its value is used).
+template <typename T>
+__device__ __forceinline__ T gload(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
 // Phase 1 of the ancestor resolution: the largest block exponent E, the
 // exclusive prefix of the aligned block sums in LDS (pre[0..nb], pre[nb] =
 // total) with each block's shift, and, if want_es, the FP64 sum of the block
 // weight sums scaled to 2^E in es.  Every thread of the block must call it
-// (want_es block-uniform).
+// (want_es block-uniform).  hook() runs once the block sums' loads are
+// issued: a caller's own loads placed there share their latency.
+template <typename Hook = NoHook>
 __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, unsigned long long *w, bool want_es,
-                                                             double &es_out, int &pow_out)
+                                                             double &es_out, int &pow_out, Hook hook = Hook())
 {
     const int tid = threadIdx.x, nt = blockDim.x, nb = pd.nb;
     const int lane = tid & 63, wid = tid >> 6, nwaves = (nt + 63) >> 6;
@@ -253,9 +268,9 @@ __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, 
     }
     // 0. E, the largest block exponent (0 without exponents)
     int E = 0;
-    if (pd.bpow && pd.pow_max) {
-        E = *pd.pow_max;
-    } else if (pd.bpow) {
+    if (pd.bpow && pd.pow_max) E = gload(pd.pow_max);
+    hook();
+    if (pd.bpow && !pd.pow_max) {
         int em = kPowEmpty;
 #pragma unroll
         for (int u = 0; u < kReg; ++u) em = max(em, re[u]);
@@ -417,11 +432,12 @@ struct ResolveStats {
     int pow;
 };
 
-// Both phases for one block of outputs.
+// Both phases for one block of outputs (hook: as for resolve_prefix).
+template <typename Hook = NoHook>
 __device__ __forceinline__ int resolve_ancestor(const Pending &pd, int N, uint32_t U, int j0, unsigned long long *w,
-                                                bool want_es, double &es_out, ResolveStats &st)
+                                                bool want_es, double &es_out, ResolveStats &st, Hook hook = Hook())
 {
-    st.total = resolve_prefix(pd, w, want_es, es_out, st.pow);
+    st.total = resolve_prefix(pd, w, want_es, es_out, st.pow, hook);
     return resolve_search(pd, N, U, st.total, j0, w);
 }
 
@@ -633,13 +649,13 @@ struct Joint {
 // The loads are unconditional buffer loads (out-of-range lanes read 0), so
 // the chunk stays in registers.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void stage_table(float4 *dst, const float4 *__restrict__ src, int n4)
+__device__ __forceinline__ void stage_table(float4 *dst, const float4 *__restrict__ src, int n4, int start = 0)
 {
     constexpr int kChunk = 4;
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(src), (short)0, n4 * 16, 0x00020000);
     u32x4 *d4 = reinterpret_cast<u32x4 *>(dst);
-    for (int base = 0; base < n4; base += kChunk * (int)blockDim.x) {
+    for (int base = start; base < n4; base += kChunk * (int)blockDim.x) {
         u32x4 r[kChunk];
 #pragma unroll
         for (int it = 0; it < kChunk; ++it) {
@@ -652,6 +668,33 @@ __device__ __forceinline__ void stage_table(float4 *dst, const float4 *__restric
             if (i < n4) d4[i] = r[it];
         }
     }
+}
+
+// The first 4 x blockDim float4 of the image loaded into registers ahead of
+// their use (out-of-range chunks read as zero), so their latency overlaps
+// other loads; table_store writes them, and stages any rest, like
+// stage_table.
+struct TablePrefetch {
+    u32x4 r[4];
+};
+__device__ __forceinline__ void table_prefetch(TablePrefetch &tp, const float4 *__restrict__ src, int n4)
+{
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(src), (short)0, n4 * 16, 0x00020000);
+#pragma unroll
+    for (int it = 0; it < 4; ++it)
+        tp.r[it] = __builtin_amdgcn_raw_buffer_load_b128(rs, (it * (int)blockDim.x + (int)threadIdx.x) * 16, 0, 0);
+}
+__device__ __forceinline__ void table_store(const TablePrefetch &tp, float4 *dst, const float4 *__restrict__ src,
+                                            int n4)
+{
+    u32x4 *d4 = reinterpret_cast<u32x4 *>(dst);
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const int i = it * (int)blockDim.x + (int)threadIdx.x;
+        if (i < n4) d4[i] = tp.r[it];
+    }
+    stage_table(dst, src, n4, 4 * (int)blockDim.x);
 }
 
 // Structure-of-arrays rows through buffer instructions: one resource per
@@ -711,8 +754,9 @@ constexpr int kRecStageBytes = 4 * kRecStageChunks * 16;
 // Wave-cooperative gather of the records of the wave's 64 particles, in four
 // rounds of 16: lane k of a round loads 16-byte chunk k % NCH of particle
 // k / NCH's ancestor src (consecutive lanes read consecutive bytes of one
-// record, then of the next), the chunks go to the wave's LDS slice gw, and
-// the round's 16 lanes take their particle's chunks into x and P.  Particles
+// record, then of the next); the four rounds' loads are issued together,
+// then each round's chunks go to the wave's LDS slice gw in turn and the
+// round's 16 lanes take their particle's chunks into x and P.  Particles
 // j0w + p >= N read nothing (the buffer resource's bound returns zeros for
 // them).  Every lane of the wave calls it.
 template <int D>
@@ -725,25 +769,31 @@ __device__ __forceinline__ void rec_gather(const float *rec, int N, int j0w, int
     const unsigned lim = (unsigned)N * RS * 4u;
     const __amdgpu_buffer_rsrc_t ra =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(rec), (short)0, (int)lim, 0x00020000);
+    // every round's loads first: the wave's 64 records share one memory
+    // latency; the rounds then pass through the LDS slice in turn
+    constexpr int NI = (16 * NCH + 63) / 64;
+    u32x4 c[4][NI];
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-        constexpr int NI = (16 * NCH + 63) / 64;
-        u32x4 c[NI];
 #pragma unroll
         for (int i = 0; i < NI; ++i) {
             const int k = i * 64 + lane;
             const int p = k / NCH, ch = k - p * NCH;
             const int sp = __shfl(src, 16 * g + (p < 16 ? p : 15), 64);
             const bool ok = k < 16 * NCH && j0w + 16 * g + p < N;
-            c[i] = __builtin_amdgcn_raw_buffer_load_b128(ra, ok ? (int)((unsigned)(sp * RS + ch * 4) * 4u) : (int)lim, 0, 0);
+            c[g][i] = __builtin_amdgcn_raw_buffer_load_b128(ra, ok ? (int)((unsigned)(sp * RS + ch * 4) * 4u) : (int)lim,
+                                                            0, 0);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
 #pragma unroll
         for (int i = 0; i < NI; ++i) {
             const int k = i * 64 + lane;
-            if (k < 16 * NCH) reinterpret_cast<u32x4 *>(gw)[k] = c[i];
+            if (k < 16 * NCH) reinterpret_cast<u32x4 *>(gw)[k] = c[g][i];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1198,10 +1248,24 @@ __device__ __forceinline__ void step_body(const StepArgs &ain)
     extern __shared__ float4 sm4[];
     int t;
     const StepArgs a = frame_args(ain, t);
-    const uint32_t U1 = a.uni[2 * t];
+    const uint32_t U1 = gload(a.uni + 2 * t);
     const int j0 = blockIdx.x * blockDim.x;
     const int j = j0 + threadIdx.x;
     const bool live = j < a.N;
+    const float4 *table4 = reinterpret_cast<const float4 *>(a.table);
+    const int *fcount = MODE == kModeCand ? a.count + (size_t)t * a.J : nullptr;
+    const float *fcand = MODE == kModeCand ? a.cand + (size_t)t * a.J * a.cap * 2 : nullptr;
+    // Records: the model image and the frame's detections are loaded beside
+    // the resolver's first loads (one shared memory latency; their LDS area
+    // is not the resolver's) and stored to LDS before the gather.
+    TablePrefetch tpf;
+    CandPrefetch cpf;
+    auto prefetch = [&]() {
+        if constexpr (REC) {
+            table_prefetch(tpf, table4, a.table_floats >> 2);
+            if (MODE == kModeCand) cand_prefetch(cpf, fcount, fcand, a.J, a.cap);
+        }
+    };
     int src;
     if (a.anc_mode == kAncFused) {
         // Consume the previous frame's resampling: a_j, then the frame's
@@ -1214,15 +1278,16 @@ __device__ __forceinline__ void step_body(const StepArgs &ain)
         // pending too) and its largest block exponent (raised by its blocks)
         Pending pd = a.pend;
         if (a.w_cloc) pd.pow_max = &a.fs->pow_max[(t - 1) % 3];
-        src = resolve_ancestor(pd, a.N, a.uni[2 * (t - 1) + 1], j0, wsp, blockIdx.x == 0, es, rs);
+        src = resolve_ancestor(pd, a.N, a.uni[2 * (t - 1) + 1], j0, wsp, blockIdx.x == 0, es, rs, prefetch);
         if (live) a.anc[j] = src;
         if (blockIdx.x == 0 && threadIdx.x == 0)
             finish_resampling(a.fs, a.parity ^ 1, es, rs, a.N, 0, a.lml ? a.lml + (t - 1) : nullptr);
     } else {
+        prefetch();
         src = live ? a.anc[j] : 0;
     }
-    // Gathered state loads, then the model staging; the LDS barrier waits for
-    // both, so they share one memory latency.
+    // Rows: gathered state loads, then the model staging; the LDS barrier
+    // waits for both, so they share one memory latency.
     const unsigned rowb = (unsigned)a.ld * 4u;
     const unsigned voff = (unsigned)src * 4u;
     float x[D], P[NP];
@@ -1233,21 +1298,22 @@ __device__ __forceinline__ void step_body(const StepArgs &ain)
                  (threadIdx.x >> 6) * kRecStageChunks;
     static_assert(!REC || rec_layout(D), "records are compiled for d <= 12");
     if constexpr (REC) {
+        table_store(tpf, sm4, table4, a.table_floats >> 2);
+        if (MODE == kModeCand) cand_store(cpf, reinterpret_cast<float *>(sm4) + a.table_floats, fcand, a.J, a.cap);
         if (a.anc_mode == kAncFused) __syncthreads();
         rec_gather<D>(a.xin, a.N, j0w, src, gw, x, P);
-    } else if (live) {
-        const __amdgpu_buffer_rsrc_t rx = soa_rsrc(a.xin, rowb * D), rP = soa_rsrc(a.Pin, rowb * NP);
+    } else {
+        if (live) {
+            const __amdgpu_buffer_rsrc_t rx = soa_rsrc(a.xin, rowb * D), rP = soa_rsrc(a.Pin, rowb * NP);
 #pragma unroll
-        for (int i = 0; i < D; ++i) x[i] = soa_ld(rx, voff, rowb * i);
+            for (int i = 0; i < D; ++i) x[i] = soa_ld(rx, voff, rowb * i);
 #pragma unroll
-        for (int p = 0; p < NP; ++p) P[p] = soa_ld(rP, voff, rowb * p);
+            for (int p = 0; p < NP; ++p) P[p] = soa_ld(rP, voff, rowb * p);
+        }
+        if (MODE == kModeCand) cand_prefetch(cpf, fcount, fcand, a.J, a.cap);
+        stage_table(sm4, table4, a.table_floats >> 2);
+        if (MODE == kModeCand) cand_store(cpf, reinterpret_cast<float *>(sm4) + a.table_floats, fcand, a.J, a.cap);
     }
-    CandPrefetch cpf;
-    if (MODE == kModeCand) cand_prefetch(cpf, a.count + (size_t)t * a.J, a.cand + (size_t)t * a.J * a.cap * 2, a.J, a.cap);
-    stage_table(sm4, reinterpret_cast<const float4 *>(a.table), a.table_floats >> 2);
-    if (MODE == kModeCand)
-        cand_store(cpf, reinterpret_cast<float *>(sm4) + a.table_floats, a.cand + (size_t)t * a.J * a.cap * 2, a.J,
-                   a.cap);
     __syncthreads();
 
     float lwj = -INFINITY;
