@@ -985,9 +985,9 @@ __device__ __forceinline__ float joint_measure(const StepArgs &a, int t, int j, 
     // coordinates u_c = L^-1 (z_c - y).
     const float lnorm = -logl - LOG2PI;
     float se = 0.f, sv0 = 0.f, sv1 = 0.f, q00 = 0.f, q01 = 0.f, q11 = 0.f;
-    for (int c = 0; c < cnt; ++c) {
+    auto one = [&](float z0, float z1) {
         float u0, u1;
-        wh(cz[2 * c], cz[2 * c + 1], u0, u1);
+        wh(z0, z1, u0, u1);
         const float d2 = fmaf(u0, u0, u1 * u1);
         if (d2 <= a.gate) {
             const float e = a.pdl * __expf(lnorm - 0.5f * d2);
@@ -998,7 +998,19 @@ __device__ __forceinline__ float joint_measure(const StepArgs &a, int t, int j, 
             q01 = fmaf(e * u0, u1, q01);
             q11 = fmaf(e * u1, u1, q11);
         }
+    };
+    // detections in groups of four read before they are used (one LDS
+    // latency per group), in the same order as one at a time
+    const float2 *cz2 = reinterpret_cast<const float2 *>(cz);
+    int c = 0;
+    for (; c + 4 <= cnt; c += 4) {
+        float2 g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) g[u] = cz2[c + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) one(g[u].x, g[u].y);
     }
+    for (; c < cnt; ++c) one(cz[2 * c], cz[2 * c + 1]);
     const float ell = a.b0 + se;
     const float inv = 1.f / ell;
     v0 = sv0 * inv;
