@@ -867,12 +867,24 @@ __device__ __forceinline__ void rec_store(float *rec, int N, int j0w, float4 *gw
 __device__ __forceinline__ int draw_component(const unsigned long long *thr, int K, uint32_t j,
                                               uint32_t U, unsigned long long total)
 {
+    // The first k < K - 1 whose threshold exceeds the target, else K - 1.
+    // The thresholds are cumulative (non-decreasing), so k is the count of
+    // thresholds at or below the target: they are read in groups of eight
+    // (one LDS latency per group), and the wave stops after the first group
+    // whose last threshold exceeds every lane's target.
     const u128 t = sys_target(j, U, total);
     int k = 0;
-    while (k < K - 1) {
-        const u128 th = ((u128)thr[2 * k + 1] << 64) | (u128)thr[2 * k];
-        if (th > t) break;
-        ++k;
+    for (int k0 = 0; k0 < K - 1; k0 += 8) {
+        u128 th[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = min(k0 + u, K - 2);
+            th[u] = ((u128)thr[2 * i + 1] << 64) | (u128)thr[2 * i];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < K - 1 && th[u] <= t) ++k;
+        if (!__any(th[7] <= t)) break;
     }
     return k;
 }
