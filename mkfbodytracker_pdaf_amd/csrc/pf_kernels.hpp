@@ -924,6 +924,29 @@ __device__ __forceinline__ void cand_store(const CandPrefetch &pf, float *dst, c
     if (tid < J) reinterpret_cast<int *>(dst + n)[tid] = pf.c;
 }
 
+// The staged detection count of joint b (candidate mode).
+__device__ __forceinline__ int staged_count(const StepArgs &a, int b, const float *sm)
+{
+    return reinterpret_cast<const int *>(sm + a.table_floats + a.J * a.cap * 2)[b];
+}
+
+// joint_inputs with the joint's staged count n already read.
+template <int MODE>
+__device__ __forceinline__ bool joint_inputs_n(const StepArgs &a, int b, int n, int &cnt, const float *&cz,
+                                               bool &clutter, const float *sm)
+{
+    clutter = (MODE == kModeCand) && ((a.clutter_mask >> b) & 1u);
+    cnt = 1;
+    cz = nullptr;
+    if (MODE == kModeCand) {
+        cnt = n;
+        cz = sm + a.table_floats + b * a.cap * 2;
+        if (cnt <= 0 && !clutter) return false;
+        if (cnt > a.cap) cnt = a.cap;
+    }
+    return true;
+}
+
 // Detections of joint b in frame t (candidate mode, from the LDS staging at
 // sm + table_floats) or the per-particle measurement (reference mode).
 // Returns false if a certain joint has no detection this frame (the joint is
@@ -932,18 +955,8 @@ template <int MODE>
 __device__ __forceinline__ bool joint_inputs(const StepArgs &a, int t, int b, int &cnt, const float *&cz,
                                              bool &clutter, const float *sm)
 {
-    clutter = (MODE == kModeCand) && ((a.clutter_mask >> b) & 1u);
-    cnt = 1;
-    cz = nullptr;
-    if (MODE == kModeCand) {
-        const float *sc = sm + a.table_floats;
-        cnt = reinterpret_cast<const int *>(sc + a.J * a.cap * 2)[b];
-        cz = sc + b * a.cap * 2;
-        if (cnt <= 0 && !clutter) return false;
-        if (cnt > a.cap) cnt = a.cap;
-    }
     (void)t;
-    return true;
+    return joint_inputs_n<MODE>(a, b, MODE == kModeCand ? staged_count(a, b, sm) : 1, cnt, cz, clutter, sm);
 }
 
 // Whitening of an innovation, u = L^-1 (z - y) with S = L L^T, L = [[l00,
@@ -1200,17 +1213,42 @@ __device__ __forceinline__ float particle_update(const StepArgs &a, int t, int j
 
     float acc = 0.f;
     if constexpr (SELJ > 0) {
+        // Candidate mode: every joint's staged detection count and
+        // innovation constants read before the first joint (one LDS
+        // latency); they are the same in every lane and are kept in scalar
+        // registers.  Reference mode reads them at each joint.
+        int jn[SELJ];
+        float jc[SELJ][5];
+        auto joint_consts = [&](int oi, int b) {
+            jn[oi] = MODE == kModeCand ? staged_count(a, b, sm) : 1;
+            jc[oi][0] = BHs[2 * b];
+            jc[oi][1] = BHs[2 * b + 1];
+            jc[oi][2] = Rs[3 * b];
+            jc[oi][3] = Rs[3 * b + 1];
+            jc[oi][4] = Rs[3 * b + 2];
+        };
+        if constexpr (MODE == kModeCand) {
+            auto suni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+#pragma unroll
+            for (int oi = 0; oi < SELJ; ++oi) joint_consts(oi, (int)((a.order_packed >> (4 * oi)) & 15u));
+#pragma unroll
+            for (int oi = 0; oi < SELJ; ++oi) {
+                jn[oi] = __builtin_amdgcn_readfirstlane(jn[oi]);
+#pragma unroll
+                for (int e = 0; e < 5; ++e) jc[oi][e] = suni(jc[oi][e]);
+            }
+        }
 #pragma unroll
         for (int oi = 0; oi < SELJ; ++oi) {
             const int b = (int)((a.order_packed >> (4 * oi)) & 15u);
+            if constexpr (MODE != kModeCand) joint_consts(oi, b);
             int cnt;
             const float *cz;
             bool clutter;
-            if (!joint_inputs<MODE>(a, t, b, cnt, cz, clutter, sm)) continue;
+            if (!joint_inputs_n<MODE>(a, b, jn[oi], cnt, cz, clutter, sm)) continue;
             float w0[D], w1[D], y0, y1, il00, l10, il11, logl;
-            const bool ok = Joint<D>::prepare_sel(x, P, 2 * oi, BHs[2 * b], BHs[2 * b + 1], Rs[3 * b],
-                                                  Rs[3 * b + 1], Rs[3 * b + 2], w0, w1, y0, y1, il00, l10, il11,
-                                                  logl);
+            const bool ok = Joint<D>::prepare_sel(x, P, 2 * oi, jc[oi][0], jc[oi][1], jc[oi][2], jc[oi][3],
+                                                  jc[oi][4], w0, w1, y0, y1, il00, l10, il11, logl);
             // States up to d = 16 apply the update inside each branch: on the
             // FP32 path the compiler may then start it before the likelihood
             // is complete (a certain joint's M is the identity), which keeps W
