@@ -370,6 +370,32 @@ __device__ int resolve_search(const Pending &pd, int N, uint32_t U, unsigned lon
     const u128 S = (u128)(unsigned long long)N << 32;
     const u128 tj = sys_target((uint32_t)jc, U, total);
     int lo = 0, hi = nb - 1;
+    if ((nt & 63) == 0) {
+        // t_j grows with j, so every output's block lies between the blocks
+        // of the block's first and last outputs.  Each wave finds those two
+        // by 64-way searches: lane l probes position lo + l * step of the
+        // range (non-decreasing in l, lane 0 at a position known to pass),
+        // and the ballot's count of passing lanes narrows the range to one
+        // step (two levels up to 4096 input blocks, one LDS read per lane
+        // each).  The output's own search then covers that span only.
+        const int lane = tid & 63;
+        const int jl = min(j0 + nt, N) - 1;
+        const u128 tf = sys_target((uint32_t)j0, U, total), tl = sys_target((uint32_t)jl, U, total);
+        int flo = 0, fhi = nb - 1, llo = 0, lhi = nb - 1;
+        while (flo < fhi || llo < lhi) {
+            const int fs = (fhi - flo + 64) >> 6, ls = (lhi - llo + 64) >> 6;
+            const int fp = min(flo + lane * fs, fhi), lp = min(llo + lane * ls, lhi);
+            const int fc = __popcll(__ballot((u128)pre[fp] * S <= tf));
+            const int lc = __popcll(__ballot((u128)pre[lp] * S <= tl));
+            const int fn = min(flo + (fc - 1) * fs, fhi), ln = min(llo + (lc - 1) * ls, lhi);
+            fhi = fc < 64 ? min(flo + fc * fs, fhi) - 1 : fhi;
+            lhi = lc < 64 ? min(llo + lc * ls, lhi) - 1 : lhi;
+            flo = fn;
+            llo = ln;
+        }
+        lo = flo;
+        hi = llo;
+    }
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if ((u128)pre[mid] * S <= tj) lo = mid;
