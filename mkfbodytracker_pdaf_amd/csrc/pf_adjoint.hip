@@ -68,18 +68,6 @@ struct AdjArgs {
     int kc;                            // components per group of the deterministic term reduction
 };
 
-// Sum over the 16 lanes of each row of the wave, in every lane of the row,
-// by four DPP steps (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
-// row_mirror): VALU only, no LDS round trips.
-__device__ __forceinline__ float row_sum16(float v)
-{
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, true));
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, true));
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, true));
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, true));
-    return v;
-}
-
 // LDS floats of the term reduction's wave slots for groups of kc components
 // (nwaves x kc x kGV values, then nwaves 64-bit masks).
 __host__ __device__ constexpr int term_slot_floats(int kc, int nwaves)

@@ -45,11 +45,78 @@ __device__ __forceinline__ double key_value(unsigned long long k)
     return __longlong_as_double((long long)b);
 }
 
+// Cross-lane primitives without LDS (every lane of the wave must be active).
+// xor_lane<O>(v): the value of lane l ^ O, O = 1, 2, 4, 8, 16, 32: quad
+// permutations, row rotation by 8 and row_half_mirror + quad reversal (DPP),
+// and the row-pair / lane-half swaps of v_permlane16_swap and
+// v_permlane32_swap.
+template <int O>
+__device__ __forceinline__ uint32_t xor_lane_u32(uint32_t v)
+{
+    static_assert(O == 1 || O == 2 || O == 4 || O == 8 || O == 16 || O == 32, "xor distance");
+    const int x = (int)v;
+    if constexpr (O == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+    if constexpr (O == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
+    if constexpr (O == 4) {
+        const int y = __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);     // l -> 7 - l in each 8
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, y, 0x1B, 0xF, 0xF, false);  // then reversed in each 4
+    }
+    if constexpr (O == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);   // row_ror:8
+    const int lane = (int)(threadIdx.x & 63);
+    if constexpr (O == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane & 16) ? r[0] : r[1];
+    }
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32) ? r[0] : r[1];
+}
+
+template <int O, typename T>
+__device__ __forceinline__ T xor_lane(T v)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, xor_lane_u32<O>(__builtin_bit_cast(uint32_t, v)));
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = xor_lane_u32<O>((uint32_t)u), hi = xor_lane_u32<O>((uint32_t)(u >> 32));
+        return __builtin_bit_cast(T, (hi << 32) | lo);
+    }
+}
+
+// x shifted by DPP control CTRL on the rows in ROWS (other rows and
+// out-of-row sources give 0).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, ROWS, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), CTRL, ROWS, 0xF, true);
+    return ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Inclusive prefix sum over the wave (lane l: lanes 0 .. l): row_shr 1, 2,
+// 4, 8 inside each row of 16, then row_bcast:15 and row_bcast:31 carry the
+// rows' totals forward (integers: the same sums as any other order).
+__device__ __forceinline__ unsigned long long wave_incl_scan(unsigned long long v)
+{
+    v += dpp_u64<0x111, 0xF>(v);
+    v += dpp_u64<0x112, 0xF>(v);
+    v += dpp_u64<0x114, 0xF>(v);
+    v += dpp_u64<0x118, 0xF>(v);
+    v += dpp_u64<0x142, 0xA>(v);
+    v += dpp_u64<0x143, 0xC>(v);
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_max(T v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    v = fmax(v, xor_lane<32>(v));
+    v = fmax(v, xor_lane<16>(v));
+    v = fmax(v, xor_lane<8>(v));
+    v = fmax(v, xor_lane<4>(v));
+    v = fmax(v, xor_lane<2>(v));
+    v = fmax(v, xor_lane<1>(v));
     return v;
 }
 
@@ -200,8 +267,12 @@ static __global__ void k_run_begin(RunDesc *d, RunDesc v, int *ctr, FrameScratch
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    v += xor_lane<32>(v);      // the butterfly order xor 32, 16, ..., 1
+    v += xor_lane<16>(v);
+    v += xor_lane<8>(v);
+    v += xor_lane<4>(v);
+    v += xor_lane<2>(v);
+    v += xor_lane<1>(v);
     return v;
 }
 
@@ -275,8 +346,12 @@ __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, 
 #pragma unroll
         for (int u = 0; u < kReg; ++u) em = max(em, re[u]);
         for (int k = c0 + kReg; k < c1; ++k) em = max(em, pd.bpow[k]);
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) em = max(em, __shfl_xor(em, o, 64));
+        em = max(em, xor_lane<32>(em));
+        em = max(em, xor_lane<16>(em));
+        em = max(em, xor_lane<8>(em));
+        em = max(em, xor_lane<4>(em));
+        em = max(em, xor_lane<2>(em));
+        em = max(em, xor_lane<1>(em));
         __syncthreads();                 // LDS of a previous use is free
         if (lane == 0) misc[8 + wid] = (unsigned long long)(long long)em;
         __syncthreads();
@@ -291,11 +366,7 @@ __device__ __forceinline__ unsigned long long resolve_prefix(const Pending &pd, 
         if (c0 + u < c1) loc += shr64(rs[u], shift_of(re[u]));
     for (int k = c0 + kReg; k < c1; ++k) loc += shr64(pd.bsum[k], shift_of(pd.bpow ? pd.bpow[k] : 0));
     unsigned long long sc = loc;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long n = __shfl_up(sc, o, 64);
-        if (lane >= o) sc += n;
-    }
+    sc = wave_incl_scan(sc);
     // FP64 weight sum in an order fixed by nb alone (lane l of wave 0 takes
     // blocks l, l + 64, ...), so it does not depend on the block size; block
     // b's sum is relative to 2^e_b and scaled to 2^E (exactly).
@@ -509,11 +580,7 @@ __device__ __forceinline__ int block_weights_log(double x, bool valid, unsigned 
     const double v = (fin && e != kPowEmpty) ? det_exp(x - pow_ref(e)) : 0.0;
     const unsigned long long q = fixed32(v);
     unsigned long long sc = q;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long n = __shfl_up(sc, o, 64);
-        if (lane >= o) sc += n;
-    }
+    sc = wave_incl_scan(sc);
     const double vs = wave_sum(v);
     if (lane == 63) s_wave[wid] = sc;
     if (lane == 0) s_dwave[wid] = vs;
@@ -1511,11 +1578,7 @@ __device__ __forceinline__ void weights_body(const WeightArgs &a)
         }
         const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         unsigned long long s = q;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned long long n = __shfl_up(s, o, 64);
-            if (lane >= o) s += n;
-        }
+        s = wave_incl_scan(s);
         const double vs = wave_sum(v);
         if (lane == 63) s_wave[wid] = s;
         if (lane == 0) s_dwave[wid] = vs;

@@ -318,15 +318,7 @@ __global__ void __launch_bounds__(kProposeBlock) k_propose(PropParams p, ArmDev 
         }
         // block-local inclusive scan over the chunk: pairs of the first half
         // (lane order), then those of the second half
-        unsigned long long s1 = q[0] + q[1], s2 = q[2] + q[3];
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned long long n1 = __shfl_up(s1, o, 64), n2 = __shfl_up(s2, o, 64);
-            if (lane >= o) {
-                s1 += n1;
-                s2 += n2;
-            }
-        }
+        unsigned long long s1 = wave_incl_scan(q[0] + q[1]), s2 = wave_incl_scan(q[2] + q[3]);
         if (lane == 63) {
             s_wave[0][wid] = s1;
             s_wave[1][wid] = s2;
