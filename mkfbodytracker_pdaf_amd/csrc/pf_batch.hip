@@ -612,12 +612,20 @@ int run_all(pf_batch *b, const int32_t *count, const float *cand, const uint32_t
         if (record) {
             a.rec_comp = b->rec_comp;
             a.rec_anc = b->rec_anc;
-            if (f % b->ckpt_every == 0) {          // the input state of frame f
-                const size_t c = (size_t)(f / b->ckpt_every);
-                HIP_TRYB(hipMemcpyAsync(b->ck_x + c * xw, b->bx[f & 1], sizeof(float) * xw, hipMemcpyDeviceToDevice,
-                                        b->stream));
-                HIP_TRYB(hipMemcpyAsync(b->ck_P + c * Pw, b->bP[f & 1], sizeof(float) * Pw, hipMemcpyDeviceToDevice,
-                                        b->stream));
+            // Checkpoint c holds the input state of frame c C: the reset state
+            // is copied there; for c >= 1 frame c C - 1 writes its updated
+            // states into the checkpoint and frame c C reads them from it.
+            const int C = b->ckpt_every;
+            if (f == 0) {
+                HIP_TRYB(hipMemcpyAsync(b->ck_x, b->bx[0], sizeof(float) * xw, hipMemcpyDeviceToDevice, b->stream));
+                HIP_TRYB(hipMemcpyAsync(b->ck_P, b->bP[0], sizeof(float) * Pw, hipMemcpyDeviceToDevice, b->stream));
+            } else if (f % C == 0) {
+                a.buf_x[f & 1] = b->ck_x + (size_t)(f / C) * xw;
+                a.buf_P[f & 1] = b->ck_P + (size_t)(f / C) * Pw;
+            }
+            if ((f + 1) % C == 0 && f + 1 < frames) {
+                a.buf_x[(f + 1) & 1] = b->ck_x + (size_t)((f + 1) / C) * xw;
+                a.buf_P[(f + 1) & 1] = b->ck_P + (size_t)((f + 1) / C) * Pw;
             }
             // The frames of the last checkpoint segment write their updated
             // states into that segment's slots (slot f - t0 for frames t0 ..
