@@ -575,6 +575,7 @@ def run_tracker(args):
         dev_ms = tr.last_run_ms()
         n_c = tr.n_c
         tr.close()
+    calib_copy(args.calib_copy_mb)
     evals = 2.0 * N * T
     bpp = tracker_bytes_per_particle()
     achieved = bpp * evals / (dev_ms * 1e-3) / 1e9
